@@ -1,0 +1,492 @@
+// C ABI of libdrpt_hip: context, workspace, parameter derivation and the
+// orchestration of the four stages (projection, splits, leaf k-NN, merge).
+//
+// The reference drives the same sequence from MDRPT::grow_trees
+// (cpp/src/drpt/algo/mdrpt.cpp:46-121) and MDRPT::gather_nns (:273-291) on
+// every MPI rank; here one context drives one GPU and one stream.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "internal.hpp"
+
+namespace {
+thread_local std::string g_error;
+}
+
+namespace drpt {
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+int ws_get(drpt_ctx* ctx, const char* name, size_t bytes, void** out) {
+  auto it = ctx->bufs.find(name);
+  if (it != ctx->bufs.end() && it->second.second >= bytes) {
+    *out = it->second.first;
+    return DRPT_OK;
+  }
+  if (it != ctx->bufs.end()) {
+    DRPT_HIP(hipStreamSynchronize(ctx->stream));
+    DRPT_HIP(hipFree(it->second.first));
+    ctx->bufs.erase(it);
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    set_error(std::string("device allocation of ") + std::to_string(bytes) + " bytes for '" + name +
+              "' failed: " + hipGetErrorString(e));
+    return DRPT_ERR_NOMEM;
+  }
+  ctx->bufs[name] = {p, bytes};
+  *out = p;
+  return DRPT_OK;
+}
+
+namespace {
+
+template <typename T>
+int upload(drpt_ctx* ctx, const char* name, const std::vector<T>& host, T** dev) {
+  DRPT_CHECK(ws(ctx, name, std::max<size_t>(host.size(), 1), dev));
+  if (!host.empty())
+    DRPT_HIP(hipMemcpyAsync(*dev, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice,
+                            ctx->stream));
+  return DRPT_OK;
+}
+
+int validate(const drpt_params* p) {
+  if (!p) {
+    set_error("params is null");
+    return DRPT_ERR_ARG;
+  }
+  if (p->n < 2 || p->n > INT32_MAX) {
+    set_error("n must be in [2, 2^31-1]");
+    return DRPT_ERR_ARG;
+  }
+  if (p->d < 1 || p->d > 65535) {
+    set_error("dimension must be in [1, 65535]");
+    return DRPT_ERR_ARG;
+  }
+  if (p->dtype != DRPT_U8 && p->dtype != DRPT_F32) {
+    set_error("dtype must be DRPT_U8 or DRPT_F32");
+    return DRPT_ERR_ARG;
+  }
+  if (p->ntrees < 1) {
+    set_error("no of trees should be greater than zero");  // drpt_global.cpp:162
+    return DRPT_ERR_DEPTH;
+  }
+  if (p->depth <= 0 || p->depth > std::log2((double)p->n) + (p->local_mode == DRPT_LOCAL_LEAFMAX ? 64 : 0)) {
+    set_error("depth should be in range [1,....,log2(rows)]");  // drpt_global.cpp:155-158
+    return DRPT_ERR_DEPTH;
+  }
+  if (p->depth > 20) {
+    set_error("depth > 20 is not supported");
+    return DRPT_ERR_UNSUPPORTED;
+  }
+  if (p->nn < 1 || p->nn > 64) {
+    set_error("nn must be in [1, 64]");
+    return DRPT_ERR_ARG;
+  }
+  if (p->local_mode == DRPT_LOCAL_LEAFMAX && p->leaf_max < 1) {
+    set_error("leaf_max must be >= 1 in leaf-max mode");
+    return DRPT_ERR_ARG;
+  }
+  if (p->precision != DRPT_FP32 && p->precision != DRPT_BF16) {
+    set_error("precision not supported by this build");
+    return DRPT_ERR_UNSUPPORTED;
+  }
+  return DRPT_OK;
+}
+
+int kprime_of(const drpt_params* p) { return p->kprime > 0 ? p->kprime : 2 * p->nn + 4; }
+
+float ms_between(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.0f;
+  return ms;
+}
+
+struct Leaf {
+  int32_t tree;
+  int32_t size;
+  int64_t start;
+};
+
+}  // namespace
+
+}  // namespace drpt
+
+using namespace drpt;
+
+extern "C" {
+
+const char* drpt_last_error(void) { return g_error.c_str(); }
+
+int drpt_version(void) { return 100; }
+
+int drpt_create(drpt_ctx** out, int device) {
+  if (!out) {
+    set_error("ctx out pointer is null");
+    return DRPT_ERR_ARG;
+  }
+  *out = nullptr;
+  int count = 0;
+  DRPT_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) {
+    set_error("device index out of range (" + std::to_string(count) + " devices)");
+    return DRPT_ERR_ARG;
+  }
+  DRPT_HIP(hipSetDevice(device));
+  drpt_ctx* ctx = new drpt_ctx();
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  if (hipStreamCreate(&ctx->stream) != hipSuccess) {
+    delete ctx;
+    set_error("hipStreamCreate failed");
+    return DRPT_ERR_HIP;
+  }
+  for (auto& e : ctx->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      set_error("hipEventCreate failed");
+      return DRPT_ERR_HIP;
+    }
+  }
+  *out = ctx;
+  return DRPT_OK;
+}
+
+int drpt_destroy(drpt_ctx* ctx) {
+  if (!ctx) return DRPT_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->bufs) (void)hipFree(kv.second.first);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return DRPT_OK;
+}
+
+int drpt_sync(drpt_ctx* ctx) {
+  if (!ctx) {
+    set_error("ctx is null");
+    return DRPT_ERR_ARG;
+  }
+  DRPT_HIP(hipStreamSynchronize(ctx->stream));
+  return DRPT_OK;
+}
+
+void* drpt_get_stream(drpt_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int drpt_derive_params(int64_t n, int d, int ntrees, int nn, double tree_depth_ratio,
+                       drpt_params* out) {
+  if (!out || n < 2 || d < 1 || ntrees < 1 || nn < 1) {
+    set_error("derive_params: invalid arguments");
+    return DRPT_ERR_ARG;
+  }
+  std::memset(out, 0, sizeof(*out));
+  out->n = n;
+  out->d = d;
+  out->dtype = DRPT_F32;
+  out->ntrees = ntrees;
+  const int tree_depth = (int)(std::log2((double)n) - 2);           // main.cpp:141-145
+  out->depth = (int)(tree_depth * tree_depth_ratio);                 // mdrpt.cpp:52
+  out->nn = nn;
+  out->local_mode = DRPT_LOCAL_REFERENCE;
+  out->leaf_max = 0;
+  out->local_offset = 2;                                             // main.cpp:37
+  out->precision = DRPT_FP32;
+  out->kprime = 0;
+  out->seed = 1;
+  out->density = (float)(1.0 / std::sqrt((double)d));                // main.cpp:129-132
+  return DRPT_OK;
+}
+
+int drpt_build_projection(int d, int L, float density, uint32_t seed, float* R_host) {
+  if (d < 1 || L < 1 || !R_host) {
+    set_error("build_projection: invalid arguments");
+    return DRPT_ERR_ARG;
+  }
+  host_build_projection(d, L, density, seed, R_host);
+  return DRPT_OK;
+}
+
+int drpt_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, const float* R_host,
+                 int L, float* P, uint32_t* colmax) {
+  if (!ctx || !X || !R_host || !P || !colmax || n < 1 || d < 1 || d > 65535 || L < 1) {
+    set_error("project: invalid arguments");
+    return DRPT_ERR_ARG;
+  }
+  DRPT_HIP(hipSetDevice(ctx->device));
+  std::vector<int32_t> colptr(L + 1, 0);
+  std::vector<uint16_t> rowidx;
+  std::vector<float> w;
+  for (int c = 0; c < L; ++c) {
+    colptr[c] = (int32_t)rowidx.size();
+    for (int j = 0; j < d; ++j) {
+      const float v = R_host[(size_t)j * L + c];
+      if (v != 0.0f) {
+        rowidx.push_back((uint16_t)j);
+        w.push_back(v);
+      }
+    }
+  }
+  colptr[L] = (int32_t)rowidx.size();
+  int32_t* dcp;
+  uint16_t* dri;
+  float* dw;
+  DRPT_CHECK(upload(ctx, "proj.colptr", colptr, &dcp));
+  DRPT_CHECK(upload(ctx, "proj.rowidx", rowidx, &dri));
+  DRPT_CHECK(upload(ctx, "proj.w", w, &dw));
+  return launch_project(ctx, X, dtype, n, d, dcp, dri, dw, L, P, colmax);
+}
+
+int drpt_grow_trees(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t n, int ntrees,
+                    int depth, int tree_mode, int leaf_max, uint32_t* leaf_heap) {
+  if (!ctx || !P || !colmax || !leaf_heap) {
+    set_error("grow_trees: null pointer");
+    return DRPT_ERR_ARG;
+  }
+  if (depth <= 0 || (tree_mode == DRPT_TREE_FIXED && depth > std::log2((double)n))) {
+    set_error("depth should be in range [1,....,log2(rows)]");
+    return DRPT_ERR_DEPTH;
+  }
+  DRPT_HIP(hipSetDevice(ctx->device));
+  return grow_trees_impl(ctx, P, colmax, n, ntrees, depth, tree_mode, leaf_max, leaf_heap);
+}
+
+int drpt_tree_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const uint32_t* leaf_heap,
+                  int32_t* cand_idx, float* cand_dist, drpt_timings* timings) {
+  if (!ctx || !X || !leaf_heap || !cand_idx || !cand_dist) {
+    set_error("tree_knn: null pointer");
+    return DRPT_ERR_ARG;
+  }
+  DRPT_CHECK(validate(p));
+  DRPT_HIP(hipSetDevice(ctx->device));
+  const int T = p->ntrees, D = p->depth, K = p->nn;
+  const int64_t n = p->n;
+  const int64_t HS = (int64_t)1 << D;
+  hipStream_t s = ctx->stream;
+  DRPT_HIP(hipEventRecord(ctx->ev[6], s));
+
+  // leaf sizes by heap index
+  uint32_t* hcount;
+  DRPT_CHECK(ws(ctx, "group.hcount", T * HS, &hcount));
+  DRPT_CHECK(launch_count_leaves(ctx, leaf_heap, n, T, D, hcount));
+  std::vector<uint32_t> hc((size_t)(T * HS));
+  DRPT_HIP(hipMemcpyAsync(hc.data(), hcount, hc.size() * 4, hipMemcpyDeviceToHost, s));
+  DRPT_HIP(hipStreamSynchronize(s));
+  std::vector<int64_t> hoff((size_t)(T * HS));
+  std::vector<Leaf> leaves;
+  for (int t = 0; t < T; ++t) {
+    int64_t acc = 0;
+    for (int64_t h = 0; h < HS; ++h) {
+      const uint32_t c = hc[(size_t)(t * HS + h)];
+      hoff[(size_t)(t * HS + h)] = acc;
+      if (c) leaves.push_back({t, (int32_t)c, acc});
+      acc += c;
+    }
+  }
+  int64_t* dhoff;
+  uint32_t* cursor;
+  int32_t* perm;
+  DRPT_CHECK(upload(ctx, "group.hoff", hoff, &dhoff));
+  DRPT_CHECK(ws(ctx, "group.cursor", T * HS, &cursor));
+  DRPT_CHECK(ws(ctx, "group.perm", T * n, &perm));
+  DRPT_CHECK(launch_scatter_leaves(ctx, leaf_heap, n, T, D, dhoff, cursor, perm));
+  const int32_t* perm_final = perm;
+
+  std::vector<Leaf> final_leaves;
+  if (p->local_mode == DRPT_LOCAL_REFERENCE) {
+    // MDRPT::get_global_minimum_leaf_size (mdrpt.cpp:293-319), grow_local_trees (:321-400)
+    int64_t gmin = INT64_MAX;
+    int max_seg = 0;
+    for (auto& l : leaves) {
+      gmin = std::min<int64_t>(gmin, l.size);
+      max_seg = std::max(max_seg, l.size);
+    }
+    const int ld = host_local_depth(gmin, K, p->local_offset);
+    if (ld <= 0) {
+      set_error("local tree depth " + std::to_string(ld) + " (smallest global leaf " +
+                std::to_string(gmin) + "): depth should be in range [1,....,log2(rows)]");
+      return DRPT_ERR_DEPTH;  // drpt_local.cpp:84-87
+    }
+    const float dint = (float)(int)p->density;
+    std::vector<float> C((size_t)p->d * ld);
+    for (int t = 0; t < T; ++t) {
+      host_build_projection(p->d, ld, dint, p->seed + 1u + (uint32_t)t, C.data());
+      for (float c : C)
+        if (c != 0.0f) {
+          set_error("reference local stage with a non-zero local projection is not supported");
+          return DRPT_ERR_UNSUPPORTED;
+        }
+    }
+    std::vector<int64_t> seg_start;
+    std::vector<int32_t> seg_size, seg_tree;
+    std::vector<int64_t> seg_tbl;
+    std::vector<int32_t> tables;
+    std::map<int, int64_t> tbl_of;
+    std::vector<int32_t> sizes((size_t)1 << ld);
+    for (auto& l : leaves) {
+      seg_start.push_back(l.start);
+      seg_size.push_back(l.size);
+      seg_tree.push_back(l.tree);
+      auto it = tbl_of.find(l.size);
+      if (it == tbl_of.end()) {
+        const int64_t off = (int64_t)tables.size();
+        tables.resize(tables.size() + l.size);
+        host_local_positions(l.size, ld, tables.data() + off);
+        it = tbl_of.emplace(l.size, off).first;
+      }
+      seg_tbl.push_back(it->second);
+      host_local_sizes(l.size, ld, sizes.data());
+      int64_t acc = l.start;
+      for (int32_t sz : sizes) {
+        if (sz > 0) final_leaves.push_back({l.tree, sz, acc});
+        acc += sz;
+      }
+    }
+    int64_t* dss;
+    int32_t *dsz, *dst, *dtb;
+    int64_t* dtbo;
+    DRPT_CHECK(upload(ctx, "local.start", seg_start, &dss));
+    DRPT_CHECK(upload(ctx, "local.size", seg_size, &dsz));
+    DRPT_CHECK(upload(ctx, "local.tree", seg_tree, &dst));
+    DRPT_CHECK(upload(ctx, "local.tblofs", seg_tbl, &dtbo));
+    DRPT_CHECK(upload(ctx, "local.tables", tables, &dtb));
+    DRPT_CHECK(launch_sort_segments(ctx, perm, n, dss, dsz, dst, (int)leaves.size(), max_seg));
+    int32_t* perm2;
+    DRPT_CHECK(ws(ctx, "group.perm2", T * n, &perm2));
+    DRPT_CHECK(launch_apply_local(ctx, perm, perm2, n, dss, dst, dtbo, dsz, (int)leaves.size(), dtb));
+    perm_final = perm2;
+  } else {
+    final_leaves = leaves;
+  }
+
+  // work items, largest leaves first
+  std::stable_sort(final_leaves.begin(), final_leaves.end(),
+                   [](const Leaf& a, const Leaf& b) { return a.size > b.size; });
+  std::vector<LeafItem> items;
+  double gram = 0.0;
+  int64_t max_leaf = 0;
+  for (auto& l : final_leaves) {
+    gram += 2.0 * p->d * (double)l.size * (double)l.size;
+    max_leaf = std::max<int64_t>(max_leaf, l.size);
+    for (int r0 = 0; r0 < l.size; r0 += 64) items.push_back({l.tree, l.size, l.start, r0, 0});
+  }
+  LeafItem* ditems;
+  DRPT_CHECK(upload(ctx, "knn.items", items, &ditems));
+
+  const void* Xq = nullptr;
+  float *nb = nullptr, *er = nullptr;
+  drpt_params pp = *p;
+  pp.kprime = kprime_of(p);
+  if (pp.precision != DRPT_FP32) {
+    if (pp.kprime < K) {
+      set_error("kprime must be >= nn");
+      return DRPT_ERR_ARG;
+    }
+    uint16_t* xb;
+    DRPT_CHECK(ws(ctx, "lowp.x", n * p->d, &xb));
+    DRPT_CHECK(ws(ctx, "lowp.nrm2", n, &nb));
+    DRPT_CHECK(ws(ctx, "lowp.err", n, &er));
+    DRPT_CHECK(launch_prep_lowp(ctx, X, p->dtype, n, p->d, pp.precision, xb, nb, er, nullptr));
+    Xq = xb;
+  }
+  int32_t *fb, *nfb;
+  const int64_t fb_cap = std::min<int64_t>((int64_t)T * n, (int64_t)1 << 22);
+  DRPT_CHECK(ws(ctx, "knn.fallback", 2 * fb_cap, &fb));
+  DRPT_CHECK(ws(ctx, "knn.nfallback", 1, &nfb));
+  DRPT_HIP(hipEventRecord(ctx->ev[7], s));
+  DRPT_HIP(hipEventRecord(ctx->ev[9], s));
+  DRPT_CHECK(launch_leaf_knn(ctx, &pp, X, perm_final, ditems, (int64_t)items.size(), Xq, nb, er,
+                             nullptr, cand_idx, cand_dist, fb, nfb, fb_cap));
+  DRPT_HIP(hipEventRecord(ctx->ev[8], s));
+  int32_t nfallback = 0;
+  if (pp.precision != DRPT_FP32) {
+    DRPT_CHECK(launch_fallback(ctx, &pp, X, perm_final, ditems, fb, nfb, fb_cap, cand_idx, cand_dist));
+    DRPT_HIP(hipMemcpyAsync(&nfallback, nfb, 4, hipMemcpyDeviceToHost, s));
+  }
+  DRPT_HIP(hipEventRecord(ctx->ev[5], s));
+  DRPT_HIP(hipStreamSynchronize(s));
+  if (nfallback > fb_cap) {
+    set_error("too many uncertified rows for the fallback list");
+    return DRPT_ERR_UNSUPPORTED;
+  }
+  if (timings) {
+    timings->group_ms = ms_between(ctx->ev[6], ctx->ev[7]);
+    timings->knn_ms = ms_between(ctx->ev[7], ctx->ev[5]);
+    timings->knn_kernel_ms = ms_between(ctx->ev[9], ctx->ev[8]);
+    timings->gram_flops = gram;
+    timings->n_leaves = (int64_t)final_leaves.size();
+    timings->n_fallback_rows = nfallback;
+    timings->max_leaf = max_leaf;
+  }
+  return DRPT_OK;
+}
+
+int drpt_merge(drpt_ctx* ctx, int64_t n, int T, int K, const int32_t* cand_idx,
+               const float* cand_dist, int32_t* out_idx, float* out_dist) {
+  if (!ctx || !cand_idx || !cand_dist || !out_idx || !out_dist || n < 1 || T < 1 || K < 1) {
+    set_error("merge: invalid arguments");
+    return DRPT_ERR_ARG;
+  }
+  DRPT_HIP(hipSetDevice(ctx->device));
+  return launch_merge(ctx, n, T, K, cand_idx, cand_dist, out_idx, out_dist);
+}
+
+int drpt_build_graph(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* out_idx,
+                     float* out_dist, drpt_timings* timings) {
+  if (!ctx || !X || !out_idx || !out_dist) {
+    set_error("build_graph: null pointer");
+    return DRPT_ERR_ARG;
+  }
+  DRPT_CHECK(validate(p));
+  DRPT_HIP(hipSetDevice(ctx->device));
+  const int T = p->ntrees, D = p->depth, K = p->nn, L = T * D;
+  const int64_t n = p->n;
+  hipStream_t s = ctx->stream;
+  std::vector<float> R((size_t)p->d * L);
+  host_build_projection(p->d, L, p->density, p->seed, R.data());
+  float* P;
+  uint32_t *colmax, *heap;
+  int32_t* ci;
+  float* cd;
+  DRPT_CHECK(ws(ctx, "build.P", (size_t)L * n, &P));
+  DRPT_CHECK(ws(ctx, "build.colmax", L, &colmax));
+  DRPT_CHECK(ws(ctx, "build.heap", (size_t)T * n, &heap));
+  DRPT_CHECK(ws(ctx, "build.cand_idx", (size_t)T * n * K, &ci));
+  DRPT_CHECK(ws(ctx, "build.cand_dist", (size_t)T * n * K, &cd));
+  DRPT_HIP(hipEventRecord(ctx->ev[0], s));
+  DRPT_CHECK(drpt_project(ctx, X, p->dtype, n, p->d, R.data(), L, P, colmax));
+  DRPT_HIP(hipEventRecord(ctx->ev[1], s));
+  // a non-finite projection cannot be split (the fixed-point statistics need finite values)
+  std::vector<uint32_t> cm(L);
+  DRPT_HIP(hipMemcpyAsync(cm.data(), colmax, 4 * L, hipMemcpyDeviceToHost, s));
+  DRPT_HIP(hipStreamSynchronize(s));
+  for (uint32_t b : cm)
+    if (b >= 0x7f800000u) {
+      set_error("non-finite projected values (input contains inf/nan or overflows float)");
+      return DRPT_ERR_NONFINITE;
+    }
+  const int mode = p->local_mode == DRPT_LOCAL_REFERENCE ? DRPT_TREE_FIXED : DRPT_TREE_LEAFMAX;
+  DRPT_CHECK(grow_trees_impl(ctx, P, colmax, n, T, D, mode, p->leaf_max, heap));
+  DRPT_HIP(hipEventRecord(ctx->ev[2], s));
+  drpt_timings tk{};
+  DRPT_CHECK(drpt_tree_knn(ctx, p, X, heap, ci, cd, &tk));
+  DRPT_HIP(hipEventRecord(ctx->ev[3], s));
+  DRPT_CHECK(launch_merge(ctx, n, T, K, ci, cd, out_idx, out_dist));
+  DRPT_HIP(hipEventRecord(ctx->ev[4], s));
+  DRPT_HIP(hipStreamSynchronize(s));
+  if (timings) {
+    *timings = tk;
+    timings->project_ms = ms_between(ctx->ev[0], ctx->ev[1]);
+    timings->split_ms = ms_between(ctx->ev[1], ctx->ev[2]);
+    timings->merge_ms = ms_between(ctx->ev[3], ctx->ev[4]);
+    timings->total_ms = ms_between(ctx->ev[0], ctx->ev[4]);
+  }
+  return DRPT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+// Internal declarations shared by the HIP translation units of libdrpt_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/drpt_hip.h"
+
+namespace drpt {
+
+void set_error(const std::string& msg);
+
+#define DRPT_HIP(x)                                                             \
+  do {                                                                          \
+    hipError_t e_ = (x);                                                        \
+    if (e_ != hipSuccess) {                                                     \
+      drpt::set_error(std::string(#x) + " failed: " + hipGetErrorString(e_) +   \
+                      " (" + __FILE__ + ":" + std::to_string(__LINE__) + ")");  \
+      return DRPT_ERR_HIP;                                                      \
+    }                                                                           \
+  } while (0)
+
+#define DRPT_CHECK(x)          \
+  do {                         \
+    int rc_ = (x);             \
+    if (rc_ != DRPT_OK) return rc_; \
+  } while (0)
+
+}  // namespace drpt
+
+// Device workspace: named buffers, grown on demand and kept for the context's
+// lifetime so repeated builds of one shape allocate nothing.
+struct drpt_ctx {
+  int device = 0;
+  int num_cus = 256;
+  hipStream_t stream = nullptr;
+  std::map<std::string, std::pair<void*, size_t>> bufs;
+  hipEvent_t ev[12] = {};
+};
+
+namespace drpt {
+
+int ws_get(drpt_ctx* ctx, const char* name, size_t bytes, void** out);
+template <typename T>
+int ws(drpt_ctx* ctx, const char* name, size_t count, T** out) {
+  void* p = nullptr;
+  int rc = ws_get(ctx, name, count * sizeof(T) + 16, &p);
+  *out = (T*)p;
+  return rc;
+}
+
+// ---- host-side reference pieces (host_ref.cpp, built with g++) -------------
+void host_build_projection(int d, int L, float density, uint32_t seed, float* R);
+// positions of a reference local tree with all-equal keys (drpt_local.cpp:121-161)
+void host_local_positions(int m, int ld, int32_t* out);
+void host_local_sizes(int m, int ld, int32_t* out);
+int host_local_depth(int64_t gmin, int nn, int offset);
+
+// ---- launchers (one per .hip unit) ------------------------------------------
+// util.hip: capture-safe fills (a kernel, not hipMemsetAsync)
+int fill_u32(drpt_ctx* ctx, uint32_t* p, uint32_t v, size_t count);
+
+// project.hip
+int launch_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, const int32_t* colptr,
+                   const uint16_t* rowidx, const float* w, int L, float* P, uint32_t* colmax);
+
+// split.hip
+int grow_trees_impl(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t n, int T, int D,
+                    int mode, int leaf_max, uint32_t* node);
+
+// group.hip / leafknn.hip
+struct LeafItem {   // one work item: a block of up to 64 rows of one leaf
+  int32_t tree;
+  int32_t size;     // leaf size m
+  int64_t start;    // offset of the leaf in perm[tree]
+  int32_t row0;     // first row of this block within the leaf
+  int32_t pad;
+};
+
+int launch_count_leaves(drpt_ctx* ctx, const uint32_t* heap, int64_t n, int T, int D, uint32_t* hcount);
+int launch_scatter_leaves(drpt_ctx* ctx, const uint32_t* heap, int64_t n, int T, int D,
+                          const int64_t* hoff, uint32_t* cursor, int32_t* perm);
+int launch_sort_segments(drpt_ctx* ctx, int32_t* perm, int64_t n, const int64_t* seg_start,
+                         const int32_t* seg_size, const int32_t* seg_tree, int nseg, int max_seg);
+int launch_apply_local(drpt_ctx* ctx, const int32_t* perm_in, int32_t* perm_out, int64_t n,
+                       const int64_t* seg_start, const int32_t* seg_tree, const int64_t* seg_tbl,
+                       const int32_t* seg_size, int nseg, const int32_t* tables);
+
+int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, int precision,
+                     void* Xq, float* nrm2, float* err, float* scale);
+int launch_leaf_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
+                    const LeafItem* items, int64_t nitems, const void* Xq, const float* nrm2,
+                    const float* err, const float* scale, int32_t* cand_idx, float* cand_dist,
+                    int32_t* fallback, int32_t* nfallback, int64_t fallback_cap);
+int launch_fallback(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
+                    const LeafItem* items, const int32_t* fallback, const int32_t* nfallback,
+                    int64_t fallback_cap, int32_t* cand_idx, float* cand_dist);
+
+// merge.hip
+int launch_merge(drpt_ctx* ctx, int64_t n, int T, int K, const int32_t* cand_idx,
+                 const float* cand_dist, int32_t* out_idx, float* out_dist);
+
+}  // namespace drpt

@@ -1,0 +1,123 @@
+// Stage (a): sparse random projection P = X.R on gfx950.
+//
+// Replaces MathOp::multiply_mat (cpp/src/drpt/math/math_operations.cpp:15-46),
+// a dense cblas_sgemm over a matrix R that is ~1/sqrt(d) dense. Here R is
+// held as compressed columns (non-zeros of each column, ascending row), X rows
+// are read once, coalesced, into an LDS tile, and every output is the
+// double-precision sum of x_ij * r_jc over column c's non-zeros in ascending j,
+// rounded once to float: a fixed order, so P is bit-reproducible (products of
+// u8/f32 by f32 are exact in double, so FMA contraction cannot change it).
+//
+// Layout: X row-major [n][d] (u8 or f32); P column-major [L][n], so the tree
+// split of level l of tree t reads one contiguous column; colmax[c] = max bits
+// of |P[c][.]| (non-negative floats order like their bit patterns).
+//
+// Roofline: HBM-bound. Algorithmic bytes = n*d*sizeof(x) + 4*n*L.
+#include "internal.hpp"
+
+namespace drpt {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+// TR rows per block; lane -> (row = lane % TR, column slot = lane / TR).
+template <typename XT, int TR>
+__global__ void __launch_bounds__(kThreads)
+k_project(const XT* __restrict__ X, int64_t n, int d, int L, const int32_t* __restrict__ colptr,
+          const uint16_t* __restrict__ rowidx, const float* __restrict__ w, float* __restrict__ P,
+          uint32_t* __restrict__ colmax, int pitch) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  XT* tile = reinterpret_cast<XT*>(smem);
+  const int64_t row0 = (int64_t)blockIdx.x * TR;
+  const int rows = (int)min((int64_t)TR, n - row0);
+
+  // Stage rows [row0, row0 + rows) into LDS (row pitch padded against bank conflicts).
+  if (sizeof(XT) == 1 && (d & 3) == 0) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(X + row0 * d);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(tile);
+    const int wpr = d >> 2;        // words per row
+    const int wpitch = pitch >> 2;
+    const int total = rows * wpr;
+    for (int e = threadIdx.x; e < total; e += kThreads) {
+      int r = e / wpr, c = e - r * wpr;
+      dst[r * wpitch + c] = src[e];
+    }
+  } else {
+    const XT* src = X + row0 * d;
+    const int total = rows * d;
+    for (int e = threadIdx.x; e < total; e += kThreads) {
+      int r = e / d, c = e - r * d;
+      tile[r * pitch + c] = src[e];
+    }
+  }
+  __syncthreads();
+
+  constexpr int kSlots = 64 / TR;  // columns a wave works on at once
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int r = lane % TR;
+  const int slot = lane / TR;
+  const bool valid_row = r < rows;
+  for (int cb = wave * kSlots; cb < L; cb += 4 * kSlots) {
+    const int c = cb + slot;
+    float pv = 0.0f;
+    if (c < L) {
+      const int beg = colptr[c], end = colptr[c + 1];
+      double acc = 0.0;
+      const XT* xr = tile + r * pitch;
+      for (int k = beg; k < end; ++k) {
+        double x = (double)xr[rowidx[k]];
+        double prod = __dmul_rn(x, (double)w[k]);
+        acc = __dadd_rn(acc, prod);
+      }
+      pv = (float)acc;
+      if (valid_row) P[(int64_t)c * n + row0 + r] = pv;
+    }
+    uint32_t bits = (valid_row && c < L) ? (__float_as_uint(pv) & 0x7fffffffu) : 0u;
+    // reduce the max over the TR lanes sharing column c
+    for (int off = TR / 2; off > 0; off >>= 1) bits = max(bits, (uint32_t)__shfl_xor((int)bits, off));
+    if (r == 0 && c < L && bits) atomicMax(&colmax[c], bits);
+  }
+}
+
+template <typename XT, int TR>
+int run(drpt_ctx* ctx, const XT* X, int64_t n, int d, const int32_t* colptr, const uint16_t* rowidx,
+        const float* w, int L, float* P, uint32_t* colmax) {
+  int pitch;
+  if (sizeof(XT) == 1) {
+    int words = (d + 3) / 4;
+    if ((words & 1) == 0) words += 1;  // odd word pitch: rows land on distinct banks
+    pitch = words * 4;
+  } else {
+    pitch = d | 1;
+  }
+  size_t lds = (size_t)TR * pitch * sizeof(XT);
+  int64_t blocks = (n + TR - 1) / TR;
+  hipLaunchKernelGGL((k_project<XT, TR>), dim3((unsigned)blocks), dim3(kThreads), lds, ctx->stream,
+                     X, n, d, L, colptr, rowidx, w, P, colmax, pitch);
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
+}
+
+}  // namespace
+
+int launch_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, const int32_t* colptr,
+                   const uint16_t* rowidx, const float* w, int L, float* P, uint32_t* colmax) {
+  DRPT_CHECK(fill_u32(ctx, colmax, 0u, (size_t)L));
+  if (n <= 0) return DRPT_OK;
+  const size_t kLdsBudget = 64 * 1024;
+  if (dtype == DRPT_U8) {
+    return run<uint8_t, 64>(ctx, (const uint8_t*)X, n, d, colptr, rowidx, w, L, P, colmax);
+  }
+  const float* Xf = (const float*)X;
+  size_t row_bytes = (size_t)((d | 1)) * 4;
+  if (64 * row_bytes <= kLdsBudget) return run<float, 64>(ctx, Xf, n, d, colptr, rowidx, w, L, P, colmax);
+  if (32 * row_bytes <= kLdsBudget) return run<float, 32>(ctx, Xf, n, d, colptr, rowidx, w, L, P, colmax);
+  if (16 * row_bytes <= kLdsBudget) return run<float, 16>(ctx, Xf, n, d, colptr, rowidx, w, L, P, colmax);
+  if (8 * row_bytes <= kLdsBudget) return run<float, 8>(ctx, Xf, n, d, colptr, rowidx, w, L, P, colmax);
+  set_error("project: dimension too large for the LDS tile");
+  return DRPT_ERR_UNSUPPORTED;
+}
+
+}  // namespace drpt

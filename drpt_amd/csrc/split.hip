@@ -1,0 +1,477 @@
+// Stage (b): per-level histogram-median splits of all trees at once.
+//
+// Replaces DRPTGlobal::grow_global_tree / grow_global_subtree
+// (cpp/src/drpt/algo/drpt_global.cpp:153-352) and MathOp::distributed_mean,
+// distributed_variance, distributed_median (math_operations.cpp:121-383).
+//
+// Layout: points never move. node[t][i] holds the heap index (root 1) of
+// point i's current node in tree t; a point is at level l iff node>>l == 1.
+// Level l of tree t reads the contiguous projection column P[t*D + l][.].
+// Per-node state (counts, fixed-point sums, mean, 16-slot edge table,
+// histogram, median) lives in small [T][2^l] arrays.
+//
+// Per level, all trees in one launch each:
+//   k_sum1   count + sum of quantised values per node      (per point)
+//   k_mu     split decision + mean                          (per node)
+//   k_sum2   sum of quantised fl(v-mu)^2 per node            (per point)
+//   k_edges  variance, sigma, edge table                     (per node)
+//   k_hist   16-slot histogram (wave-ballot when a wave shares a node)
+//   k_median cumulative search + interpolation               (per node)
+//   k_part   v <= median -> 2h, else 2h+1                     (per point)
+//
+// Arithmetic (bit-exact with the oracle's canonical statistics):
+//   q(v) = rint(v * 2^-kq1) as int64; S = sum q (order-independent);
+//   mu = fl32(S * 2^kq1 / n) (double); w = fl32(v - mu); w2 = fl32(w*w);
+//   var = fl32(sum rint(w2 * 2^-kq2) * 2^kq2 / (n-1)); sigma = sqrtf(var);
+//   edges/histogram/median exactly as math_operations.cpp:223-377 with
+//   E[-1] = 0. No floating-point contraction anywhere in this file.
+//
+// Roofline: HBM-bound; algorithmic bytes 12 per point per tree per level
+// (4 B value + 4 B node read + 4 B node write).
+#include "internal.hpp"
+
+namespace drpt {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kLdsSumNodes = 2048;   // nodes per level kept in LDS for the sums
+constexpr int kLdsHistNodes = 512;   // nodes per level kept in LDS for histograms
+
+__device__ __forceinline__ void col_exponents(uint32_t cm, int bitlen, int& kq1, int& kq2) {
+  int E = (int)((cm >> 23) & 0xffu);
+  int e = (E == 0) ? -127 : E - 127;
+  int shift = 63 - bitlen;
+  kq1 = e + 1 - shift;
+  kq2 = 2 * (e + 2) - shift;
+}
+
+__device__ __forceinline__ long long quant(float v, int kq) {
+  return (long long)rint(ldexp((double)v, -kq));
+}
+
+__device__ __forceinline__ long long wave_sum_i64(long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Points [b0, b1) of tree blockIdx.y handled by this block.
+__device__ __forceinline__ void block_range(int64_t n, int64_t ppb, int64_t& b0, int64_t& b1) {
+  b0 = (int64_t)blockIdx.x * ppb;
+  b1 = min(n, b0 + ppb);
+}
+
+template <bool kLds>
+__global__ void __launch_bounds__(kThreads)
+k_sum1(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n, int D, int l,
+       const uint32_t* __restrict__ colmax, int bitlen, uint32_t* __restrict__ cnt,
+       unsigned long long* __restrict__ S1, int64_t ppb) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int t = blockIdx.y;
+  const int NN = 1 << l;
+  unsigned long long* lS = reinterpret_cast<unsigned long long*>(smem);
+  uint32_t* lc = reinterpret_cast<uint32_t*>(lS + (kLds ? NN : 0));
+  if (kLds) {
+    for (int k = threadIdx.x; k < NN; k += kThreads) { lS[k] = 0; lc[k] = 0; }
+    __syncthreads();
+  }
+  const int64_t col = (int64_t)t * D + l;
+  int kq1, kq2;
+  col_exponents(colmax[col], bitlen, kq1, kq2);
+  const float* Pc = P + col * n;
+  const uint32_t* nt = node + (int64_t)t * n;
+  uint32_t* gc = cnt + (int64_t)t * NN;
+  unsigned long long* gS = S1 + (int64_t)t * NN;
+  int64_t b0, b1;
+  block_range(n, ppb, b0, b1);
+  const int lane = threadIdx.x & 63;
+  for (int64_t base = b0; base < b1; base += kThreads) {
+    int64_t i = base + threadIdx.x;
+    bool act = false;
+    int nd = 0;
+    long long q = 0;
+    if (i < b1) {
+      uint32_t h = nt[i];
+      act = (h >> l) == 1u;
+      nd = (int)(h - (uint32_t)NN);
+      if (act) q = quant(Pc[i], kq1);
+    }
+    unsigned long long am = __ballot(act);
+    if (am == 0) continue;
+    int leader = __ffsll((long long)am) - 1;
+    int ndl = __shfl(nd, leader);
+    unsigned long long same = __ballot(act && nd == ndl);
+    if (same == am) {
+      long long s = wave_sum_i64(q);
+      if (lane == leader) {
+        if (kLds) {
+          atomicAdd(&lc[ndl], (uint32_t)__popcll(am));
+          atomicAdd(&lS[ndl], (unsigned long long)s);
+        } else {
+          atomicAdd(&gc[ndl], (uint32_t)__popcll(am));
+          atomicAdd(&gS[ndl], (unsigned long long)s);
+        }
+      }
+    } else if (act) {
+      if (kLds) {
+        atomicAdd(&lc[nd], 1u);
+        atomicAdd(&lS[nd], (unsigned long long)q);
+      } else {
+        atomicAdd(&gc[nd], 1u);
+        atomicAdd(&gS[nd], (unsigned long long)q);
+      }
+    }
+  }
+  if (kLds) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < NN; k += kThreads) {
+      if (lc[k]) {
+        atomicAdd(&gc[k], lc[k]);
+        atomicAdd(&gS[k], lS[k]);
+      }
+    }
+  }
+}
+
+__global__ void k_mu(int T, int l, int D, int mode, int leaf_max, int bitlen,
+                     const uint32_t* __restrict__ colmax, const uint32_t* __restrict__ cnt,
+                     const unsigned long long* __restrict__ S1, float* __restrict__ mu,
+                     uint32_t* __restrict__ split) {
+  const int NN = 1 << l;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)T * NN) return;
+  int t = (int)(idx / NN);
+  int kq1, kq2;
+  col_exponents(colmax[(int64_t)t * D + l], bitlen, kq1, kq2);
+  uint32_t c = cnt[idx];
+  split[idx] = (mode == DRPT_TREE_FIXED) ? 1u : (c > (uint32_t)leaf_max ? 1u : 0u);
+  double s = ldexp((double)(long long)S1[idx], kq1);
+  mu[idx] = (float)(s / (double)c);
+}
+
+template <bool kLds>
+__global__ void __launch_bounds__(kThreads)
+k_sum2(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n, int D, int l,
+       const uint32_t* __restrict__ colmax, int bitlen, const float* __restrict__ mu,
+       const uint32_t* __restrict__ split, unsigned long long* __restrict__ S2, int64_t ppb) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int t = blockIdx.y;
+  const int NN = 1 << l;
+  unsigned long long* lS = reinterpret_cast<unsigned long long*>(smem);
+  if (kLds) {
+    for (int k = threadIdx.x; k < NN; k += kThreads) lS[k] = 0;
+    __syncthreads();
+  }
+  const int64_t col = (int64_t)t * D + l;
+  int kq1, kq2;
+  col_exponents(colmax[col], bitlen, kq1, kq2);
+  const float* Pc = P + col * n;
+  const uint32_t* nt = node + (int64_t)t * n;
+  const float* mt = mu + (int64_t)t * NN;
+  const uint32_t* st = split + (int64_t)t * NN;
+  unsigned long long* gS = S2 + (int64_t)t * NN;
+  int64_t b0, b1;
+  block_range(n, ppb, b0, b1);
+  const int lane = threadIdx.x & 63;
+  for (int64_t base = b0; base < b1; base += kThreads) {
+    int64_t i = base + threadIdx.x;
+    bool act = false;
+    int nd = 0;
+    long long q = 0;
+    if (i < b1) {
+      uint32_t h = nt[i];
+      act = (h >> l) == 1u;
+      nd = (int)(h - (uint32_t)NN);
+      if (act) act = st[nd] != 0u;
+      if (act) {
+        float w = __fsub_rn(Pc[i], mt[nd]);
+        float w2 = __fmul_rn(w, w);
+        q = quant(w2, kq2);
+      }
+    }
+    unsigned long long am = __ballot(act);
+    if (am == 0) continue;
+    int leader = __ffsll((long long)am) - 1;
+    int ndl = __shfl(nd, leader);
+    unsigned long long same = __ballot(act && nd == ndl);
+    if (same == am) {
+      long long s = wave_sum_i64(q);
+      if (lane == leader) {
+        if (kLds) atomicAdd(&lS[ndl], (unsigned long long)s);
+        else atomicAdd(&gS[ndl], (unsigned long long)s);
+      }
+    } else if (act) {
+      if (kLds) atomicAdd(&lS[nd], (unsigned long long)q);
+      else atomicAdd(&gS[nd], (unsigned long long)q);
+    }
+  }
+  if (kLds) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < NN; k += kThreads)
+      if (lS[k]) atomicAdd(&gS[k], lS[k]);
+  }
+}
+
+// Edge table: math_operations.cpp:250-288 (factor 1, no_of_bins 7).
+__global__ void k_edges(int T, int l, int D, int bitlen, const uint32_t* __restrict__ colmax,
+                        const uint32_t* __restrict__ cnt, const unsigned long long* __restrict__ S2,
+                        const float* __restrict__ mu, const uint32_t* __restrict__ split,
+                        float* __restrict__ E) {
+  const int NN = 1 << l;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)T * NN) return;
+  if (!split[idx]) return;
+  int t = (int)(idx / NN);
+  int kq1, kq2;
+  col_exponents(colmax[(int64_t)t * D + l], bitlen, kq1, kq2);
+  long long c = (long long)cnt[idx];
+  double s2 = ldexp((double)(long long)S2[idx], kq2);
+  float var = (float)(s2 / (double)(c - 1));
+  float m = mu[idx];
+  float sigma = sqrtf(var);
+  float s1 = (float)((double)sigma / 30.0);
+  float s2f = (float)((double)sigma / 6.0);
+  float s3 = (float)((double)sigma / 2.0);
+  float e[16];
+#pragma unroll
+  for (int j = 1; j <= 4; ++j) {
+    float r = __fmul_rn((float)j, s1);
+    e[6 + j] = __fadd_rn(m, r);
+    e[7 - j] = __fsub_rn(m, r);
+  }
+#pragma unroll
+  for (int j = 1; j <= 2; ++j) {
+    float r = __fadd_rn(sigma, __fmul_rn((float)j, s2f));
+    e[10 + j] = __fadd_rn(m, r);
+    e[3 - j] = __fsub_rn(m, r);
+  }
+  {
+    float r = __fadd_rn(__fmul_rn(2.0f, sigma), s3);
+    e[13] = __fadd_rn(m, r);
+    e[0] = __fsub_rn(m, r);
+  }
+  e[14] = 0.0f;
+  e[15] = 0.0f;
+  float4* dst = reinterpret_cast<float4*>(E + idx * 16);
+  dst[0] = make_float4(e[0], e[1], e[2], e[3]);
+  dst[1] = make_float4(e[4], e[5], e[6], e[7]);
+  dst[2] = make_float4(e[8], e[9], e[10], e[11]);
+  dst[3] = make_float4(e[12], e[13], e[14], e[15]);
+}
+
+// Slot of v: math_operations.cpp:292-308 (no slot matched -> slot 0).
+__device__ __forceinline__ int slot_of(float v, const float* e) {
+  int b = 0;
+#pragma unroll
+  for (int j = 1; j < 16; ++j) {
+    bool m = (e[j - 1] < v) && (e[j] >= v);
+    b = m ? j : b;
+  }
+  return b;
+}
+
+template <bool kLds>
+__global__ void __launch_bounds__(kThreads)
+k_hist(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n, int D, int l,
+       const float* __restrict__ E, const uint32_t* __restrict__ split, uint32_t* __restrict__ H,
+       int64_t ppb) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int t = blockIdx.y;
+  const int NN = 1 << l;
+  uint32_t* lH = reinterpret_cast<uint32_t*>(smem);
+  if (kLds) {
+    for (int k = threadIdx.x; k < NN * 16; k += kThreads) lH[k] = 0;
+    __syncthreads();
+  }
+  const int64_t col = (int64_t)t * D + l;
+  const float* Pc = P + col * n;
+  const uint32_t* nt = node + (int64_t)t * n;
+  const float* Et = E + (int64_t)t * NN * 16;
+  const uint32_t* st = split + (int64_t)t * NN;
+  uint32_t* gH = H + (int64_t)t * NN * 16;
+  int64_t b0, b1;
+  block_range(n, ppb, b0, b1);
+  const int lane = threadIdx.x & 63;
+  for (int64_t base = b0; base < b1; base += kThreads) {
+    int64_t i = base + threadIdx.x;
+    bool act = false;
+    int nd = 0, b = 0;
+    if (i < b1) {
+      uint32_t h = nt[i];
+      act = (h >> l) == 1u;
+      nd = (int)(h - (uint32_t)NN);
+      if (act) act = st[nd] != 0u;
+      if (act) {
+        float e[16];
+        const float4* src = reinterpret_cast<const float4*>(Et + (int64_t)nd * 16);
+        float4 a0 = src[0], a1 = src[1], a2 = src[2], a3 = src[3];
+        e[0] = a0.x; e[1] = a0.y; e[2] = a0.z; e[3] = a0.w;
+        e[4] = a1.x; e[5] = a1.y; e[6] = a1.z; e[7] = a1.w;
+        e[8] = a2.x; e[9] = a2.y; e[10] = a2.z; e[11] = a2.w;
+        e[12] = a3.x; e[13] = a3.y; e[14] = a3.z; e[15] = a3.w;
+        b = slot_of(Pc[i], e);
+      }
+    }
+    unsigned long long am = __ballot(act);
+    if (am == 0) continue;
+    int leader = __ffsll((long long)am) - 1;
+    int ndl = __shfl(nd, leader);
+    unsigned long long same = __ballot(act && nd == ndl);
+    if (same == am) {
+      // wave-ballot histogram: lane k < 16 adds the population of slot k
+      uint32_t mine = 0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        uint32_t c = (uint32_t)__popcll(__ballot(act && b == k));
+        mine = (lane == k) ? c : mine;
+      }
+      if (lane < 16 && mine) {
+        if (kLds) atomicAdd(&lH[ndl * 16 + lane], mine);
+        else atomicAdd(&gH[(int64_t)ndl * 16 + lane], mine);
+      }
+    } else if (act) {
+      if (kLds) atomicAdd(&lH[nd * 16 + b], 1u);
+      else atomicAdd(&gH[(int64_t)nd * 16 + b], 1u);
+    }
+  }
+  if (kLds) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < NN * 16; k += kThreads)
+      if (lH[k]) atomicAdd(&gH[k], lH[k]);
+  }
+}
+
+// Cumulative search + interpolation: math_operations.cpp:333-377, E[-1] = 0.
+__global__ void k_median(int T, int l, const uint32_t* __restrict__ cnt,
+                         const uint32_t* __restrict__ H, const float* __restrict__ E,
+                         const uint32_t* __restrict__ split, float* __restrict__ med) {
+  const int NN = 1 << l;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)T * NN) return;
+  if (!split[idx]) return;
+  long long c = (long long)cnt[idx];
+  const uint32_t* h = H + idx * 16;
+  const float* e = E + idx * 16;
+  float val = (float)((50 * c) / 100);
+  float cf = 0.0f;
+  int sel = -1;
+  for (int k = 0; k < 16; ++k) {
+    cf = (float)__dadd_rn((double)cf, (double)h[k]);
+    if (cf >= val) {
+      sel = k;
+      break;
+    }
+  }
+  if (sel < 0) sel = 15;
+  double count = (double)h[sel];
+  float lo = (sel == 0) ? 0.0f : e[sel - 1];
+  float width = __fsub_rn(e[sel], lo);
+  double num = __dsub_rn((double)(c / 2), __dsub_rn((double)cf, count));
+  double frac = num / count;
+  double m = __dadd_rn((double)lo, __dmul_rn(frac, (double)width));
+  med[idx] = (float)m;
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_part(const float* __restrict__ P, uint32_t* __restrict__ node, int64_t n, int D, int l,
+       const float* __restrict__ med, const uint32_t* __restrict__ split) {
+  const int t = blockIdx.y;
+  const int NN = 1 << l;
+  const float* Pc = P + ((int64_t)t * D + l) * n;
+  uint32_t* nt = node + (int64_t)t * n;
+  const float* mt = med + (int64_t)t * NN;
+  const uint32_t* st = split + (int64_t)t * NN;
+  int64_t stride = (int64_t)gridDim.x * kThreads;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+    uint32_t h = nt[i];
+    if ((h >> l) != 1u) continue;
+    int nd = (int)(h - (uint32_t)NN);
+    if (!st[nd]) continue;
+    float v = Pc[i];
+    nt[i] = 2u * h + ((v <= mt[nd]) ? 0u : 1u);
+  }
+}
+
+int bitlen_of(int64_t n) {
+  int b = 0;
+  while (b < 63 && (n >> b) != 0) ++b;
+  return b;
+}
+
+}  // namespace
+
+int grow_trees_impl(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t n, int T, int D,
+                    int mode, int leaf_max, uint32_t* node) {
+  if (D < 1 || D > 24 || T < 1 || n < 1) {
+    set_error("grow_trees: depth must be in [1, 24], ntrees >= 1, n >= 1");
+    return DRPT_ERR_ARG;
+  }
+  DRPT_CHECK(fill_u32(ctx, node, 1u, (size_t)T * n));
+  if (D == 1) return DRPT_OK;
+  const int64_t NMAX = (int64_t)1 << (D - 2);
+  uint32_t *cnt, *split, *H;
+  unsigned long long *S1, *S2;
+  float *mu, *E, *med;
+  DRPT_CHECK(ws(ctx, "split.cnt", T * NMAX, &cnt));
+  DRPT_CHECK(ws(ctx, "split.flag", T * NMAX, &split));
+  DRPT_CHECK(ws(ctx, "split.S1", T * NMAX, &S1));
+  DRPT_CHECK(ws(ctx, "split.S2", T * NMAX, &S2));
+  DRPT_CHECK(ws(ctx, "split.mu", T * NMAX, &mu));
+  DRPT_CHECK(ws(ctx, "split.med", T * NMAX, &med));
+  DRPT_CHECK(ws(ctx, "split.E", T * NMAX * 16, &E));
+  DRPT_CHECK(ws(ctx, "split.H", T * NMAX * 16, &H));
+  const int bitlen = bitlen_of(n);
+  hipStream_t s = ctx->stream;
+  for (int l = 0; l <= D - 2; ++l) {
+    const int64_t NN = (int64_t)1 << l;
+    const int64_t TN = (int64_t)T * NN;
+    DRPT_CHECK(fill_u32(ctx, cnt, 0u, TN));
+    DRPT_CHECK(fill_u32(ctx, (uint32_t*)S1, 0u, 2 * TN));
+    DRPT_CHECK(fill_u32(ctx, (uint32_t*)S2, 0u, 2 * TN));
+    DRPT_CHECK(fill_u32(ctx, H, 0u, 16 * TN));
+    // points per block: enough work to amortise flushing the per-node LDS arrays
+    int64_t ppb = std::max<int64_t>(8192, NN * 32);
+    int64_t bx = (n + ppb - 1) / ppb;
+    dim3 pgrid((unsigned)bx, (unsigned)T);
+    unsigned ngrid = (unsigned)((TN + 255) / 256);
+    bool lds_sum = NN <= kLdsSumNodes;
+    bool lds_hist = NN <= kLdsHistNodes;
+    if (lds_sum)
+      hipLaunchKernelGGL(k_sum1<true>, pgrid, dim3(kThreads), NN * 12, s, P, node, n, D, l, colmax,
+                         bitlen, cnt, S1, ppb);
+    else
+      hipLaunchKernelGGL(k_sum1<false>, pgrid, dim3(kThreads), 0, s, P, node, n, D, l, colmax,
+                         bitlen, cnt, S1, ppb);
+    DRPT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_mu, dim3(ngrid), dim3(256), 0, s, T, l, D, mode, leaf_max, bitlen, colmax,
+                       cnt, S1, mu, split);
+    DRPT_HIP(hipGetLastError());
+    if (lds_sum)
+      hipLaunchKernelGGL(k_sum2<true>, pgrid, dim3(kThreads), NN * 8, s, P, node, n, D, l, colmax,
+                         bitlen, mu, split, S2, ppb);
+    else
+      hipLaunchKernelGGL(k_sum2<false>, pgrid, dim3(kThreads), 0, s, P, node, n, D, l, colmax,
+                         bitlen, mu, split, S2, ppb);
+    DRPT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_edges, dim3(ngrid), dim3(256), 0, s, T, l, D, bitlen, colmax, cnt, S2, mu,
+                       split, E);
+    DRPT_HIP(hipGetLastError());
+    if (lds_hist)
+      hipLaunchKernelGGL(k_hist<true>, pgrid, dim3(kThreads), NN * 64, s, P, node, n, D, l, E,
+                         split, H, ppb);
+    else
+      hipLaunchKernelGGL(k_hist<false>, pgrid, dim3(kThreads), 0, s, P, node, n, D, l, E, split, H,
+                         ppb);
+    DRPT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_median, dim3(ngrid), dim3(256), 0, s, T, l, cnt, H, E, split, med);
+    DRPT_HIP(hipGetLastError());
+    int64_t pbx = std::min<int64_t>((n + kThreads * 4 - 1) / (kThreads * 4), 4096);
+    hipLaunchKernelGGL(k_part, dim3((unsigned)pbx, (unsigned)T), dim3(kThreads), 0, s, P, node, n,
+                       D, l, med, split);
+    DRPT_HIP(hipGetLastError());
+  }
+  return DRPT_OK;
+}
+
+}  // namespace drpt
