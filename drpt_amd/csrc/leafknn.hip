@@ -277,7 +277,7 @@ k_knn_bf16(const XT* __restrict__ X, const uint16_t* __restrict__ Xb,
           const int k = k0 + part * 8;
           const int ida = rowid[r], idb = colid[r];
           uint4 va = make_uint4(0, 0, 0, 0), vb = make_uint4(0, 0, 0, 0);
-          if (k + 8 <= d) {
+          if ((d & 7) == 0 && k + 8 <= d) {  // 16-byte aligned rows only
             if (ida >= 0) va = *reinterpret_cast<const uint4*>(Xb + (int64_t)ida * d + k);
             if (idb >= 0) vb = *reinterpret_cast<const uint4*>(Xb + (int64_t)idb * d + k);
           } else if (k < d) {
