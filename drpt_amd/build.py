@@ -26,6 +26,7 @@ HIP_FLAGS = [
     "-Wno-unused-result", "-Wno-pass-failed",
 ]
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall"]
+CLI_FLAGS = ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
 HIP_SOURCES = ["api.hip", "util.hip", "project.hip", "split.hip", "group.hip", "leafknn.hip",
                "merge.hip"]
@@ -49,22 +50,24 @@ def build(verbose=False):
     jobs = []
     for s in HIP_SOURCES:
         jobs.append(["hipcc", *HIP_FLAGS, "-c", os.path.join(CSRC, s), "-o", _obj(s)])
-    for s in HOST_SOURCES + CLI_SOURCES:
-        if os.path.exists(os.path.join(CSRC, s)):
-            jobs.append(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, s), "-o", _obj(s)])
+    for s in HOST_SOURCES:
+        jobs.append(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, s), "-o", _obj(s)])
+    for s in CLI_SOURCES:
+        jobs.append(["g++", *HOST_FLAGS, *CLI_FLAGS, "-c", os.path.join(CSRC, s), "-o", _obj(s)])
     workers = min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(workers) as ex:
         for r in ex.map(_run, jobs):
-            if verbose and (r.stdout or r.stderr):
-                print(r.stdout + r.stderr, file=sys.stderr)
+            msg = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
+                            if "packed-fp32-ops" not in l and l.strip())
+            if verbose and msg:
+                print(msg, file=sys.stderr)
     lib_objs = [_obj(s) for s in HIP_SOURCES + HOST_SOURCES]
     tmp = LIB + ".tmp"
     _run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-o", tmp, *lib_objs])
     os.replace(tmp, LIB)
-    cli_objs = [_obj(s) for s in CLI_SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    if len(cli_objs) == len(CLI_SOURCES):
-        _run(["g++", "-o", CLI, *cli_objs, f"-L{HERE}", "-ldrpt_hip", f"-Wl,-rpath,{HERE}",
-              "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64"])
+    cli_objs = [_obj(s) for s in CLI_SOURCES]
+    _run(["g++", "-o", CLI, *cli_objs, f"-L{HERE}", "-ldrpt_hip", "-Wl,-rpath,$ORIGIN",
+          "-L/opt/rocm/lib", "-lamdhip64"])
     return LIB
 
 

@@ -1,0 +1,161 @@
+// drpt — command-line driver of the MI355X k-NN graph builder.
+//
+// Keeps the reference driver's options and flow (cpp/src/main.cpp:25-259):
+// parse flags (:43-97), validate (:99-127), default density 1/sqrt(d)
+// (:129-132), tree depth int(log2(N) - 2) (:141-145), read (:158-180), build
+// (:198-217), write nn-1 edges per point to <output>results.txt (:227-228) and
+// print "IO Time (s) ... Index building (s) ... Querying Time (s) ..."
+// (:247-251). One process drives one GPU; the build runs through the C ABI.
+//
+// Options beyond the reference: -seed (the reference draws its seeds from
+// std::random_device), -precision fp32|bf16, -kprime, -leaf-max (split nodes
+// larger than this instead of the reference local stage), -depth, -device.
+// -locality is accepted and has no effect on one GPU.
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../../include/drpt_hip.h"
+#include "io.hpp"
+
+namespace {
+
+void die(const char* msg) {
+  std::printf("%s\n", msg);
+  std::exit(1);
+}
+
+int check(int rc, const char* what) {
+  if (rc != DRPT_OK) {
+    std::fprintf(stderr, "%s failed (%d): %s\n", what, rc, drpt_last_error());
+    std::exit(1);
+  }
+  return rc;
+}
+
+}  // namespace
+
+int main(int argc, char* argv[]) {
+  std::string input_path, output_path;
+  long data_set_size = 0;
+  int dimension = 0, ntrees = 10, nn = 0, local_tree_offset = 2, file_format = 0;
+  int data_starting_index = 8, bytes_for_data_type = 4;
+  double tree_depth_ratio = 0.6, density = 0;
+  uint32_t seed = 1;
+  std::string precision = "fp32";
+  int kprime = 0, leaf_max = 0, depth_override = 0, device = 0;
+  for (int p = 0; p + 1 < argc; p++) {
+    const char* a = argv[p];
+    const char* v = argv[p + 1];
+    if (!std::strcmp(a, "-input")) input_path = v;
+    else if (!std::strcmp(a, "-output")) output_path = v;
+    else if (!std::strcmp(a, "-data-set-size")) data_set_size = std::atol(v);
+    else if (!std::strcmp(a, "-dimension")) dimension = std::atoi(v);
+    else if (!std::strcmp(a, "-ntrees")) ntrees = std::atoi(v);
+    else if (!std::strcmp(a, "-tree-depth-ratio")) tree_depth_ratio = std::stof(v);
+    else if (!std::strcmp(a, "-density")) density = std::atoi(v);  // integer, as main.cpp:71
+    else if (!std::strcmp(a, "-nn")) nn = std::atoi(v);
+    else if (!std::strcmp(a, "-locality")) (void)std::atoi(v);
+    else if (!std::strcmp(a, "-local-tree-offset")) local_tree_offset = std::atoi(v);
+    else if (!std::strcmp(a, "-data-file-format")) file_format = std::atoi(v);
+    else if (!std::strcmp(a, "-bytes-for-data-type")) bytes_for_data_type = std::atoi(v);
+    else if (!std::strcmp(a, "-data-starting-index")) data_starting_index = std::atoi(v);
+    else if (!std::strcmp(a, "-seed")) seed = (uint32_t)std::strtoul(v, nullptr, 10);
+    else if (!std::strcmp(a, "-precision")) precision = v;
+    else if (!std::strcmp(a, "-kprime")) kprime = std::atoi(v);
+    else if (!std::strcmp(a, "-leaf-max")) leaf_max = std::atoi(v);
+    else if (!std::strcmp(a, "-depth")) depth_override = std::atoi(v);
+    else if (!std::strcmp(a, "-device")) device = std::atoi(v);
+  }
+  if (input_path.empty()) die("Valid input path needed!...");
+  if (output_path.empty()) die("Valid out path needed!...");
+  if (data_set_size == 0) die("Dataset size should be greater than 0");
+  if (dimension == 0) die("Dimension size should be greater than 0");
+  if (nn == 0) die("Nearest neighbours size should be greater than 0");
+  if (density == 0) density = 1.0 / std::sqrt((double)dimension);
+  if (file_format != 0 && file_format != 1)
+    die("Unsupported data file format, ubyte and bin file formats are supported!...");
+
+  drpt_params p;
+  check(drpt_derive_params(data_set_size, dimension, ntrees, nn, tree_depth_ratio, &p),
+        "derive_params");
+  std::printf(" tree depth %d\n", (int)(std::log2((double)data_set_size) - 2));
+  p.local_offset = local_tree_offset;
+  p.density = (float)density;
+  p.seed = seed;
+  p.kprime = kprime;
+  if (precision == "fp32") p.precision = DRPT_FP32;
+  else if (precision == "bf16") p.precision = DRPT_BF16;
+  else if (precision == "fp8") p.precision = DRPT_FP8;
+  else die("-precision must be fp32, bf16 or fp8");
+  if (leaf_max > 0) {
+    p.local_mode = DRPT_LOCAL_LEAFMAX;
+    p.leaf_max = leaf_max;
+    int need = (int)std::ceil(std::log2(std::max(2.0, (double)data_set_size / leaf_max))) + 4;
+    if (need > p.depth) p.depth = need;
+  }
+  if (depth_override > 0) p.depth = depth_override;
+
+  auto t_io0 = std::chrono::steady_clock::now();
+  std::vector<uint8_t> xu8;
+  std::vector<float> xf;
+  std::string err;
+  if (file_format == 0) {
+    if (drpt_io::read_idx3_ubyte(input_path, data_set_size, dimension, xu8, err)) die(err.c_str());
+    p.dtype = DRPT_U8;
+  } else {
+    if (drpt_io::read_fbin(input_path, data_set_size, dimension, bytes_for_data_type,
+                           data_starting_index, xf, err))
+      die(err.c_str());
+    p.dtype = DRPT_F32;
+  }
+  auto t_io1 = std::chrono::steady_clock::now();
+  std::printf("Rank 0 IO reading completed %ld*%d\n", data_set_size, dimension);
+
+  drpt_ctx* ctx = nullptr;
+  check(drpt_create(&ctx, device), "drpt_create");
+  const size_t xbytes = p.dtype == DRPT_U8 ? xu8.size() : xf.size() * 4;
+  void* dX = nullptr;
+  int32_t* dI = nullptr;
+  float* dD = nullptr;
+  if (hipMalloc(&dX, xbytes) != hipSuccess || hipMalloc((void**)&dI, 4ull * data_set_size * nn) ||
+      hipMalloc((void**)&dD, 4ull * data_set_size * nn))
+    die("device allocation failed");
+  if (hipMemcpy(dX, p.dtype == DRPT_U8 ? (const void*)xu8.data() : (const void*)xf.data(), xbytes,
+                hipMemcpyHostToDevice) != hipSuccess)
+    die("upload failed");
+  drpt_timings tm;
+  check(drpt_build_graph(ctx, &p, dX, dI, dD, &tm), "drpt_build_graph");
+  std::vector<int32_t> idx((size_t)data_set_size * nn);
+  if (hipMemcpy(idx.data(), dI, idx.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    die("download failed");
+  auto t_w0 = std::chrono::steady_clock::now();
+  if (drpt_io::write_edge_list(output_path + "results.txt", idx.data(), data_set_size, nn, nn - 1,
+                               err))
+    die(err.c_str());
+  auto t_w1 = std::chrono::steady_clock::now();
+  const double io_s = std::chrono::duration<double>(t_io1 - t_io0).count() +
+                      std::chrono::duration<double>(t_w1 - t_w0).count();
+  const double build_s = (tm.project_ms + tm.split_ms + tm.group_ms) / 1e3;
+  const double query_s = (tm.knn_ms + tm.merge_ms) / 1e3;
+  std::printf("IO Time (s)%g Index building (s) %g Querying Time (s) %g\n", io_s, build_s, query_s);
+  std::ofstream stats(output_path + "stats.txt", std::ios::app);
+  stats << "{\"n\": " << data_set_size << ", \"d\": " << dimension << ", \"ntrees\": " << ntrees
+        << ", \"nn\": " << nn << ", \"precision\": \"" << precision << "\", \"project_ms\": "
+        << tm.project_ms << ", \"split_ms\": " << tm.split_ms << ", \"group_ms\": " << tm.group_ms
+        << ", \"knn_ms\": " << tm.knn_ms << ", \"merge_ms\": " << tm.merge_ms
+        << ", \"total_ms\": " << tm.total_ms << ", \"leaves\": " << tm.n_leaves
+        << ", \"fallback_rows\": " << tm.n_fallback_rows << "}\n";
+  (void)hipFree(dX);
+  (void)hipFree(dI);
+  (void)hipFree(dD);
+  drpt_destroy(ctx);
+  return 0;
+}
