@@ -256,8 +256,21 @@ int drpt_grow_trees(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64
   return grow_trees_impl(ctx, P, colmax, n, ntrees, depth, tree_mode, leaf_max, leaf_heap);
 }
 
+static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
+                         const uint32_t* leaf_heap, int32_t* cand_idx, float* cand_dist,
+                         drpt_timings* timings, bool prune);
+
 int drpt_tree_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const uint32_t* leaf_heap,
                   int32_t* cand_idx, float* cand_dist, drpt_timings* timings) {
+  return tree_knn_impl(ctx, p, X, leaf_heap, cand_idx, cand_dist, timings, false);
+}
+
+// prune: trees 1..T-1 skip leafmates that are provably worse than the K-th
+// entry of tree 0's list (build_graph only: the merged lists are unchanged,
+// the per-tree lists of trees >= 1 then hold the K best of the survivors).
+static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
+                         const uint32_t* leaf_heap, int32_t* cand_idx, float* cand_dist,
+                         drpt_timings* timings, bool prune) {
   if (!ctx || !X || !leaf_heap || !cand_idx || !cand_dist) {
     set_error("tree_knn: null pointer");
     return DRPT_ERR_ARG;
@@ -271,34 +284,51 @@ int drpt_tree_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const uint
   DRPT_HIP(hipEventRecord(ctx->ev[6], s));
 
   // leaf sizes by heap index
-  uint32_t* hcount;
-  DRPT_CHECK(ws(ctx, "group.hcount", T * HS, &hcount));
-  DRPT_CHECK(launch_count_leaves(ctx, leaf_heap, n, T, D, hcount));
-  std::vector<uint32_t> hc((size_t)(T * HS));
-  DRPT_HIP(hipMemcpyAsync(hc.data(), hcount, hc.size() * 4, hipMemcpyDeviceToHost, s));
-  DRPT_HIP(hipStreamSynchronize(s));
-  std::vector<int64_t> hoff((size_t)(T * HS));
-  std::vector<Leaf> leaves;
-  for (int t = 0; t < T; ++t) {
-    int64_t acc = 0;
-    for (int64_t h = 0; h < HS; ++h) {
-      const uint32_t c = hc[(size_t)(t * HS + h)];
-      hoff[(size_t)(t * HS + h)] = acc;
-      if (c) leaves.push_back({t, (int32_t)c, acc});
-      acc += c;
-    }
-  }
-  int64_t* dhoff;
-  uint32_t* cursor;
+  uint32_t *hcount, *cursor;
+  int64_t *hoff, *tree_items, *base, *range;
   int32_t* perm;
-  DRPT_CHECK(upload(ctx, "group.hoff", hoff, &dhoff));
+  LeafItem* items;
+  unsigned long long* stats;
+  const int64_t item_cap = (int64_t)T * (n / 64 + std::min<int64_t>(n, HS) + 1);
+  DRPT_CHECK(ws(ctx, "group.hcount", T * HS, &hcount));
   DRPT_CHECK(ws(ctx, "group.cursor", T * HS, &cursor));
+  DRPT_CHECK(ws(ctx, "group.hoff", T * HS, &hoff));
+  DRPT_CHECK(ws(ctx, "group.tree_items", T, &tree_items));
+  DRPT_CHECK(ws(ctx, "group.base", T + 1, &base));
+  DRPT_CHECK(ws(ctx, "group.range", 6, &range));
+  DRPT_CHECK(ws(ctx, "group.stats", 4, &stats));
   DRPT_CHECK(ws(ctx, "group.perm", T * n, &perm));
-  DRPT_CHECK(launch_scatter_leaves(ctx, leaf_heap, n, T, D, dhoff, cursor, perm));
+  DRPT_CHECK(ws(ctx, "knn.items", item_cap, &items));
+  DRPT_CHECK(fill_u32(ctx, (uint32_t*)stats, 0u, 8));
+  DRPT_CHECK(launch_count_leaves(ctx, leaf_heap, n, T, D, hcount));
   const int32_t* perm_final = perm;
+  const bool two_phase = prune && p->precision != DRPT_FP32 && T > 1;
+  double host_gram = -1.0;
+  int64_t host_leaves = 0, host_max = 0;
 
-  std::vector<Leaf> final_leaves;
-  if (p->local_mode == DRPT_LOCAL_REFERENCE) {
+  if (p->local_mode == DRPT_LOCAL_LEAFMAX) {
+    // everything on the device: offsets, work items, scatter
+    DRPT_CHECK(launch_device_items(ctx, hcount, T, D, hoff, tree_items, base, range, items, stats));
+    DRPT_CHECK(launch_scatter_leaves(ctx, leaf_heap, n, T, D, hoff, cursor, perm));
+  } else {
+    // reference local stage: the host derives the position tables, so the
+    // global leaf sizes come back once
+    std::vector<uint32_t> hc((size_t)(T * HS));
+    DRPT_HIP(hipMemcpyAsync(hc.data(), hcount, hc.size() * 4, hipMemcpyDeviceToHost, s));
+    DRPT_HIP(hipStreamSynchronize(s));
+    std::vector<int64_t> hoffh((size_t)(T * HS));
+    std::vector<Leaf> leaves;
+    for (int t = 0; t < T; ++t) {
+      int64_t acc = 0;
+      for (int64_t h = 0; h < HS; ++h) {
+        const uint32_t c = hc[(size_t)(t * HS + h)];
+        hoffh[(size_t)(t * HS + h)] = acc;
+        if (c) leaves.push_back({t, (int32_t)c, acc});
+        acc += c;
+      }
+    }
+    DRPT_HIP(hipMemcpyAsync(hoff, hoffh.data(), hoffh.size() * 8, hipMemcpyHostToDevice, s));
+    DRPT_CHECK(launch_scatter_leaves(ctx, leaf_heap, n, T, D, hoff, cursor, perm));
     // MDRPT::get_global_minimum_leaf_size (mdrpt.cpp:293-319), grow_local_trees (:321-400)
     int64_t gmin = INT64_MAX;
     int max_seg = 0;
@@ -328,6 +358,7 @@ int drpt_tree_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const uint
     std::vector<int32_t> tables;
     std::map<int, int64_t> tbl_of;
     std::vector<int32_t> sizes((size_t)1 << ld);
+    std::vector<Leaf> final_leaves;
     for (auto& l : leaves) {
       seg_start.push_back(l.start);
       seg_size.push_back(l.size);
@@ -360,23 +391,34 @@ int drpt_tree_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const uint
     DRPT_CHECK(ws(ctx, "group.perm2", T * n, &perm2));
     DRPT_CHECK(launch_apply_local(ctx, perm, perm2, n, dss, dst, dtbo, dsz, (int)leaves.size(), dtb));
     perm_final = perm2;
-  } else {
-    final_leaves = leaves;
+    // work items in tree order (tree 0 first), largest leaves first inside a tree
+    std::stable_sort(final_leaves.begin(), final_leaves.end(), [](const Leaf& a, const Leaf& b) {
+      if (a.tree != b.tree) return a.tree < b.tree;
+      return a.size > b.size;
+    });
+    std::vector<LeafItem> hitems;
+    int64_t nfirst = 0;
+    host_gram = 0.0;
+    for (auto& l : final_leaves) {
+      host_gram += 2.0 * p->d * (double)l.size * (double)l.size;
+      host_max = std::max<int64_t>(host_max, l.size);
+      for (int r0 = 0; r0 < l.size; r0 += 64) {
+        hitems.push_back({l.tree, l.size, l.start, r0, 0});
+        if (l.tree == 0) ++nfirst;
+      }
+    }
+    host_leaves = (int64_t)final_leaves.size();
+    if ((int64_t)hitems.size() > item_cap) {
+      set_error("internal: work item capacity exceeded");
+      return DRPT_ERR_ARG;
+    }
+    const int64_t ni = (int64_t)hitems.size();
+    std::vector<int64_t> hr = {0, T > 1 ? nfirst : ni, T > 1 ? nfirst : ni, ni, 0, ni};
+    DRPT_HIP(hipMemcpyAsync(items, hitems.data(), hitems.size() * sizeof(LeafItem),
+                            hipMemcpyHostToDevice, s));
+    DRPT_HIP(hipMemcpyAsync(range, hr.data(), 6 * 8, hipMemcpyHostToDevice, s));
+    DRPT_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
   }
-
-  // work items, largest leaves first
-  std::stable_sort(final_leaves.begin(), final_leaves.end(),
-                   [](const Leaf& a, const Leaf& b) { return a.size > b.size; });
-  std::vector<LeafItem> items;
-  double gram = 0.0;
-  int64_t max_leaf = 0;
-  for (auto& l : final_leaves) {
-    gram += 2.0 * p->d * (double)l.size * (double)l.size;
-    max_leaf = std::max<int64_t>(max_leaf, l.size);
-    for (int r0 = 0; r0 < l.size; r0 += 64) items.push_back({l.tree, l.size, l.start, r0, 0});
-  }
-  LeafItem* ditems;
-  DRPT_CHECK(upload(ctx, "knn.items", items, &ditems));
 
   const void* Xq = nullptr;
   float *nb = nullptr, *er = nullptr;
@@ -398,17 +440,26 @@ int drpt_tree_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const uint
   const int64_t fb_cap = std::min<int64_t>((int64_t)T * n, (int64_t)1 << 22);
   DRPT_CHECK(ws(ctx, "knn.fallback", 2 * fb_cap, &fb));
   DRPT_CHECK(ws(ctx, "knn.nfallback", 1, &nfb));
+  DRPT_CHECK(fill_u32(ctx, (uint32_t*)nfb, 0u, 1));
   DRPT_HIP(hipEventRecord(ctx->ev[7], s));
   DRPT_HIP(hipEventRecord(ctx->ev[9], s));
-  DRPT_CHECK(launch_leaf_knn(ctx, &pp, X, perm_final, ditems, (int64_t)items.size(), Xq, nb, er,
-                             nullptr, cand_idx, cand_dist, fb, nfb, fb_cap));
-  DRPT_HIP(hipEventRecord(ctx->ev[8], s));
-  int32_t nfallback = 0;
-  if (pp.precision != DRPT_FP32) {
-    DRPT_CHECK(launch_fallback(ctx, &pp, X, perm_final, ditems, fb, nfb, fb_cap, cand_idx, cand_dist));
-    DRPT_HIP(hipMemcpyAsync(&nfallback, nfb, 4, hipMemcpyDeviceToHost, s));
+  if (two_phase) {
+    DRPT_CHECK(launch_leaf_knn(ctx, &pp, X, perm_final, items, range + 0, Xq, nb, er, cand_idx,
+                               cand_dist, fb, nfb, fb_cap, nullptr));
+    DRPT_CHECK(launch_leaf_knn(ctx, &pp, X, perm_final, items, range + 2, Xq, nb, er, cand_idx,
+                               cand_dist, fb, nfb, fb_cap, cand_dist));
+  } else {
+    DRPT_CHECK(launch_leaf_knn(ctx, &pp, X, perm_final, items, range + 4, Xq, nb, er, cand_idx,
+                               cand_dist, fb, nfb, fb_cap, nullptr));
   }
+  DRPT_HIP(hipEventRecord(ctx->ev[8], s));
+  if (pp.precision != DRPT_FP32)
+    DRPT_CHECK(launch_fallback(ctx, &pp, X, perm_final, items, fb, nfb, fb_cap, cand_idx, cand_dist));
   DRPT_HIP(hipEventRecord(ctx->ev[5], s));
+  unsigned long long st[4] = {0, 0, 0, 0};
+  int32_t nfallback = 0;
+  DRPT_HIP(hipMemcpyAsync(st, stats, sizeof(st), hipMemcpyDeviceToHost, s));
+  DRPT_HIP(hipMemcpyAsync(&nfallback, nfb, 4, hipMemcpyDeviceToHost, s));
   DRPT_HIP(hipStreamSynchronize(s));
   if (nfallback > fb_cap) {
     set_error("too many uncertified rows for the fallback list");
@@ -418,10 +469,16 @@ int drpt_tree_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const uint
     timings->group_ms = ms_between(ctx->ev[6], ctx->ev[7]);
     timings->knn_ms = ms_between(ctx->ev[7], ctx->ev[5]);
     timings->knn_kernel_ms = ms_between(ctx->ev[9], ctx->ev[8]);
-    timings->gram_flops = gram;
-    timings->n_leaves = (int64_t)final_leaves.size();
+    if (host_gram >= 0.0) {
+      timings->gram_flops = host_gram;
+      timings->n_leaves = host_leaves;
+      timings->max_leaf = host_max;
+    } else {
+      timings->gram_flops = 2.0 * p->d * (double)st[0];
+      timings->n_leaves = (int64_t)st[2];
+      timings->max_leaf = (int64_t)st[1];
+    }
     timings->n_fallback_rows = nfallback;
-    timings->max_leaf = max_leaf;
   }
   return DRPT_OK;
 }
@@ -474,7 +531,7 @@ int drpt_build_graph(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t
   DRPT_CHECK(grow_trees_impl(ctx, P, colmax, n, T, D, mode, p->leaf_max, heap));
   DRPT_HIP(hipEventRecord(ctx->ev[2], s));
   drpt_timings tk{};
-  DRPT_CHECK(drpt_tree_knn(ctx, p, X, heap, ci, cd, &tk));
+  DRPT_CHECK(tree_knn_impl(ctx, p, X, heap, ci, cd, &tk, true));
   DRPT_HIP(hipEventRecord(ctx->ev[3], s));
   DRPT_CHECK(launch_merge(ctx, n, T, K, ci, cd, out_idx, out_dist));
   DRPT_HIP(hipEventRecord(ctx->ev[4], s));

@@ -92,6 +92,124 @@ __global__ void k_apply_local(const int32_t* __restrict__ in, int32_t* __restric
   for (int k = threadIdx.x; k < m; k += blockDim.x) out[base + k] = in[base + tb[k]];
 }
 
+// Block-wide exclusive scan of two int64 sequences (1024 threads).
+__device__ __forceinline__ void block_scan2(long long& a, long long& b, long long& ta, long long& tb,
+                                            long long* sa, long long* sb) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  long long ia = a, ib = b;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const long long xa = __shfl_up(ia, off), xb = __shfl_up(ib, off);
+    if (lane >= off) {
+      ia += xa;
+      ib += xb;
+    }
+  }
+  if (lane == 63) {
+    sa[wave] = ia;
+    sb[wave] = ib;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long ca = 0, cb = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      const long long va = sa[w], vb = sb[w];
+      sa[w] = ca;
+      sb[w] = cb;
+      ca += va;
+      cb += vb;
+    }
+    sa[16] = ca;
+    sb[16] = cb;
+  }
+  __syncthreads();
+  a = sa[wave] + ia - a;
+  b = sb[wave] + ib - b;
+  ta = sa[16];
+  tb = sb[16];
+  __syncthreads();
+}
+
+// Per tree (one block): hoff[h] = first position of heap slot h's points in
+// perm[t] (slots in ascending order); tree totals of work items (64-row
+// blocks) and leaves; sum of m^2 and max m over leaves.
+__global__ void __launch_bounds__(1024)
+k_leaf_scan(const uint32_t* __restrict__ hcount, int HS, int64_t* __restrict__ hoff,
+            int64_t* __restrict__ tree_items, unsigned long long* __restrict__ stats) {
+  __shared__ long long sa[17], sb[17];
+  const int t = blockIdx.x;
+  const uint32_t* hc = hcount + (int64_t)t * HS;
+  int64_t* ho = hoff + (int64_t)t * HS;
+  long long cp = 0, ci = 0, leaves = 0;
+  unsigned long long m2 = 0, mx = 0;
+  for (int base = 0; base < HS; base += blockDim.x) {
+    const int h = base + threadIdx.x;
+    const long long c = h < HS ? (long long)hc[h] : 0;
+    long long a = c, b = (c + 63) / 64, ta, tb;
+    block_scan2(a, b, ta, tb, sa, sb);
+    if (h < HS) ho[h] = cp + a;
+    cp += ta;
+    ci += tb;
+    if (c > 0) {
+      ++leaves;
+      m2 += (unsigned long long)(c * c);
+      mx = max(mx, (unsigned long long)c);
+    }
+  }
+  if (threadIdx.x == 0) tree_items[t] = ci;
+  atomicAdd(&stats[0], m2);
+  atomicMax(&stats[1], mx);
+  atomicAdd(&stats[2], (unsigned long long)leaves);
+}
+
+// Item ranges: base[t] = sum of items of trees < t; range[] = {0, base[1],
+// base[1], base[T]} (tree 0 alone, then the others) and {0, base[T]} (all).
+__global__ void k_tree_bases(const int64_t* __restrict__ tree_items, int T, int64_t* __restrict__ base,
+                             int64_t* __restrict__ range) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  long long acc = 0;
+  for (int t = 0; t < T; ++t) {
+    base[t] = acc;
+    acc += tree_items[t];
+  }
+  base[T] = acc;
+  range[0] = 0;
+  range[1] = T > 1 ? base[1] : acc;
+  range[2] = T > 1 ? base[1] : acc;
+  range[3] = acc;
+  range[4] = 0;
+  range[5] = acc;
+}
+
+// Per tree (one block): the 64-row work items of every non-empty slot.
+__global__ void __launch_bounds__(1024)
+k_emit_items(const uint32_t* __restrict__ hcount, const int64_t* __restrict__ hoff, int HS,
+             const int64_t* __restrict__ base, LeafItem* __restrict__ items) {
+  __shared__ long long sa[17], sb[17];
+  const int t = blockIdx.x;
+  const uint32_t* hc = hcount + (int64_t)t * HS;
+  const int64_t* ho = hoff + (int64_t)t * HS;
+  long long ci = base[t];
+  for (int b0 = 0; b0 < HS; b0 += blockDim.x) {
+    const int h = b0 + threadIdx.x;
+    const long long c = h < HS ? (long long)hc[h] : 0;
+    long long a = (c + 63) / 64, z = 0, ta, tz;
+    block_scan2(a, z, ta, tz, sa, sb);
+    if (c > 0) {
+      LeafItem it;
+      it.tree = t;
+      it.size = (int32_t)c;
+      it.start = ho[h];
+      it.pad = 0;
+      for (int r0 = 0, k = 0; r0 < c; r0 += 64, ++k) {
+        it.row0 = r0;
+        items[ci + a + k] = it;
+      }
+    }
+    ci += ta;
+  }
+}
+
 unsigned grid_for(int64_t n) {
   int64_t b = (n + 1023) / 1024;
   if (b > 2048) b = 2048;
@@ -140,6 +258,25 @@ int launch_apply_local(drpt_ctx* ctx, const int32_t* perm_in, int32_t* perm_out,
   if (nseg <= 0) return DRPT_OK;
   hipLaunchKernelGGL(k_apply_local, dim3(nseg), dim3(256), 0, ctx->stream, perm_in, perm_out, n,
                      seg_start, seg_tree, seg_tbl, seg_size, tables);
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
+}
+
+}  // namespace drpt
+
+namespace drpt {
+
+int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, int64_t* hoff,
+                        int64_t* tree_items, int64_t* base, int64_t* range, LeafItem* items,
+                        unsigned long long* stats) {
+  const int HS = 1 << D;
+  hipLaunchKernelGGL(k_leaf_scan, dim3(T), dim3(1024), 0, ctx->stream, hcount, HS, hoff, tree_items,
+                     stats);
+  DRPT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_tree_bases, dim3(1), dim3(64), 0, ctx->stream, tree_items, T, base, range);
+  DRPT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_emit_items, dim3(T), dim3(1024), 0, ctx->stream, hcount, hoff, HS, base,
+                     items);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }

@@ -83,6 +83,12 @@ struct LeafItem {   // one work item: a block of up to 64 rows of one leaf
 int launch_count_leaves(drpt_ctx* ctx, const uint32_t* heap, int64_t n, int T, int D, uint32_t* hcount);
 int launch_scatter_leaves(drpt_ctx* ctx, const uint32_t* heap, int64_t n, int T, int D,
                           const int64_t* hoff, uint32_t* cursor, int32_t* perm);
+// device-side grouping for leaf-max builds: heap offsets, per-tree item
+// counts, item ranges (tree 0 / trees 1.. / all) and the work items;
+// stats[0] += sum m^2, stats[1] = max m, stats[2] += leaves
+int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, int64_t* hoff,
+                        int64_t* tree_items, int64_t* base, int64_t* range, LeafItem* items,
+                        unsigned long long* stats);
 int launch_sort_segments(drpt_ctx* ctx, int32_t* perm, int64_t n, const int64_t* seg_start,
                          const int32_t* seg_size, const int32_t* seg_tree, int nseg, int max_seg);
 int launch_apply_local(drpt_ctx* ctx, const int32_t* perm_in, int32_t* perm_out, int64_t n,
@@ -91,10 +97,12 @@ int launch_apply_local(drpt_ctx* ctx, const int32_t* perm_in, int32_t* perm_out,
 
 int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, int precision,
                      void* Xq, float* nrm2, float* err, float* scale);
+// items[range[0] .. range[1]) (range in device memory); fallback rows are
+// appended to fallback/nfallback (which the caller zeroes)
 int launch_leaf_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
-                    const LeafItem* items, int64_t nitems, const void* Xq, const float* nrm2,
-                    const float* err, const float* scale, int32_t* cand_idx, float* cand_dist,
-                    int32_t* fallback, int32_t* nfallback, int64_t fallback_cap);
+                    const LeafItem* items, const int64_t* range, const void* Xq, const float* nrm2,
+                    const float* err, int32_t* cand_idx, float* cand_dist, int32_t* fallback,
+                    int32_t* nfallback, int64_t fallback_cap, const float* bound);
 int launch_fallback(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
                     const LeafItem* items, const int32_t* fallback, const int32_t* nfallback,
                     int64_t fallback_cap, int32_t* cand_idx, float* cand_dist);

@@ -13,6 +13,8 @@
 // of |P[c][.]| (non-negative floats order like their bit patterns).
 //
 // Roofline: HBM-bound. Algorithmic bytes = n*d*sizeof(x) + 4*n*L.
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace drpt {
@@ -21,64 +23,73 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// TR rows per block; lane -> (row = lane % TR, column slot = lane / TR).
+// TR rows per tile; lane -> (row = lane % TR, column slot = lane / TR).
+// Grid-stride over row tiles; the per-column max |P| is kept in LDS and
+// flushed once per block.
 template <typename XT, int TR>
 __global__ void __launch_bounds__(kThreads)
 k_project(const XT* __restrict__ X, int64_t n, int d, int L, const int32_t* __restrict__ colptr,
           const uint16_t* __restrict__ rowidx, const float* __restrict__ w, float* __restrict__ P,
           uint32_t* __restrict__ colmax, int pitch) {
   extern __shared__ __align__(16) unsigned char smem[];
-  XT* tile = reinterpret_cast<XT*>(smem);
-  const int64_t row0 = (int64_t)blockIdx.x * TR;
-  const int rows = (int)min((int64_t)TR, n - row0);
-
-  // Stage rows [row0, row0 + rows) into LDS (row pitch padded against bank conflicts).
-  if (sizeof(XT) == 1 && (d & 3) == 0) {
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(X + row0 * d);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(tile);
-    const int wpr = d >> 2;        // words per row
-    const int wpitch = pitch >> 2;
-    const int total = rows * wpr;
-    for (int e = threadIdx.x; e < total; e += kThreads) {
-      int r = e / wpr, c = e - r * wpr;
-      dst[r * wpitch + c] = src[e];
-    }
-  } else {
-    const XT* src = X + row0 * d;
-    const int total = rows * d;
-    for (int e = threadIdx.x; e < total; e += kThreads) {
-      int r = e / d, c = e - r * d;
-      tile[r * pitch + c] = src[e];
-    }
-  }
-  __syncthreads();
-
+  uint32_t* lmax = reinterpret_cast<uint32_t*>(smem);
+  XT* tile = reinterpret_cast<XT*>(smem + (((size_t)L * 4 + 15) & ~(size_t)15));
+  for (int c = threadIdx.x; c < L; c += kThreads) lmax[c] = 0u;
   constexpr int kSlots = 64 / TR;  // columns a wave works on at once
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int r = lane % TR;
   const int slot = lane / TR;
-  const bool valid_row = r < rows;
-  for (int cb = wave * kSlots; cb < L; cb += 4 * kSlots) {
-    const int c = cb + slot;
-    float pv = 0.0f;
-    if (c < L) {
-      const int beg = colptr[c], end = colptr[c + 1];
-      double acc = 0.0;
-      const XT* xr = tile + r * pitch;
-      for (int k = beg; k < end; ++k) {
-        double x = (double)xr[rowidx[k]];
-        double prod = __dmul_rn(x, (double)w[k]);
-        acc = __dadd_rn(acc, prod);
+  const int64_t ntiles = (n + TR - 1) / TR;
+  for (int64_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
+    const int64_t row0 = tl * TR;
+    const int rows = (int)min((int64_t)TR, n - row0);
+    __syncthreads();
+    // Stage rows [row0, row0 + rows) into LDS (row pitch padded against bank conflicts).
+    if (sizeof(XT) == 1 && (d & 3) == 0) {
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(X + row0 * d);
+      uint32_t* dst = reinterpret_cast<uint32_t*>(tile);
+      const int wpr = d >> 2;  // words per row
+      const int wpitch = pitch >> 2;
+      const int total = rows * wpr;
+      for (int e = threadIdx.x; e < total; e += kThreads) {
+        int rr = e / wpr, c = e - rr * wpr;
+        dst[rr * wpitch + c] = src[e];
       }
-      pv = (float)acc;
-      if (valid_row) P[(int64_t)c * n + row0 + r] = pv;
+    } else {
+      const XT* src = X + row0 * d;
+      const int total = rows * d;
+      for (int e = threadIdx.x; e < total; e += kThreads) {
+        int rr = e / d, c = e - rr * d;
+        tile[rr * pitch + c] = src[e];
+      }
     }
-    uint32_t bits = (valid_row && c < L) ? (__float_as_uint(pv) & 0x7fffffffu) : 0u;
-    // reduce the max over the TR lanes sharing column c
-    for (int off = TR / 2; off > 0; off >>= 1) bits = max(bits, (uint32_t)__shfl_xor((int)bits, off));
-    if (r == 0 && c < L && bits) atomicMax(&colmax[c], bits);
+    __syncthreads();
+    const bool valid_row = r < rows;
+    for (int cb = wave * kSlots; cb < L; cb += 4 * kSlots) {
+      const int c = cb + slot;
+      float pv = 0.0f;
+      if (c < L) {
+        const int beg = colptr[c], end = colptr[c + 1];
+        double acc = 0.0;
+        const XT* xr = tile + r * pitch;
+        for (int k = beg; k < end; ++k) {
+          double x = (double)xr[rowidx[k]];
+          double prod = __dmul_rn(x, (double)w[k]);
+          acc = __dadd_rn(acc, prod);
+        }
+        pv = (float)acc;
+        if (valid_row) P[(int64_t)c * n + row0 + r] = pv;
+      }
+      uint32_t bits = (valid_row && c < L) ? (__float_as_uint(pv) & 0x7fffffffu) : 0u;
+      // reduce the max over the TR lanes sharing column c
+      for (int off = TR / 2; off > 0; off >>= 1) bits = max(bits, (uint32_t)__shfl_xor((int)bits, off));
+      if (r == 0 && c < L && bits) atomicMax(&lmax[c], bits);
+    }
   }
+  __syncthreads();
+  for (int c = threadIdx.x; c < L; c += kThreads)
+    if (lmax[c]) atomicMax(&colmax[c], lmax[c]);
 }
 
 template <typename XT, int TR>
@@ -92,8 +103,12 @@ int run(drpt_ctx* ctx, const XT* X, int64_t n, int d, const int32_t* colptr, con
   } else {
     pitch = d | 1;
   }
-  size_t lds = (size_t)TR * pitch * sizeof(XT);
-  int64_t blocks = (n + TR - 1) / TR;
+  size_t lds = (((size_t)L * 4 + 15) & ~(size_t)15) + (size_t)TR * pitch * sizeof(XT);
+  if (lds > 64 * 1024) {
+    set_error("project: LDS tile too large for this dimension / column count");
+    return DRPT_ERR_UNSUPPORTED;
+  }
+  int64_t blocks = std::min<int64_t>((n + TR - 1) / TR, (int64_t)ctx->num_cus * 8);
   hipLaunchKernelGGL((k_project<XT, TR>), dim3((unsigned)blocks), dim3(kThreads), lds, ctx->stream,
                      X, n, d, L, colptr, rowidx, w, P, colmax, pitch);
   DRPT_HIP(hipGetLastError());
@@ -106,7 +121,7 @@ int launch_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, co
                    const uint16_t* rowidx, const float* w, int L, float* P, uint32_t* colmax) {
   DRPT_CHECK(fill_u32(ctx, colmax, 0u, (size_t)L));
   if (n <= 0) return DRPT_OK;
-  const size_t kLdsBudget = 64 * 1024;
+  const size_t kLdsBudget = 64 * 1024 - (((size_t)L * 4 + 15) & ~(size_t)15);
   if (dtype == DRPT_U8) {
     return run<uint8_t, 64>(ctx, (const uint8_t*)X, n, d, colptr, rowidx, w, L, P, colmax);
   }
