@@ -98,7 +98,7 @@ int validate(const drpt_params* p) {
   return DRPT_OK;
 }
 
-int kprime_of(const drpt_params* p) { return p->kprime > 0 ? p->kprime : 2 * p->nn + 4; }
+int kprime_of(const drpt_params* p) { return p->kprime > 0 ? p->kprime : std::min(64, 2 * p->nn + 4); }
 
 float ms_between(hipEvent_t a, hipEvent_t b) {
   float ms = 0.0f;
@@ -265,9 +265,14 @@ int drpt_tree_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const uint
   return tree_knn_impl(ctx, p, X, leaf_heap, cand_idx, cand_dist, timings, false);
 }
 
-// prune: trees 1..T-1 skip leafmates that are provably worse than the K-th
-// entry of tree 0's list (build_graph only: the merged lists are unchanged,
-// the per-tree lists of trees >= 1 then hold the K best of the survivors).
+// Trees of the first phase of a pruned low-precision build; the K-th distance
+// of their merged lists bounds every later tree's candidates.
+constexpr int kFirstTrees = 4;
+
+// prune (build_graph only): trees >= kFirstTrees skip leafmates that provably
+// cannot enter the merged top K of the first kFirstTrees trees' lists; the
+// merged lists are unchanged, the per-tree lists of the later trees then hold
+// the K best of the survivors.
 static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
                          const uint32_t* leaf_heap, int32_t* cand_idx, float* cand_dist,
                          drpt_timings* timings, bool prune) {
@@ -302,13 +307,18 @@ static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
   DRPT_CHECK(fill_u32(ctx, (uint32_t*)stats, 0u, 8));
   DRPT_CHECK(launch_count_leaves(ctx, leaf_heap, n, T, D, hcount));
   const int32_t* perm_final = perm;
-  const bool two_phase = prune && p->precision != DRPT_FP32 && T > 1;
+  const bool lowp = p->precision != DRPT_FP32;
+  const bool two_phase = prune && lowp && T > kFirstTrees;
+  const int first_trees = two_phase ? kFirstTrees : T;
+  // FP32: blocks of 64 rows; low precision: whole leaves (one block stages the leaf)
+  const int rows_per_item = lowp ? (1 << 30) : 64;
   double host_gram = -1.0;
   int64_t host_leaves = 0, host_max = 0;
 
   if (p->local_mode == DRPT_LOCAL_LEAFMAX) {
     // everything on the device: offsets, work items, scatter
-    DRPT_CHECK(launch_device_items(ctx, hcount, T, D, hoff, tree_items, base, range, items, stats));
+    DRPT_CHECK(launch_device_items(ctx, hcount, T, D, rows_per_item, first_trees, hoff, tree_items,
+                                   base, range, items, stats));
     DRPT_CHECK(launch_scatter_leaves(ctx, leaf_heap, n, T, D, hoff, cursor, perm));
   } else {
     // reference local stage: the host derives the position tables, so the
@@ -402,9 +412,9 @@ static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
     for (auto& l : final_leaves) {
       host_gram += 2.0 * p->d * (double)l.size * (double)l.size;
       host_max = std::max<int64_t>(host_max, l.size);
-      for (int r0 = 0; r0 < l.size; r0 += 64) {
-        hitems.push_back({l.tree, l.size, l.start, r0, 0});
-        if (l.tree == 0) ++nfirst;
+      for (int64_t r0 = 0; r0 < l.size; r0 += rows_per_item) {
+        hitems.push_back({l.tree, l.size, l.start, (int32_t)r0, 0});
+        if (l.tree < first_trees) ++nfirst;
       }
     }
     host_leaves = (int64_t)final_leaves.size();
@@ -413,7 +423,7 @@ static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
       return DRPT_ERR_ARG;
     }
     const int64_t ni = (int64_t)hitems.size();
-    std::vector<int64_t> hr = {0, T > 1 ? nfirst : ni, T > 1 ? nfirst : ni, ni, 0, ni};
+    std::vector<int64_t> hr = {0, nfirst, nfirst, ni, 0, ni};
     DRPT_HIP(hipMemcpyAsync(items, hitems.data(), hitems.size() * sizeof(LeafItem),
                             hipMemcpyHostToDevice, s));
     DRPT_HIP(hipMemcpyAsync(range, hr.data(), 6 * 8, hipMemcpyHostToDevice, s));
@@ -444,16 +454,25 @@ static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
   DRPT_HIP(hipEventRecord(ctx->ev[7], s));
   DRPT_HIP(hipEventRecord(ctx->ev[9], s));
   if (two_phase) {
+    // phase 1: exact per-tree lists of the first trees, merged into a bound
+    int32_t* bi;
+    float* bd;
+    DRPT_CHECK(ws(ctx, "knn.bound_idx", n * K, &bi));
+    DRPT_CHECK(ws(ctx, "knn.bound_dist", n * K, &bd));
     DRPT_CHECK(launch_leaf_knn(ctx, &pp, X, perm_final, items, range + 0, Xq, nb, er, cand_idx,
                                cand_dist, fb, nfb, fb_cap, nullptr));
+    DRPT_CHECK(launch_fallback(ctx, &pp, X, perm_final, items, fb, nfb, fb_cap, cand_idx, cand_dist));
+    DRPT_CHECK(launch_merge(ctx, n, first_trees, K, cand_idx, cand_dist, bi, bd));
+    // phase 2: the other trees, filtered against the merged K-th distance
     DRPT_CHECK(launch_leaf_knn(ctx, &pp, X, perm_final, items, range + 2, Xq, nb, er, cand_idx,
-                               cand_dist, fb, nfb, fb_cap, cand_dist));
+                               cand_dist, fb, nfb, fb_cap, bd));
   } else {
     DRPT_CHECK(launch_leaf_knn(ctx, &pp, X, perm_final, items, range + 4, Xq, nb, er, cand_idx,
                                cand_dist, fb, nfb, fb_cap, nullptr));
   }
   DRPT_HIP(hipEventRecord(ctx->ev[8], s));
-  if (pp.precision != DRPT_FP32)
+  // rows whose candidate list overflowed (re-running phase-1 rows is idempotent)
+  if (lowp)
     DRPT_CHECK(launch_fallback(ctx, &pp, X, perm_final, items, fb, nfb, fb_cap, cand_idx, cand_dist));
   DRPT_HIP(hipEventRecord(ctx->ev[5], s));
   unsigned long long st[4] = {0, 0, 0, 0};
@@ -462,7 +481,7 @@ static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
   DRPT_HIP(hipMemcpyAsync(&nfallback, nfb, 4, hipMemcpyDeviceToHost, s));
   DRPT_HIP(hipStreamSynchronize(s));
   if (nfallback > fb_cap) {
-    set_error("too many uncertified rows for the fallback list");
+    set_error("too many candidate-list overflows for the fallback list");
     return DRPT_ERR_UNSUPPORTED;
   }
   if (timings) {

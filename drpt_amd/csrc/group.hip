@@ -131,10 +131,10 @@ __device__ __forceinline__ void block_scan2(long long& a, long long& b, long lon
 }
 
 // Per tree (one block): hoff[h] = first position of heap slot h's points in
-// perm[t] (slots in ascending order); tree totals of work items (64-row
-// blocks) and leaves; sum of m^2 and max m over leaves.
+// perm[t] (slots in ascending order); tree totals of work items (blocks of
+// rpi rows) and leaves; sum of m^2 and max m over leaves.
 __global__ void __launch_bounds__(1024)
-k_leaf_scan(const uint32_t* __restrict__ hcount, int HS, int64_t* __restrict__ hoff,
+k_leaf_scan(const uint32_t* __restrict__ hcount, int HS, int rpi, int64_t* __restrict__ hoff,
             int64_t* __restrict__ tree_items, unsigned long long* __restrict__ stats) {
   __shared__ long long sa[17], sb[17];
   const int t = blockIdx.x;
@@ -145,7 +145,7 @@ k_leaf_scan(const uint32_t* __restrict__ hcount, int HS, int64_t* __restrict__ h
   for (int base = 0; base < HS; base += blockDim.x) {
     const int h = base + threadIdx.x;
     const long long c = h < HS ? (long long)hc[h] : 0;
-    long long a = c, b = (c + 63) / 64, ta, tb;
+    long long a = c, b = (c + rpi - 1) / rpi, ta, tb;
     block_scan2(a, b, ta, tb, sa, sb);
     if (h < HS) ho[h] = cp + a;
     cp += ta;
@@ -162,10 +162,10 @@ k_leaf_scan(const uint32_t* __restrict__ hcount, int HS, int64_t* __restrict__ h
   atomicAdd(&stats[2], (unsigned long long)leaves);
 }
 
-// Item ranges: base[t] = sum of items of trees < t; range[] = {0, base[1],
-// base[1], base[T]} (tree 0 alone, then the others) and {0, base[T]} (all).
-__global__ void k_tree_bases(const int64_t* __restrict__ tree_items, int T, int64_t* __restrict__ base,
-                             int64_t* __restrict__ range) {
+// Item ranges: base[t] = sum of items of trees < t; range[] = {0, base[a],
+// base[a], base[T]} (trees < a, then the others) and {0, base[T]} (all).
+__global__ void k_tree_bases(const int64_t* __restrict__ tree_items, int T, int a,
+                             int64_t* __restrict__ base, int64_t* __restrict__ range) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   long long acc = 0;
   for (int t = 0; t < T; ++t) {
@@ -174,16 +174,16 @@ __global__ void k_tree_bases(const int64_t* __restrict__ tree_items, int T, int6
   }
   base[T] = acc;
   range[0] = 0;
-  range[1] = T > 1 ? base[1] : acc;
-  range[2] = T > 1 ? base[1] : acc;
+  range[1] = base[a];
+  range[2] = base[a];
   range[3] = acc;
   range[4] = 0;
   range[5] = acc;
 }
 
-// Per tree (one block): the 64-row work items of every non-empty slot.
+// Per tree (one block): the work items (blocks of rpi rows) of every non-empty slot.
 __global__ void __launch_bounds__(1024)
-k_emit_items(const uint32_t* __restrict__ hcount, const int64_t* __restrict__ hoff, int HS,
+k_emit_items(const uint32_t* __restrict__ hcount, const int64_t* __restrict__ hoff, int HS, int rpi,
              const int64_t* __restrict__ base, LeafItem* __restrict__ items) {
   __shared__ long long sa[17], sb[17];
   const int t = blockIdx.x;
@@ -193,7 +193,7 @@ k_emit_items(const uint32_t* __restrict__ hcount, const int64_t* __restrict__ ho
   for (int b0 = 0; b0 < HS; b0 += blockDim.x) {
     const int h = b0 + threadIdx.x;
     const long long c = h < HS ? (long long)hc[h] : 0;
-    long long a = (c + 63) / 64, z = 0, ta, tz;
+    long long a = (c + rpi - 1) / rpi, z = 0, ta, tz;
     block_scan2(a, z, ta, tz, sa, sb);
     if (c > 0) {
       LeafItem it;
@@ -201,8 +201,8 @@ k_emit_items(const uint32_t* __restrict__ hcount, const int64_t* __restrict__ ho
       it.size = (int32_t)c;
       it.start = ho[h];
       it.pad = 0;
-      for (int r0 = 0, k = 0; r0 < c; r0 += 64, ++k) {
-        it.row0 = r0;
+      for (long long r0 = 0, k = 0; r0 < c; r0 += rpi, ++k) {
+        it.row0 = (int32_t)r0;
         items[ci + a + k] = it;
       }
     }
@@ -266,17 +266,22 @@ int launch_apply_local(drpt_ctx* ctx, const int32_t* perm_in, int32_t* perm_out,
 
 namespace drpt {
 
-int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, int64_t* hoff,
-                        int64_t* tree_items, int64_t* base, int64_t* range, LeafItem* items,
-                        unsigned long long* stats) {
+int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, int rows_per_item,
+                        int first_trees, int64_t* hoff, int64_t* tree_items, int64_t* base,
+                        int64_t* range, LeafItem* items, unsigned long long* stats) {
   const int HS = 1 << D;
-  hipLaunchKernelGGL(k_leaf_scan, dim3(T), dim3(1024), 0, ctx->stream, hcount, HS, hoff, tree_items,
-                     stats);
+  if (first_trees < 0 || first_trees > T || rows_per_item < 1) {
+    set_error("internal: invalid work item layout");
+    return DRPT_ERR_ARG;
+  }
+  hipLaunchKernelGGL(k_leaf_scan, dim3(T), dim3(1024), 0, ctx->stream, hcount, HS, rows_per_item,
+                     hoff, tree_items, stats);
   DRPT_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_tree_bases, dim3(1), dim3(64), 0, ctx->stream, tree_items, T, base, range);
+  hipLaunchKernelGGL(k_tree_bases, dim3(1), dim3(64), 0, ctx->stream, tree_items, T, first_trees,
+                     base, range);
   DRPT_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_emit_items, dim3(T), dim3(1024), 0, ctx->stream, hcount, hoff, HS, base,
-                     items);
+  hipLaunchKernelGGL(k_emit_items, dim3(T), dim3(1024), 0, ctx->stream, hcount, hoff, HS,
+                     rows_per_item, base, items);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }

@@ -72,7 +72,7 @@ int grow_trees_impl(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64
                     int mode, int leaf_max, uint32_t* node);
 
 // group.hip / leafknn.hip
-struct LeafItem {   // one work item: a block of up to 64 rows of one leaf
+struct LeafItem {   // one work item: a block of rows of one leaf (or the whole leaf)
   int32_t tree;
   int32_t size;     // leaf size m
   int64_t start;    // offset of the leaf in perm[tree]
@@ -84,11 +84,12 @@ int launch_count_leaves(drpt_ctx* ctx, const uint32_t* heap, int64_t n, int T, i
 int launch_scatter_leaves(drpt_ctx* ctx, const uint32_t* heap, int64_t n, int T, int D,
                           const int64_t* hoff, uint32_t* cursor, int32_t* perm);
 // device-side grouping for leaf-max builds: heap offsets, per-tree item
-// counts, item ranges (tree 0 / trees 1.. / all) and the work items;
-// stats[0] += sum m^2, stats[1] = max m, stats[2] += leaves
-int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, int64_t* hoff,
-                        int64_t* tree_items, int64_t* base, int64_t* range, LeafItem* items,
-                        unsigned long long* stats);
+// counts, item ranges (trees < first_trees / the others / all) and the work
+// items (blocks of rows_per_item rows); stats[0] += sum m^2, stats[1] = max m,
+// stats[2] += leaves
+int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, int rows_per_item,
+                        int first_trees, int64_t* hoff, int64_t* tree_items, int64_t* base,
+                        int64_t* range, LeafItem* items, unsigned long long* stats);
 int launch_sort_segments(drpt_ctx* ctx, int32_t* perm, int64_t n, const int64_t* seg_start,
                          const int32_t* seg_size, const int32_t* seg_tree, int nseg, int max_seg);
 int launch_apply_local(drpt_ctx* ctx, const int32_t* perm_in, int32_t* perm_out, int64_t n,

@@ -319,22 +319,42 @@ k_knn_exact(const XT* __restrict__ X, int d, const int32_t* __restrict__ perm, i
 }
 
 // ---------------------------------------------------------------------------
-// BF16 candidate path (MFMA) + exact re-rank + certification.
+// Low-precision path: BF16 Gram on the MFMA cores, exact FP32 distances for a
+// filtered candidate set.
 //
-// One work item = 64 rows of one leaf. LDS holds, in turn: the bf16 row and
-// column tiles and the 64x64 approximate-distance tile (Gram phase), then a
-// chunk of dimensions of every point of the leaf as FP32 (re-rank). Each wave
-// keeps the candidate queues of 16 rows. Candidates are leaf positions (ties
-// broken by position: the certification below depends only on the (k'+1)-th
-// approximate value, so the choice among tied candidates cannot change the
-// result).
+// One work item = one whole leaf (its point ids and |x~|^2 staged in LDS),
+// processed in blocks of 64 query rows. For a block, the Gram tile against 64
+// leafmates is computed with the leafmates as the MFMA's A operand and the
+// query rows as B, so each lane owns one query row (l % 32) and holds 16
+// leafmates of it per tile; four threads share a row, each seeing a fixed
+// quarter of the columns. Approximate distance a = |x~_i|^2 + |x~_j|^2 - 2 G_ij.
+//
+// Lower bound. With e = |x~ - x| (rounded up), eta the FP32 error bound of a
+// and c = (d + 8) 2^-24, every leafmate j satisfies
+//   D(i, j) >= LB(a_ij) = (max(0, sqrt(max(0, a_ij - eta)) - e_i - e_max))^2 (1 - c)
+// so for an upper bound B of the K-th exact distance that row i needs,
+//   LB(a_ij) <= B  <=>  a_ij <= A_cut(B) = (sqrt(B / (1 - c)) + e_i + e_max)^2 + eta
+// and every leafmate with D(i, j) <= B (in particular every member of the
+// row's exact top K) passes the filter a_ij <= A_cut(B).
+//
+// Bound pass (own bound, used for per-tree lists): each thread keeps the
+// ceil(k'/4) smallest a of its quarter as packed keys (a's FP32 bits with the
+// low 11 bits replaced by the leaf position, compared as integers), so the row
+// gets k' widened candidates; their exact D are computed and B = K-th smallest
+// of them (an upper bound of the row's K-th exact distance within the leaf).
+// Filter pass: leafmates with a <= A_cut(B) that are not already candidates
+// are appended to the row's list (capacity kCap), their exact D computed, and
+// the K best by (D, index) written out. A row whose list overflows is redone
+// by k_knn_fallback. With an external bound (the K-th distance of a list merged
+// over other trees) only the filter pass runs: the per-tree list then holds
+// the K best of the leafmates that can still enter the merged top K, which
+// leaves the merged result unchanged.
 // ---------------------------------------------------------------------------
-constexpr int kGramChunk = 128;                  // dims per Gram pass
-constexpr int kTilePitch = kGramChunk + 8;       // bf16 elements per LDS row
-constexpr int kBigBytes = 2 * kRows * kTilePitch * 2 + kRows * (kCols + 1) * 4;
-constexpr int kMaxStagedLeaf = kBigBytes / 4;    // leaves above this go to the exact fallback
-
-constexpr int kLeafStage = 2048;  // leaves above this go to the exact fallback
+constexpr int kGramChunk = 128;             // dims per staged Gram chunk
+constexpr int kTilePitch = kGramChunk + 8;  // bf16 elements per LDS row (16-byte aligned rows)
+constexpr int kLeafStage = 2048;            // leaves above this go to the exact fallback
+constexpr int kCap = 64;                    // candidate slots per query row
+constexpr uint16_t kNoPos = 0xFFFF;
 
 // 8 consecutive bf16 of row id from dim k (zeros past d, or for id < 0)
 __device__ __forceinline__ uint4 load_piece(const uint16_t* __restrict__ Xb, int id, int d, int k) {
@@ -347,60 +367,178 @@ __device__ __forceinline__ uint4 load_piece(const uint16_t* __restrict__ Xb, int
   return make_uint4(t[0] | (t[1] << 16), t[2] | (t[3] << 16), t[4] | (t[5] << 16), t[6] | (t[7] << 16));
 }
 
+__device__ __forceinline__ float4 load4f(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ float4 load4f(const uint8_t* p) {
+  const uchar4 b = *reinterpret_cast<const uchar4*>(p);
+  return make_float4((float)b.x, (float)b.y, (float)b.z, (float)b.w);
+}
+
+// exact D (same fma chain as exact_dist), vector loads when rows are 4-aligned
 template <typename XT>
+__device__ __forceinline__ float exact_dist4(const XT* __restrict__ X, int64_t a, int64_t b, int d) {
+  if ((d & 3) != 0) return exact_dist(X, a, b, d);
+  const XT* xa = X + a * d;
+  const XT* xb = X + b * d;
+  float acc = 0.0f;
+#pragma unroll 4
+  for (int k = 0; k < d; k += 4) {
+    const float4 u = load4f(xa + k), v = load4f(xb + k);
+    float t = __fsub_rn(u.x, v.x);
+    acc = fmaf(t, t, acc);
+    t = __fsub_rn(u.y, v.y);
+    acc = fmaf(t, t, acc);
+    t = __fsub_rn(u.z, v.z);
+    acc = fmaf(t, t, acc);
+    t = __fsub_rn(u.w, v.w);
+    acc = fmaf(t, t, acc);
+  }
+  return acc;
+}
+
+// A_cut(B) of the header comment, rounded up to float (+inf when B is not finite)
+__device__ __forceinline__ float acut_of(float B, float nb_i, float e_i, double nm, float emax,
+                                         int d) {
+  if (!(B < INFINITY)) return INFINITY;
+  const double u = 5.9604644775390625e-08;  // 2^-24
+  const double ni = sqrt((double)nb_i * (1.0 + (d + 4.0) * u));
+  const double eta = (2.0 * d + 16.0) * u * (ni + nm) * (ni + nm);
+  const double sb = sqrt((double)B / (1.0 - (d + 8.0) * u)) + (double)e_i + (double)emax;
+  return (float)((sb * sb + eta) * (1.0 + 1e-6) + 1e-30);
+}
+
+struct LowpShared {
+  uint16_t As[kRows][kTilePitch];   // query rows, bf16 (one chunk of dims)
+  uint16_t Bs[kCols][kTilePitch];   // leafmate tile, bf16
+  int lid[kLeafStage];              // point id of every leaf position
+  float lnb[kLeafStage];            // |x~|^2 of every leaf position
+  uint16_t cpos[kRows][kCap];       // candidate leaf positions per query row
+  float cdist[kRows][kCap];         // their exact distances
+  int ccnt[kRows];
+  float cacut[kRows];
+  int cpre[kRows + 1];
+  float red_n[kThreads / 64], red_e[kThreads / 64];
+  int ticket[2];
+};
+
+// One pass of Gram tiles of the query block [row0, row0 + 64) against every
+// leafmate tile; epi(c0, acc) sees each thread's 16 accumulators.
+template <typename Epi>
+__device__ __forceinline__ void gram_pass(LowpShared& S, const uint16_t* __restrict__ Xb, int d,
+                                          int m, int row0, int rows, bool a_resident, Epi&& epi) {
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int qh = (wave & 1) * 32, ch = (wave >> 1) * 32;
+  const int ksteps_res = (d + 15) / 16;
+  uint4 rb[4];
+  if (a_resident) {
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) {
+      const int c = tid + kThreads * s4;
+      const int r = c >> 4, part = c & 15;
+      rb[s4] = load_piece(Xb, r < m ? S.lid[r] : -1, d, part * 8);
+    }
+  }
+  for (int c0 = 0; c0 < m; c0 += kCols) {
+    floatx16 acc;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+    if (a_resident) {
+      __syncthreads();  // previous tile's operand reads are done
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        const int c = tid + kThreads * s4;
+        *reinterpret_cast<uint4*>(&S.Bs[c >> 4][(c & 15) * 8]) = rb[s4];
+      }
+      __syncthreads();
+      if (c0 + kCols < m) {  // next leafmate tile in flight during this one
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          const int c = tid + kThreads * s4;
+          const int r = c >> 4, part = c & 15;
+          const int col = c0 + kCols + r;
+          rb[s4] = load_piece(Xb, col < m ? S.lid[col] : -1, d, part * 8);
+        }
+      }
+      for (int ks = 0; ks < ksteps_res; ++ks) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&S.Bs[ch + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(&S.As[qh + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+      }
+    } else {
+      for (int k0 = 0; k0 < d; k0 += kGramChunk) {
+        __syncthreads();
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          const int c = tid + kThreads * s4;
+          const int r = c >> 4, part = c & 15;
+          const int col = c0 + r;
+          const uint4 va = load_piece(Xb, r < rows ? S.lid[row0 + r] : -1, d, k0 + part * 8);
+          const uint4 vb = load_piece(Xb, col < m ? S.lid[col] : -1, d, k0 + part * 8);
+          *reinterpret_cast<uint4*>(&S.As[r][part * 8]) = va;
+          *reinterpret_cast<uint4*>(&S.Bs[r][part * 8]) = vb;
+        }
+        __syncthreads();
+        const int ksteps = min(kGramChunk, d - k0 + 15) / 16;
+        for (int ks = 0; ks < ksteps; ++ks) {
+          const bf16x8 a = *reinterpret_cast<const bf16x8*>(&S.Bs[ch + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
+          const bf16x8 b = *reinterpret_cast<const bf16x8*>(&S.As[qh + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+      }
+    }
+    epi(c0, acc);
+  }
+}
+
+template <typename XT, int T>
 __global__ void __launch_bounds__(kThreads, 2)
-k_knn_bf16(const XT* __restrict__ X, const uint16_t* __restrict__ Xb,
+k_knn_lowp(const XT* __restrict__ X, const uint16_t* __restrict__ Xb,
            const float* __restrict__ nb, const float* __restrict__ err, int d,
            const int32_t* __restrict__ perm, int64_t n, const LeafItem* __restrict__ items,
-           const int64_t* __restrict__ range, int K, int kp, int32_t* __restrict__ cand_idx,
+           const int64_t* __restrict__ range, int K, int32_t* __restrict__ cand_idx,
            float* __restrict__ cand_dist, unsigned* __restrict__ queue,
            int32_t* __restrict__ fallback, int32_t* __restrict__ nfallback, int64_t fb_cap,
            const float* __restrict__ bound) {
-  __shared__ __align__(16) unsigned char big[kBigBytes];
-  uint16_t(*As)[kTilePitch] = reinterpret_cast<uint16_t(*)[kTilePitch]>(big);
-  uint16_t(*Bs)[kTilePitch] = reinterpret_cast<uint16_t(*)[kTilePitch]>(big + kRows * kTilePitch * 2);
-  float(*Ds)[kCols + 1] = reinterpret_cast<float(*)[kCols + 1]>(big + 2 * kRows * kTilePitch * 2);
-  float* Ls = reinterpret_cast<float*>(big);
-  __shared__ int lid[kLeafStage];     // point id of every leaf position
-  __shared__ float lnb[kLeafStage];   // |x~|^2 of every leaf position
-  __shared__ float red_n[kThreads / 64], red_e[kThreads / 64];
-  __shared__ int s_ticket[2];
+  constexpr int KP = 4 * T;  // widened candidates per row in the bound pass
+  __shared__ __align__(16) LowpShared S;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const bool own_bound = bound == nullptr;
   const bool a_resident = d <= kGramChunk;
-  float qd[kRowsPerWave];
-  int qi[kRowsPerWave];
+  // this thread's query row within the block and quarter of the columns
+  const int q = (wave & 1) * 32 + (lane & 31);
+  const int sub = (wave >> 1) * 2 + (lane >> 5);
+  const int cq = (wave >> 1) * 32 + 4 * (lane >> 5);  // + 8 * (e / 4) + e % 4
+  const int base_slot = own_bound ? KP : 0;
   const int64_t begin = range[0], end = range[1];
-  if (tid == 0) s_ticket[0] = (int)atomicAdd(queue, 1u);
+  if (tid == 0) S.ticket[0] = (int)atomicAdd(queue, 1u);
   __syncthreads();
-  int64_t item = begin + s_ticket[0];
+  int64_t item = begin + S.ticket[0];
   for (int parity = 1;; parity ^= 1) {
     if (item >= end) break;
-    if (tid == 0) s_ticket[parity] = (int)atomicAdd(queue, 1u);  // next item, fetched early
+    if (tid == 0) S.ticket[parity] = (int)atomicAdd(queue, 1u);  // next item, fetched early
     const LeafItem it = items[item];
     const int32_t* pl = perm + (int64_t)it.tree * n + it.start;
     const int m = it.size;
-    const int rows = min(kRows, m - it.row0);
-    if (m > kLeafStage) {  // too large to stage: exact path for these rows
-      if (tid < rows) {
+    if (m > kLeafStage) {  // too large to stage: exact path for every row
+      for (int r = tid; r < m; r += kThreads) {
         const int slot = atomicAdd(nfallback, 1);
         if (slot < fb_cap) {
           fallback[2 * slot] = (int32_t)item;
-          fallback[2 * slot + 1] = tid;
+          fallback[2 * slot + 1] = r;
         }
       }
       __syncthreads();
-      item = begin + s_ticket[parity];
+      item = begin + S.ticket[parity];
       continue;
     }
-    // stage ids and norms of the whole leaf; leaf maxima for the bound
+    // stage ids and norms of the leaf; leaf maxima for the bound
     float mn = 0.0f, me_ = 0.0f;
     for (int j = tid; j < m; j += kThreads) {
       const int id = pl[j];
       const float v = nb[id];
-      lid[j] = id;
-      lnb[j] = v;
+      S.lid[j] = id;
+      S.lnb[j] = v;
       mn = fmaxf(mn, v);
       me_ = fmaxf(me_, err[id]);
     }
@@ -410,234 +548,180 @@ k_knn_bf16(const XT* __restrict__ X, const uint16_t* __restrict__ Xb,
       me_ = fmaxf(me_, __shfl_xor(me_, off));
     }
     if (lane == 0) {
-      red_n[wave] = mn;
-      red_e[wave] = me_;
+      S.red_n[wave] = mn;
+      S.red_e[wave] = me_;
     }
     __syncthreads();
-    float nmax = red_n[0], emax = red_e[0];
+    float nmax = S.red_n[0], emax = S.red_e[0];
 #pragma unroll
     for (int w = 1; w < kThreads / 64; ++w) {
-      nmax = fmaxf(nmax, red_n[w]);
-      emax = fmaxf(emax, red_e[w]);
+      nmax = fmaxf(nmax, S.red_n[w]);
+      emax = fmaxf(emax, S.red_e[w]);
     }
-    // per-row inputs of the bound, one row per lane (rows wave*16 .. +15)
-    float b_l = INFINITY, e_l = 0.0f, n_l = 0.0f;
-    {
-      const int r = wave * kRowsPerWave + lane;
-      if (lane < kRowsPerWave && r < rows) {
-        const int id = lid[it.row0 + r];
-        n_l = lnb[it.row0 + r];
-        e_l = err[id];
-        if (bound != nullptr) b_l = bound[(int64_t)id * K + (K - 1)];
-      }
-    }
-#pragma unroll
-    for (int rr = 0; rr < kRowsPerWave; ++rr) {
-      qd[rr] = INFINITY;
-      qi[rr] = INT_MAX;
-    }
-    // first tile's operands in flight while the rest is set up
-    uint4 ra[4], rb[4];
-#pragma unroll
-    for (int s4 = 0; s4 < 4; ++s4) {
-      const int c = tid + kThreads * s4;
-      const int r = c >> 4, part = c & 15;
-      ra[s4] = load_piece(Xb, r < rows ? lid[it.row0 + r] : -1, d, part * 8);
-      rb[s4] = load_piece(Xb, r < m ? lid[r] : -1, d, part * 8);
-    }
-    // Pruning against a known K-th exact distance B(i) (from another tree's
-    // list): a leafmate whose certified lower bound on D exceeds B(i) cannot
-    // enter point i's final K, so it is skipped. LB(a) <= B  <=>  a <= A_cut
-    // with A_cut = (sqrt(B/(1-c)) + e_i + e_max)^2 + eta (the bound used below).
-    const double u = 5.9604644775390625e-08;  // 2^-24
-    const double infl = 1.0 + (d + 4.0) * u;
-    const double nm = sqrt((double)nmax * infl);
-    float acut[kRowsPerWave];
-#pragma unroll
-    for (int rr = 0; rr < kRowsPerWave; ++rr) {
-      acut[rr] = INFINITY;
-      const double b = (double)lane_read(b_l, rr);
-      if (b < INFINITY) {
-        const double ni = sqrt((double)lane_read(n_l, rr) * infl);
-        const double eta = (2.0 * d + 16.0) * u * (ni + nm) * (ni + nm);
-        const double sb = sqrt(b / (1.0 - (d + 8.0) * u)) + (double)lane_read(e_l, rr) + (double)emax;
-        acut[rr] = (float)((sb * sb + eta) * (1.0 + 1e-6) + 1e-30);
-      }
-    }
-    for (int c0 = 0; c0 < m; c0 += kCols) {
-      const int cols = min(kCols, m - c0);
-      floatx16 acc;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
-      if (a_resident) {
+    const double nm = sqrt((double)nmax * (1.0 + (d + 4.0) * 5.9604644775390625e-08));
+
+    for (int row0 = 0; row0 < m; row0 += kRows) {
+      const int rows = min(kRows, m - row0);
+      const bool qv = q < rows;
+      const int qpos = row0 + q;
+      const float rn = qv ? S.lnb[qpos] : 0.0f;
+      if (a_resident) {  // query rows, all dims (visible after gram_pass's first barrier)
 #pragma unroll
         for (int s4 = 0; s4 < 4; ++s4) {
           const int c = tid + kThreads * s4;
           const int r = c >> 4, part = c & 15;
-          if (c0 == 0) *reinterpret_cast<uint4*>(&As[r][part * 8]) = ra[s4];
-          *reinterpret_cast<uint4*>(&Bs[r][part * 8]) = rb[s4];
+          *reinterpret_cast<uint4*>(&S.As[r][part * 8]) =
+              load_piece(Xb, r < rows ? S.lid[row0 + r] : -1, d, part * 8);
+        }
+      }
+      uint32_t key[T];
+#pragma unroll
+      for (int s = 0; s < T; ++s) key[s] = 0xFFFFFFFFu;
+      if (own_bound) {
+        // bound pass: per-thread T smallest packed keys
+        gram_pass(S, Xb, d, m, row0, rows, a_resident, [&](int c0, const floatx16& acc) {
+#pragma unroll
+          for (int g = 0; g < 4; ++g) {
+            const int cb = c0 + cq + 8 * g;
+            const float4 cn4 = *reinterpret_cast<const float4*>(&S.lnb[cb]);
+            const float cn[4] = {cn4.x, cn4.y, cn4.z, cn4.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int c = cb + u;
+              const float a = fmaf(-2.0f, acc[4 * g + u], __fadd_rn(rn, cn[u]));
+              uint32_t kk = (c < m && c != qpos)
+                                ? ((__float_as_uint(a > 0.0f ? a : 0.0f) & 0xFFFFF800u) | (uint32_t)c)
+                                : 0xFFFFFFFFu;
+              if (kk < key[T - 1]) {
+#pragma unroll
+                for (int s = 0; s < T; ++s) {
+                  const uint32_t lo = min(kk, key[s]);
+                  kk = max(kk, key[s]);
+                  key[s] = lo;
+                }
+              }
+            }
+          }
+        });
+        if (qv) {
+#pragma unroll
+          for (int s = 0; s < T; ++s)
+            S.cpos[q][sub * T + s] = key[s] == 0xFFFFFFFFu ? kNoPos : (uint16_t)(key[s] & 0x7FFu);
         }
         __syncthreads();
-        if (c0 + kCols < m) {  // next column tile in flight during this one
-#pragma unroll
-          for (int s4 = 0; s4 < 4; ++s4) {
-            const int c = tid + kThreads * s4;
-            const int r = c >> 4, part = c & 15;
-            const int col = c0 + kCols + r;
-            rb[s4] = load_piece(Xb, col < m ? lid[col] : -1, d, part * 8);
-          }
+        for (int p = tid; p < rows * KP; p += kThreads) {
+          const int r = p / KP, s = p - r * KP;
+          const uint16_t pos = S.cpos[r][s];
+          S.cdist[r][s] = pos == kNoPos ? INFINITY : exact_dist4(X, S.lid[row0 + r], S.lid[pos], d);
         }
-        const int ksteps = (d + 15) / 16;
-        for (int ks = 0; ks < ksteps; ++ks) {
-          const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wr + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
-          const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wc + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-        }
-      } else {
-        for (int k0 = 0; k0 < d; k0 += kGramChunk) {
-#pragma unroll
-          for (int s4 = 0; s4 < 4; ++s4) {
-            const int c = tid + kThreads * s4;
-            const int r = c >> 4, part = c & 15;
-            const int col = c0 + r;
-            const uint4 va = load_piece(Xb, r < rows ? lid[it.row0 + r] : -1, d, k0 + part * 8);
-            const uint4 vb = load_piece(Xb, col < m ? lid[col] : -1, d, k0 + part * 8);
-            *reinterpret_cast<uint4*>(&As[r][part * 8]) = va;
-            *reinterpret_cast<uint4*>(&Bs[r][part * 8]) = vb;
-          }
-          __syncthreads();
-          const int ksteps = min(kGramChunk, d - k0 + 15) / 16;
-          for (int ks = 0; ks < ksteps; ++ks) {
-            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wr + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
-            const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wc + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-          }
-          __syncthreads();
-        }
-      }
-      // 32x32 accumulator layout: element e of lane l is
-      // row 8*(e/4) + 4*(l/32) + e%4, column l%32.
-#pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int r = wr + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-        const int c = wc + (lane & 31);
-        const float rn = r < rows ? lnb[it.row0 + r] : 0.0f;
-        const float cn = c < cols ? lnb[c0 + c] : 0.0f;
-        Ds[r][c] = __fsub_rn(__fadd_rn(rn, cn), __fmul_rn(2.0f, acc[e]));
-      }
-      __syncthreads();
-      {
-        const int pos = c0 + lane;
-#pragma unroll
+        __syncthreads();
+        // B = K-th smallest exact distance of the row's KP candidates
+#pragma unroll 1
         for (int rr = 0; rr < kRowsPerWave; ++rr) {
           const int r = wave * kRowsPerWave + rr;
-          if (r < rows) {
-            const float a = Ds[r][lane];
-            const bool ok = lane < cols && pos != it.row0 + r && a <= acut[rr];
-            wq_offer(qd[rr], qi[rr], ok ? a : INFINITY, ok ? pos : INT_MAX, kp, lane);
+          if (r >= rows) break;
+          const uint16_t pos = lane < KP ? S.cpos[r][lane] : kNoPos;
+          const float dv = pos == kNoPos ? INFINITY : S.cdist[r][lane];
+          const int iv = pos == kNoPos ? INT_MAX : S.lid[pos];
+          int rank = 0;
+          for (int s = 0; s < KP; ++s) rank += better(lane_read(dv, s), lane_read(iv, s), dv, iv) ? 1 : 0;
+          const uint64_t hit = __ballot(iv != INT_MAX && rank == K - 1);
+          const float B = hit ? lane_read(dv, __builtin_ctzll(hit)) : INFINITY;
+          if (lane == 0) {
+            const int id = S.lid[row0 + r];
+            S.cacut[r] = acut_of(B, S.lnb[row0 + r], err[id], nm, emax, d);
+            S.ccnt[r] = KP;
           }
+        }
+      } else {
+        if (tid < rows) {
+          const int id = S.lid[row0 + tid];
+          S.cacut[tid] = acut_of(bound[(int64_t)id * K + (K - 1)], S.lnb[row0 + tid], err[id], nm,
+                                 emax, d);
+          S.ccnt[tid] = 0;
         }
       }
       __syncthreads();
-    }
-    // candidates (lanes < k') and the (k'+1)-th approximate value per row
-    int cpos[kRowsPerWave];
-    float thr[kRowsPerWave], racc[kRowsPerWave];
+      // filter pass: append every leafmate that can enter the row's top K
+      const float acut = qv ? S.cacut[q] : -INFINITY;
+      gram_pass(S, Xb, d, m, row0, rows, a_resident, [&](int c0, const floatx16& acc) {
 #pragma unroll
-    for (int rr = 0; rr < kRowsPerWave; ++rr) {
-      thr[rr] = lane_read(qd[rr], kp);
-      if (lane_read(qi[rr], kp) == INT_MAX) thr[rr] = INFINITY;
-      cpos[rr] = (lane < kp && qi[rr] != INT_MAX) ? qi[rr] : -1;
-      racc[rr] = 0.0f;
-    }
-    // exact re-rank: stage chunks of dims of every leaf point in LDS as FP32;
-    // lane l of row r accumulates D(row, candidate l), dims ascending
-    int RK = min(d, kBigBytes / 4 / m);
-    if (RK >= 4) RK &= ~3;
-    const bool vec = (RK & 3) == 0 && (d & 3) == 0;
-    for (int k0 = 0; k0 < d; k0 += RK) {
-      const int w = min(RK, d - k0);
-      const bool vw = vec && (w & 3) == 0;
-      for (int pos = tid; pos < m; pos += kThreads) {
-        const XT* src = X + (int64_t)lid[pos] * d + k0;
-        float* dst = Ls + pos * w;
-        if (vw) {
-          for (int q = 0; q < w; q += 4) {
-            float4 v;
-            if (sizeof(XT) == 4) {
-              v = *reinterpret_cast<const float4*>(src + q);
-            } else {
-              const uchar4 b = *reinterpret_cast<const uchar4*>(src + q);
-              v = make_float4((float)b.x, (float)b.y, (float)b.z, (float)b.w);
+        for (int g = 0; g < 4; ++g) {
+          const int cb = c0 + cq + 8 * g;
+          const float4 cn4 = *reinterpret_cast<const float4*>(&S.lnb[cb]);
+          const float cn[4] = {cn4.x, cn4.y, cn4.z, cn4.w};
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int c = cb + u;
+            const float a = fmaf(-2.0f, acc[4 * g + u], __fadd_rn(rn, cn[u]));
+            if (a <= acut && c < m && c != qpos) {
+              bool dup = false;
+#pragma unroll
+              for (int s = 0; s < T; ++s)
+                dup |= key[s] != 0xFFFFFFFFu && (key[s] & 0x7FFu) == (uint32_t)c;
+              if (!dup) {
+                const int slot = atomicAdd(&S.ccnt[q], 1);
+                if (slot < kCap) S.cpos[q][slot] = (uint16_t)c;
+              }
             }
-            *reinterpret_cast<float4*>(dst + q) = v;
           }
-        } else {
-          for (int q = 0; q < w; ++q) dst[q] = (float)src[q];
         }
+      });
+      __syncthreads();
+      // exact D of the appended candidates, flattened over the block's rows
+      if (wave == 0) {
+        int c = lane < rows ? min(S.ccnt[lane], kCap) - base_slot : 0;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const int v = __shfl_up(c, off);
+          if (lane >= off) c += v;
+        }
+        S.cpre[lane + 1] = c;
+        if (lane == 0) S.cpre[0] = 0;
       }
       __syncthreads();
-#pragma unroll
+      const int total = S.cpre[rows];
+      for (int p = tid; p < total; p += kThreads) {
+        int lo = 0, hi = rows;  // largest r with cpre[r] <= p
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (S.cpre[mid] <= p) lo = mid; else hi = mid;
+        }
+        const int s = base_slot + p - S.cpre[lo];
+        S.cdist[lo][s] = exact_dist4(X, S.lid[row0 + lo], S.lid[S.cpos[lo][s]], d);
+      }
+      __syncthreads();
+      // the K best of each row's list, or the exact fallback on overflow
+#pragma unroll 1
       for (int rr = 0; rr < kRowsPerWave; ++rr) {
         const int r = wave * kRowsPerWave + rr;
-        if (r < rows && cpos[rr] >= 0) {
-          const int rp = it.row0 + r;
-          if (vw) {
-            const float4* L4 = reinterpret_cast<const float4*>(Ls);
-            const int w4 = w >> 2;
-            for (int kq = 0; kq < w4; ++kq) {
-              const float4 a = L4[rp * w4 + kq];
-              const float4 b = L4[cpos[rr] * w4 + kq];
-              float t = __fsub_rn(a.x, b.x);
-              racc[rr] = fmaf(t, t, racc[rr]);
-              t = __fsub_rn(a.y, b.y);
-              racc[rr] = fmaf(t, t, racc[rr]);
-              t = __fsub_rn(a.z, b.z);
-              racc[rr] = fmaf(t, t, racc[rr]);
-              t = __fsub_rn(a.w, b.w);
-              racc[rr] = fmaf(t, t, racc[rr]);
-            }
-          } else {
-            for (int kk = 0; kk < w; ++kk) {
-              const float t = __fsub_rn(Ls[rp * w + kk], Ls[cpos[rr] * w + kk]);
-              racc[rr] = fmaf(t, t, racc[rr]);
-            }
-          }
-        }
-      }
-      __syncthreads();
-    }
-#pragma unroll
-    for (int rr = 0; rr < kRowsPerWave; ++rr) {
-      const int r = wave * kRowsPerWave + rr;
-      if (r < rows) {
-        const float dv = cpos[rr] >= 0 ? racc[rr] : INFINITY;
-        const int iv = cpos[rr] >= 0 ? lid[cpos[rr]] : INT_MAX;
-        const int me = lid[it.row0 + r];
-        const int64_t o = ((int64_t)it.tree * n + me) * K;
-        const float dk = rank_store(dv, iv, kp, K, lane, cand_idx + o, cand_dist + o);
-        if (lane == 0 && thr[rr] < INFINITY) {
-          const double ni = sqrt((double)lnb[it.row0 + r] * infl);
-          const double eta = (2.0 * d + 16.0) * u * (ni + nm) * (ni + nm);
-          const double rt = sqrt(fmax(0.0, (double)thr[rr] - eta));
-          const double rl = fmax(0.0, rt - ((double)err[me] + (double)emax));
-          const double lb = rl * rl * (1.0 - (d + 8.0) * u);
-          if (!((double)dk < lb)) {
+        if (r >= rows) break;
+        const int cnt = S.ccnt[r];
+        const int me = S.lid[row0 + r];
+        if (cnt > kCap) {
+          if (lane == 0) {
             const int slot = atomicAdd(nfallback, 1);
             if (slot < fb_cap) {
               fallback[2 * slot] = (int32_t)item;
-              fallback[2 * slot + 1] = r;
+              fallback[2 * slot + 1] = row0 + r;
             }
           }
+          continue;
         }
+        const uint16_t pos = lane < cnt ? S.cpos[r][lane] : kNoPos;
+        const float dv = pos == kNoPos ? INFINITY : S.cdist[r][lane];
+        const int iv = pos == kNoPos ? INT_MAX : S.lid[pos];
+        const int64_t o = ((int64_t)it.tree * n + me) * K;
+        rank_store(dv, iv, cnt, K, lane, cand_idx + o, cand_dist + o);
       }
+      __syncthreads();
     }
-    __syncthreads();
-    item = begin + s_ticket[parity];
+    item = begin + S.ticket[parity];
   }
 }
 
-// Exact redo of uncertified rows: one block per row, all leafmates.
+// Exact redo of single rows (candidate-list overflow, leaves too large to
+// stage): one block per row, all leafmates.
 constexpr int kFbThreads = 128;
 
 template <typename XT, int KP>
@@ -721,6 +805,20 @@ int persistent_grid(drpt_ctx* ctx, T kernel, int threads, unsigned* grid) {
   return DRPT_OK;
 }
 
+template <typename XT, int T>
+int launch_lowp(drpt_ctx* ctx, const XT* X, const void* Xq, const float* nrm2, const float* err,
+                int d, const int32_t* perm, int64_t n, const LeafItem* items, const int64_t* range,
+                int K, int32_t* ci, float* cd, unsigned* q, int32_t* fb, int32_t* nfb, int64_t cap,
+                const float* bound) {
+  unsigned grid;
+  DRPT_CHECK(persistent_grid(ctx, k_knn_lowp<XT, T>, kThreads, &grid));
+  hipLaunchKernelGGL((k_knn_lowp<XT, T>), dim3(grid), dim3(kThreads), 0, ctx->stream, X,
+                     (const uint16_t*)Xq, nrm2, err, d, perm, n, items, range, K, ci, cd, q, fb,
+                     nfb, cap, bound);
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
+}
+
 template <typename XT>
 int dispatch(drpt_ctx* ctx, const drpt_params* p, const XT* X, const int32_t* perm,
              const LeafItem* items, const int64_t* range, const void* Xq, const float* nrm2,
@@ -742,16 +840,17 @@ int dispatch(drpt_ctx* ctx, const drpt_params* p, const XT* X, const int32_t* pe
   }
   if (p->precision == DRPT_BF16) {
     const int kp = p->kprime;
-    if (kp < K || kp > 63) {
-      set_error("leaf_knn: kprime must be in [nn, 63] for BF16");
+    if (kp < K || kp > 4 * 16) {
+      set_error("leaf_knn: kprime must be in [nn, 64] for BF16");
       return DRPT_ERR_UNSUPPORTED;
     }
-    DRPT_CHECK(persistent_grid(ctx, k_knn_bf16<XT>, kThreads, &grid));
-    hipLaunchKernelGGL((k_knn_bf16<XT>), dim3(grid), dim3(kThreads), 0, ctx->stream, X,
-                       (const uint16_t*)Xq, nrm2, err, d, perm, n, items, range, K, kp, ci, cd, q,
-                       fb, nfb, cap, bound);
-    DRPT_HIP(hipGetLastError());
-    return DRPT_OK;
+    const int t = (kp + 3) / 4;  // per-thread slots of the bound pass (4 threads per row)
+    if (t <= 3) return launch_lowp<XT, 3>(ctx, X, Xq, nrm2, err, d, perm, n, items, range, K, ci, cd, q, fb, nfb, cap, bound);
+    if (t <= 4) return launch_lowp<XT, 4>(ctx, X, Xq, nrm2, err, d, perm, n, items, range, K, ci, cd, q, fb, nfb, cap, bound);
+    if (t <= 6) return launch_lowp<XT, 6>(ctx, X, Xq, nrm2, err, d, perm, n, items, range, K, ci, cd, q, fb, nfb, cap, bound);
+    if (t <= 8) return launch_lowp<XT, 8>(ctx, X, Xq, nrm2, err, d, perm, n, items, range, K, ci, cd, q, fb, nfb, cap, bound);
+    if (t <= 12) return launch_lowp<XT, 12>(ctx, X, Xq, nrm2, err, d, perm, n, items, range, K, ci, cd, q, fb, nfb, cap, bound);
+    return launch_lowp<XT, 16>(ctx, X, Xq, nrm2, err, d, perm, n, items, range, K, ci, cd, q, fb, nfb, cap, bound);
   }
   set_error("leaf_knn: precision not supported");
   return DRPT_ERR_UNSUPPORTED;

@@ -63,7 +63,7 @@ typedef struct {
   double gram_flops;       /* algorithmic Gram flops, 2*d*sum(m^2) over final leaves */
   double knn_kernel_ms;    /* leaf k-NN kernel alone */
   int64_t n_leaves;        /* final leaves over all trees */
-  int64_t n_fallback_rows; /* low-precision rows re-done exactly (certification failed) */
+  int64_t n_fallback_rows; /* low-precision rows re-done exactly (candidate-list overflow, leaf > 2048) */
   int64_t max_leaf;        /* largest final leaf */
   int64_t reserved[4];
 } drpt_timings;

@@ -92,24 +92,72 @@ def test_tree_knn_fp32_bitexact(ctx, orc, case, K):
         assert np.array_equal(_bits(cd[t]), _bits(rd))
 
 
-@pytest.mark.parametrize("case", ["int128_f32_4000", "gauss_d37_1500", "mnist_u8_3000",
-                                  "gauss960_800"])
-def test_bf16_lists_equal_fp32(ctx, case):
-    _log(f"[bf16==fp32] {case}")
+@pytest.mark.parametrize("case,K,leaf_max,kprime", [("int128_f32_4000", 10, 150, 0),
+                                                    ("int128_f32_4000", 10, 700, 12),
+                                                    ("gauss_d37_1500", 7, 150, 0),
+                                                    ("mnist_u8_3000", 16, 300, 0),
+                                                    ("gauss960_800", 32, 150, 0)])
+def test_tree_knn_bf16_bitexact(ctx, orc, case, K, leaf_max, kprime):
+    """Per-tree BF16 lists (bound pass + filter pass, exact re-rank) equal the
+    oracle's exact per-leaf lists bit for bit."""
+    _log(f"[tree_knn bf16] {case} K={K} leaf_max={leaf_max} kprime={kprime}")
+    X = CASES[case]()
+    n, d = X.shape
+    p = drpt_amd.leafmax_params(n, d, 2, K, leaf_max=leaf_max, precision="bf16", kprime=kprime,
+                                seed=3)
+    p.dtype = drpt_amd.DRPT_U8 if X.dtype == np.uint8 else drpt_amd.DRPT_F32
+    R = orc.build_R(d, p.ntrees * p.depth, p.density, p.seed)
+    P0, cm0 = orc.project(X, R)
+    heaps = np.stack([orc.grow_tree(P0, cm0, p.depth, t, 1, p.leaf_max, 0) for t in range(2)])
+    ci, cd, tm = ctx.tree_knn(p, _dev(X), _dev(heaps.view(np.int32)))
+    ci, cd = ci.cpu().numpy(), cd.cpu().numpy()
+    _log(f"  fallback rows {tm.n_fallback_rows} of {2 * n}, largest leaf {tm.max_leaf}")
+    for t in range(2):
+        perm, offs = _groups(heaps[t])
+        ri, rd = orc.leaf_knn(X, perm, offs, K)
+        assert np.array_equal(ci[t], ri)
+        assert np.array_equal(_bits(cd[t]), _bits(rd))
+
+
+@pytest.mark.parametrize("case,T", [("int128_f32_4000", 4), ("int128_f32_4000", 7),
+                                    ("gauss_d37_1500", 6), ("mnist_u8_3000", 6),
+                                    ("gauss960_800", 5)])
+def test_bf16_lists_equal_fp32(ctx, case, T):
+    """Whole build: BF16 (trees beyond the first four pruned against the merged
+    bound) returns the FP32 build's lists."""
+    _log(f"[bf16==fp32] {case} T={T}")
     X = CASES[case]()
     n, d = X.shape
     out = {}
     for prec in ("fp32", "bf16"):
-        p = drpt_amd.leafmax_params(n, d, 4, 10, leaf_max=200, precision=prec, seed=5)
+        p = drpt_amd.leafmax_params(n, d, T, 10, leaf_max=200, precision=prec, seed=5)
         p.dtype = drpt_amd.DRPT_U8 if X.dtype == np.uint8 else drpt_amd.DRPT_F32
         oi, od, tm = ctx.build_graph(p, _dev(X))
         out[prec] = (oi.cpu().numpy(), od.cpu().numpy(), tm)
     assert np.array_equal(out["fp32"][0], out["bf16"][0])
     assert np.array_equal(_bits(out["fp32"][1]), _bits(out["bf16"][1]))
     tm = out["bf16"][2]
-    _log(f"  fallback rows {tm.n_fallback_rows} of {n * 4}")
-    if X.dtype == np.uint8 or case.startswith("int"):
-        assert tm.n_fallback_rows <= n * 4 // 100  # integer data: bf16 Gram nearly exact
+    _log(f"  fallback rows {tm.n_fallback_rows} of {n * T}")
+    assert tm.n_fallback_rows <= n * T // 100
+
+
+def test_bf16_duplicates_overflow(ctx, orc):
+    """80 copies of each vector: candidate lists overflow (ties at distance 0)
+    and those rows go through the exact fallback; lists still match."""
+    _log("[edge] bf16 duplicate overflow")
+    rng = np.random.default_rng(4)
+    base = rng.integers(0, 50, size=(20, 16)).astype(np.float32)
+    X = np.repeat(base, 80, axis=0)
+    n, d = X.shape
+    for T in (3, 6):
+        p = drpt_amd.leafmax_params(n, d, T, 10, leaf_max=300, precision="bf16", seed=8)
+        oi, od, tm = ctx.build_graph(p, _dev(X))
+        ri, rd, _, _ = orc.build_graph(X, T, p.depth, 10, local_mode=1, leaf_max=300, seed=8,
+                                       density=p.density)
+        _log(f"  T={T} fallback rows {tm.n_fallback_rows}")
+        assert tm.n_fallback_rows > 0
+        assert np.array_equal(oi.cpu().numpy(), ri)
+        assert np.array_equal(_bits(od.cpu().numpy()), _bits(rd))
 
 
 def test_merge_bitexact(ctx, orc):
