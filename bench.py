@@ -114,6 +114,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (scale for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -122,6 +125,15 @@ def main():
 
     import torch
     import drpt_amd
+
+    if a.copy_probe > 0:
+        src = torch.ones(a.copy_probe << 18, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 4, "copies": 3}))
+        return
 
     w = workload(a.config)
     prec = a.precision or ("fp32" if a.config == "c1" else "bf16")

@@ -256,38 +256,25 @@ int drpt_grow_trees(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64
   return grow_trees_impl(ctx, P, colmax, n, ntrees, depth, tree_mode, leaf_max, leaf_heap);
 }
 
-static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
-                         const uint32_t* leaf_heap, int32_t* cand_idx, float* cand_dist,
-                         drpt_timings* timings, bool prune);
+// Final leaves as work items: the points of every leaf contiguous in G->perm
+// (tree-major) and items of rows_per_item rows; items of trees < first_trees
+// come first. Leaf-max builds do this on the device; the reference local stage
+// derives its position tables on the host.
+struct Grouping {
+  const int32_t* perm = nullptr;
+  LeafItem* items = nullptr;
+  int64_t* range = nullptr;  // device {0, first, first, all, 0, all}
+  unsigned long long* stats = nullptr;
+  double host_gram = -1.0;
+  int64_t host_leaves = 0, host_max = 0;
+};
 
-int drpt_tree_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const uint32_t* leaf_heap,
-                  int32_t* cand_idx, float* cand_dist, drpt_timings* timings) {
-  return tree_knn_impl(ctx, p, X, leaf_heap, cand_idx, cand_dist, timings, false);
-}
-
-// Trees of the first phase of a pruned low-precision build; the K-th distance
-// of their merged lists bounds every later tree's candidates.
-constexpr int kFirstTrees = 4;
-
-// prune (build_graph only): trees >= kFirstTrees skip leafmates that provably
-// cannot enter the merged top K of the first kFirstTrees trees' lists; the
-// merged lists are unchanged, the per-tree lists of the later trees then hold
-// the K best of the survivors.
-static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
-                         const uint32_t* leaf_heap, int32_t* cand_idx, float* cand_dist,
-                         drpt_timings* timings, bool prune) {
-  if (!ctx || !X || !leaf_heap || !cand_idx || !cand_dist) {
-    set_error("tree_knn: null pointer");
-    return DRPT_ERR_ARG;
-  }
-  DRPT_CHECK(validate(p));
-  DRPT_HIP(hipSetDevice(ctx->device));
+static int group_impl(drpt_ctx* ctx, const drpt_params* p, const uint32_t* leaf_heap,
+                      int rows_per_item, int first_trees, Grouping* G) {
   const int T = p->ntrees, D = p->depth, K = p->nn;
   const int64_t n = p->n;
   const int64_t HS = (int64_t)1 << D;
   hipStream_t s = ctx->stream;
-  DRPT_HIP(hipEventRecord(ctx->ev[6], s));
-
   // leaf sizes by heap index
   uint32_t *hcount, *cursor;
   int64_t *hoff, *tree_items, *base, *range;
@@ -307,11 +294,6 @@ static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
   DRPT_CHECK(fill_u32(ctx, (uint32_t*)stats, 0u, 8));
   DRPT_CHECK(launch_count_leaves(ctx, leaf_heap, n, T, D, hcount));
   const int32_t* perm_final = perm;
-  const bool lowp = p->precision != DRPT_FP32;
-  const bool two_phase = prune && lowp && T > kFirstTrees;
-  const int first_trees = two_phase ? kFirstTrees : T;
-  // FP32: blocks of 64 rows; low precision: whole leaves (one block stages the leaf)
-  const int rows_per_item = lowp ? (1 << 30) : 64;
   double host_gram = -1.0;
   int64_t host_leaves = 0, host_max = 0;
 
@@ -430,56 +412,121 @@ static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
     DRPT_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
   }
 
-  const void* Xq = nullptr;
-  float *nb = nullptr, *er = nullptr;
+  G->perm = perm_final;
+  G->items = items;
+  G->range = range;
+  G->stats = stats;
+  G->host_gram = host_gram;
+  G->host_leaves = host_leaves;
+  G->host_max = host_max;
+  return DRPT_OK;
+}
+
+// leaf statistics of a grouping into timings (reads the device counters: syncs)
+static int leaf_stats(drpt_ctx* ctx, const drpt_params* p, const Grouping& G, drpt_timings* tm) {
+  unsigned long long st[4] = {0, 0, 0, 0};
+  DRPT_HIP(hipMemcpyAsync(st, G.stats, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+  DRPT_HIP(hipStreamSynchronize(ctx->stream));
+  if (!tm) return DRPT_OK;
+  if (G.host_gram >= 0.0) {
+    tm->gram_flops = G.host_gram;
+    tm->n_leaves = G.host_leaves;
+    tm->max_leaf = G.host_max;
+  } else {
+    tm->gram_flops = 2.0 * p->d * (double)st[0];
+    tm->n_leaves = (int64_t)st[2];
+    tm->max_leaf = (int64_t)st[1];
+  }
+  return DRPT_OK;
+}
+
+// bf16 copy, |x~|^2 and |x~ - x| per point, then their leaf-ordered copies
+static int prep_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
+                     KnnArgs* a) {
+  const int64_t n = p->n;
+  uint16_t* xb;
+  float *nb, *er, *lnbT, *lerrT;
+  DRPT_CHECK(ws(ctx, "lowp.x", n * p->d, &xb));
+  DRPT_CHECK(ws(ctx, "lowp.nrm2", n, &nb));
+  DRPT_CHECK(ws(ctx, "lowp.err", n, &er));
+  DRPT_CHECK(launch_prep_lowp(ctx, X, p->dtype, n, p->d, p->precision, xb, nb, er, nullptr));
+  DRPT_CHECK(ws(ctx, "lowp.lnbT", (size_t)p->ntrees * n, &lnbT));
+  DRPT_CHECK(ws(ctx, "lowp.lerrT", (size_t)p->ntrees * n, &lerrT));
+  DRPT_CHECK(launch_leaf_gather(ctx, perm, (int64_t)p->ntrees * n, nb, er, lnbT, lerrT));
+  a->X = X;
+  a->Xb = xb;
+  a->lnbT = lnbT;
+  a->lerrT = lerrT;
+  a->perm = perm;
+  a->n = n;
+  a->d = p->d;
+  a->K = p->nn;
+  return DRPT_OK;
+}
+
+static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
+                         const uint32_t* leaf_heap, int32_t* cand_idx, float* cand_dist,
+                         drpt_timings* timings);
+
+int drpt_tree_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const uint32_t* leaf_heap,
+                  int32_t* cand_idx, float* cand_dist, drpt_timings* timings) {
+  return tree_knn_impl(ctx, p, X, leaf_heap, cand_idx, cand_dist, timings);
+}
+
+// Exact per-tree lists (FP32: every pair; BF16: bound pass + filter pass).
+static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
+                         const uint32_t* leaf_heap, int32_t* cand_idx, float* cand_dist,
+                         drpt_timings* timings) {
+  if (!ctx || !X || !leaf_heap || !cand_idx || !cand_dist) {
+    set_error("tree_knn: null pointer");
+    return DRPT_ERR_ARG;
+  }
+  DRPT_CHECK(validate(p));
+  DRPT_HIP(hipSetDevice(ctx->device));
+  const int T = p->ntrees;
+  const int64_t n = p->n;
+  hipStream_t s = ctx->stream;
+  DRPT_HIP(hipEventRecord(ctx->ev[6], s));
+  Grouping G;
+  DRPT_CHECK(group_impl(ctx, p, leaf_heap, 64, T, &G));
   drpt_params pp = *p;
   pp.kprime = kprime_of(p);
-  if (pp.precision != DRPT_FP32) {
-    if (pp.kprime < K) {
+  int32_t *fb = nullptr, *nfb = nullptr;
+  const int64_t fb_cap = std::min<int64_t>((int64_t)T * n, (int64_t)1 << 22);
+  if (pp.precision == DRPT_FP32) {
+    DRPT_HIP(hipEventRecord(ctx->ev[7], s));
+    DRPT_HIP(hipEventRecord(ctx->ev[9], s));
+    DRPT_CHECK(launch_leaf_knn(ctx, &pp, X, G.perm, G.items, G.range + 4, cand_idx, cand_dist));
+    DRPT_HIP(hipEventRecord(ctx->ev[8], s));
+  } else {
+    if (pp.kprime < p->nn) {
       set_error("kprime must be >= nn");
       return DRPT_ERR_ARG;
     }
-    uint16_t* xb;
-    DRPT_CHECK(ws(ctx, "lowp.x", n * p->d, &xb));
-    DRPT_CHECK(ws(ctx, "lowp.nrm2", n, &nb));
-    DRPT_CHECK(ws(ctx, "lowp.err", n, &er));
-    DRPT_CHECK(launch_prep_lowp(ctx, X, p->dtype, n, p->d, pp.precision, xb, nb, er, nullptr));
-    Xq = xb;
+    KnnArgs a{};
+    DRPT_CHECK(prep_lowp(ctx, &pp, X, G.perm, &a));
+    DRPT_CHECK(ws(ctx, "knn.fallback", 2 * fb_cap, &fb));
+    DRPT_CHECK(ws(ctx, "knn.nfallback", 1, &nfb));
+    DRPT_CHECK(fill_u32(ctx, (uint32_t*)nfb, 0u, 1));
+    a.items = G.items;
+    a.range = G.range + 4;
+    a.rows_per_item = 64;
+    a.out_idx = cand_idx;
+    a.out_dist = cand_dist;
+    a.fallback = fb;
+    a.nfallback = nfb;
+    a.fb_cap = fb_cap;
+    DRPT_HIP(hipEventRecord(ctx->ev[7], s));
+    DRPT_HIP(hipEventRecord(ctx->ev[9], s));
+    DRPT_CHECK(launch_lowp_knn(ctx, &pp, kModeTree, a));
+    DRPT_HIP(hipEventRecord(ctx->ev[8], s));
+    // rows whose candidate list overflowed, or of leaves too large to stage
+    DRPT_CHECK(launch_fallback(ctx, &pp, X, G.perm, G.items, fb, nfb, fb_cap, cand_idx, cand_dist));
   }
-  int32_t *fb, *nfb;
-  const int64_t fb_cap = std::min<int64_t>((int64_t)T * n, (int64_t)1 << 22);
-  DRPT_CHECK(ws(ctx, "knn.fallback", 2 * fb_cap, &fb));
-  DRPT_CHECK(ws(ctx, "knn.nfallback", 1, &nfb));
-  DRPT_CHECK(fill_u32(ctx, (uint32_t*)nfb, 0u, 1));
-  DRPT_HIP(hipEventRecord(ctx->ev[7], s));
-  DRPT_HIP(hipEventRecord(ctx->ev[9], s));
-  if (two_phase) {
-    // phase 1: exact per-tree lists of the first trees, merged into a bound
-    int32_t* bi;
-    float* bd;
-    DRPT_CHECK(ws(ctx, "knn.bound_idx", n * K, &bi));
-    DRPT_CHECK(ws(ctx, "knn.bound_dist", n * K, &bd));
-    DRPT_CHECK(launch_leaf_knn(ctx, &pp, X, perm_final, items, range + 0, Xq, nb, er, cand_idx,
-                               cand_dist, fb, nfb, fb_cap, nullptr));
-    DRPT_CHECK(launch_fallback(ctx, &pp, X, perm_final, items, fb, nfb, fb_cap, cand_idx, cand_dist));
-    DRPT_CHECK(launch_merge(ctx, n, first_trees, K, cand_idx, cand_dist, bi, bd));
-    // phase 2: the other trees, filtered against the merged K-th distance
-    DRPT_CHECK(launch_leaf_knn(ctx, &pp, X, perm_final, items, range + 2, Xq, nb, er, cand_idx,
-                               cand_dist, fb, nfb, fb_cap, bd));
-  } else {
-    DRPT_CHECK(launch_leaf_knn(ctx, &pp, X, perm_final, items, range + 4, Xq, nb, er, cand_idx,
-                               cand_dist, fb, nfb, fb_cap, nullptr));
-  }
-  DRPT_HIP(hipEventRecord(ctx->ev[8], s));
-  // rows whose candidate list overflowed (re-running phase-1 rows is idempotent)
-  if (lowp)
-    DRPT_CHECK(launch_fallback(ctx, &pp, X, perm_final, items, fb, nfb, fb_cap, cand_idx, cand_dist));
   DRPT_HIP(hipEventRecord(ctx->ev[5], s));
-  unsigned long long st[4] = {0, 0, 0, 0};
   int32_t nfallback = 0;
-  DRPT_HIP(hipMemcpyAsync(st, stats, sizeof(st), hipMemcpyDeviceToHost, s));
-  DRPT_HIP(hipMemcpyAsync(&nfallback, nfb, 4, hipMemcpyDeviceToHost, s));
-  DRPT_HIP(hipStreamSynchronize(s));
+  if (nfb) DRPT_HIP(hipMemcpyAsync(&nfallback, nfb, 4, hipMemcpyDeviceToHost, s));
+  DRPT_CHECK(leaf_stats(ctx, p, G, timings));
   if (nfallback > fb_cap) {
     set_error("too many candidate-list overflows for the fallback list");
     return DRPT_ERR_UNSUPPORTED;
@@ -488,16 +535,123 @@ static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
     timings->group_ms = ms_between(ctx->ev[6], ctx->ev[7]);
     timings->knn_ms = ms_between(ctx->ev[7], ctx->ev[5]);
     timings->knn_kernel_ms = ms_between(ctx->ev[9], ctx->ev[8]);
-    if (host_gram >= 0.0) {
-      timings->gram_flops = host_gram;
-      timings->n_leaves = host_leaves;
-      timings->max_leaf = host_max;
-    } else {
-      timings->gram_flops = 2.0 * p->d * (double)st[0];
-      timings->n_leaves = (int64_t)st[2];
-      timings->max_leaf = (int64_t)st[1];
-    }
     timings->n_fallback_rows = nfallback;
+  }
+  return DRPT_OK;
+}
+
+// Trees whose merged widened candidates bound the K-th distance of every row.
+constexpr int kFirstTrees = 4;
+// Pool capacity per row (entries beyond it send the row to the exact row path).
+constexpr int kPoolCap = 256;
+
+// Low-precision graph (build_graph): the final lists straight from the leaves.
+//  1. kModeCand on trees < a: k' widened candidates per row and tree
+//  2. row merge: distinct candidates, exact D, K best -> known list; its K-th
+//     distance B(i) bounds the K-th distance of the final list
+//  3. kModePool on every tree: leafmates with LB <= B(i) not in the known
+//     list go to the row's pool (every member of the final top K passes)
+//  4. row merge of pool and known list -> final lists; rows whose pool
+//     overflowed are redone exactly over every tree
+static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
+                          const uint32_t* leaf_heap, int32_t* out_idx, float* out_dist,
+                          drpt_timings* timings) {
+  const int T = p->ntrees, K = p->nn;
+  const int64_t n = p->n;
+  hipStream_t s = ctx->stream;
+  drpt_params pp = *p;
+  pp.kprime = kprime_of(p);
+  if (pp.kprime < K) {
+    set_error("kprime must be >= nn");
+    return DRPT_ERR_ARG;
+  }
+  const int KP = lowp_slots(&pp);
+  const int first = std::max(1, std::min(std::min(T, kFirstTrees), 256 / KP));
+  const int rows_per_item = 1 << 30;  // one item per leaf
+  DRPT_HIP(hipEventRecord(ctx->ev[6], s));
+  Grouping G;
+  DRPT_CHECK(group_impl(ctx, p, leaf_heap, rows_per_item, first, &G));
+  KnnArgs a{};
+  DRPT_CHECK(prep_lowp(ctx, &pp, X, G.perm, &a));
+  int32_t *s1, *ki, *pool, *pcnt, *oflag, *olist, *novf;
+  float *kd, *boundT;
+  DRPT_CHECK(ws(ctx, "knn.s1", (size_t)first * n * KP, &s1));
+  DRPT_CHECK(ws(ctx, "knn.known_idx", (size_t)n * K, &ki));
+  DRPT_CHECK(ws(ctx, "knn.known_dist", (size_t)n * K, &kd));
+  DRPT_CHECK(ws(ctx, "knn.boundT", (size_t)T * n, &boundT));
+  DRPT_CHECK(ws(ctx, "knn.pool", (size_t)n * kPoolCap, &pool));
+  DRPT_CHECK(ws(ctx, "knn.pool_cnt", n, &pcnt));
+  DRPT_CHECK(ws(ctx, "knn.ovf_flag", n, &oflag));
+  DRPT_CHECK(ws(ctx, "knn.ovf_list", n, &olist));
+  DRPT_CHECK(ws(ctx, "knn.novf", 1, &novf));
+  DRPT_CHECK(fill_u32(ctx, (uint32_t*)pcnt, 0u, n));
+  DRPT_CHECK(fill_u32(ctx, (uint32_t*)oflag, 0u, n));
+  DRPT_CHECK(fill_u32(ctx, (uint32_t*)novf, 0u, 1));
+  a.items = G.items;
+  a.rows_per_item = rows_per_item;
+  DRPT_HIP(hipEventRecord(ctx->ev[7], s));
+  // 1. widened candidates of the first trees
+  a.range = G.range + 0;
+  a.out_idx = s1;
+  a.tree0 = 0;
+  DRPT_CHECK(launch_lowp_knn(ctx, &pp, kModeCand, a));
+  // 2. known lists and bounds
+  RowMergeArgs m{};
+  m.X = X;
+  m.n = n;
+  m.d = p->d;
+  m.K = K;
+  m.src = s1;
+  m.src_tree_stride = n * KP;
+  m.S = KP;
+  m.ntsrc = first;
+  m.out_idx = ki;
+  m.out_dist = kd;
+  DRPT_CHECK(launch_row_merge(ctx, p->dtype, m));
+  DRPT_CHECK(launch_leaf_gather_val(ctx, G.perm, (int64_t)T * n, kd, K, K - 1, boundT));
+  // 3. pools over every tree
+  DRPT_HIP(hipEventRecord(ctx->ev[9], s));
+  a.range = G.range + 4;
+  a.out_idx = pool;
+  a.out_dist = nullptr;
+  a.boundT = boundT;
+  a.known = ki;
+  a.pool_cnt = pcnt;
+  a.pool_cap = kPoolCap;
+  a.ovf_flag = oflag;
+  a.ovf_list = olist;
+  a.novf = novf;
+  DRPT_CHECK(launch_lowp_knn(ctx, &pp, kModePool, a));
+  DRPT_HIP(hipEventRecord(ctx->ev[8], s));
+  // 4. final lists
+  RowMergeArgs f{};
+  f.X = X;
+  f.n = n;
+  f.d = p->d;
+  f.K = K;
+  f.src = pool;
+  f.S = kPoolCap;
+  f.ntsrc = 1;
+  f.cnt = pcnt;
+  f.known_idx = ki;
+  f.known_dist = kd;
+  f.skip_flag = oflag;
+  f.out_idx = out_idx;
+  f.out_dist = out_dist;
+  DRPT_CHECK(launch_row_merge(ctx, p->dtype, f));
+  int32_t nov = 0;
+  DRPT_HIP(hipMemcpyAsync(&nov, novf, 4, hipMemcpyDeviceToHost, s));
+  DRPT_HIP(hipStreamSynchronize(s));
+  if (nov > 0)
+    DRPT_CHECK(launch_row_exact(ctx, &pp, X, G.perm, G.items, G.range + 4, rows_per_item, olist,
+                                novf, n, out_idx, out_dist));
+  DRPT_HIP(hipEventRecord(ctx->ev[5], s));
+  DRPT_CHECK(leaf_stats(ctx, p, G, timings));
+  if (timings) {
+    timings->group_ms = ms_between(ctx->ev[6], ctx->ev[7]);
+    timings->knn_ms = ms_between(ctx->ev[7], ctx->ev[5]);
+    timings->knn_kernel_ms = ms_between(ctx->ev[9], ctx->ev[8]);
+    timings->n_fallback_rows = nov;
   }
   return DRPT_OK;
 }
@@ -527,13 +681,9 @@ int drpt_build_graph(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t
   host_build_projection(p->d, L, p->density, p->seed, R.data());
   float* P;
   uint32_t *colmax, *heap;
-  int32_t* ci;
-  float* cd;
   DRPT_CHECK(ws(ctx, "build.P", (size_t)L * n, &P));
   DRPT_CHECK(ws(ctx, "build.colmax", L, &colmax));
   DRPT_CHECK(ws(ctx, "build.heap", (size_t)T * n, &heap));
-  DRPT_CHECK(ws(ctx, "build.cand_idx", (size_t)T * n * K, &ci));
-  DRPT_CHECK(ws(ctx, "build.cand_dist", (size_t)T * n * K, &cd));
   DRPT_HIP(hipEventRecord(ctx->ev[0], s));
   DRPT_CHECK(drpt_project(ctx, X, p->dtype, n, p->d, R.data(), L, P, colmax));
   DRPT_HIP(hipEventRecord(ctx->ev[1], s));
@@ -550,9 +700,20 @@ int drpt_build_graph(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t
   DRPT_CHECK(grow_trees_impl(ctx, P, colmax, n, T, D, mode, p->leaf_max, heap));
   DRPT_HIP(hipEventRecord(ctx->ev[2], s));
   drpt_timings tk{};
-  DRPT_CHECK(tree_knn_impl(ctx, p, X, heap, ci, cd, &tk, true));
-  DRPT_HIP(hipEventRecord(ctx->ev[3], s));
-  DRPT_CHECK(launch_merge(ctx, n, T, K, ci, cd, out_idx, out_dist));
+  if (p->precision == DRPT_FP32) {
+    // per-tree lists (MDRPT::calculate_nns), then the merge (:180-198)
+    int32_t* ci;
+    float* cd;
+    DRPT_CHECK(ws(ctx, "build.cand_idx", (size_t)T * n * K, &ci));
+    DRPT_CHECK(ws(ctx, "build.cand_dist", (size_t)T * n * K, &cd));
+    DRPT_CHECK(tree_knn_impl(ctx, p, X, heap, ci, cd, &tk));
+    DRPT_HIP(hipEventRecord(ctx->ev[3], s));
+    DRPT_CHECK(launch_merge(ctx, n, T, K, ci, cd, out_idx, out_dist));
+  } else {
+    // the merged lists directly (the merge is the last row merge)
+    DRPT_CHECK(graph_knn_lowp(ctx, p, X, heap, out_idx, out_dist, &tk));
+    DRPT_HIP(hipEventRecord(ctx->ev[3], s));
+  }
   DRPT_HIP(hipEventRecord(ctx->ev[4], s));
   DRPT_HIP(hipStreamSynchronize(s));
   if (timings) {

@@ -98,12 +98,81 @@ int launch_apply_local(drpt_ctx* ctx, const int32_t* perm_in, int32_t* perm_out,
 
 int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, int precision,
                      void* Xq, float* nrm2, float* err, float* scale);
-// items[range[0] .. range[1]) (range in device memory); fallback rows are
-// appended to fallback/nfallback (which the caller zeroes)
+// leaf-ordered |x~|^2 and |x~ - x|: lnbT[i] = nrm2[perm[i]], lerrT[i] = err[perm[i]]
+int launch_leaf_gather(drpt_ctx* ctx, const int32_t* perm, int64_t total, const float* nrm2,
+                       const float* err, float* lnbT, float* lerrT);
+// FP32 exact per-tree lists for items[range[0] .. range[1]) (range in device memory)
 int launch_leaf_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
-                    const LeafItem* items, const int64_t* range, const void* Xq, const float* nrm2,
-                    const float* err, int32_t* cand_idx, float* cand_dist, int32_t* fallback,
-                    int32_t* nfallback, int64_t fallback_cap, const float* bound);
+                    const LeafItem* items, const int64_t* range, int32_t* cand_idx,
+                    float* cand_dist);
+// ---- low-precision (BF16) leaf k-NN, leafknn.hip -----------------------------
+// modes of launch_lowp_knn
+enum { kModeTree = 0, kModeCand = 1, kModePool = 2 };
+
+struct KnnArgs {
+  const void* X;
+  const uint16_t* Xb;
+  const float* lnbT;     // leaf-ordered |x~|^2, [T][n]
+  const float* lerrT;    // leaf-ordered |x~ - x|, [T][n]
+  const int32_t* perm;   // [T][n] points in leaf order
+  const LeafItem* items;
+  const int64_t* range;  // items [range[0], range[1])
+  int64_t n;
+  int d, K;
+  int rows_per_item;  // rows of a leaf one item covers (from row0)
+  unsigned* queue;    // ticket counters (set by launch_lowp_knn)
+  // kModeTree: per-tree lists [T][n][K]; kModeCand: ids [tree - tree0][n][k'];
+  // kModePool: pool ids [n][pool_cap]
+  int32_t* out_idx;
+  float* out_dist;
+  int tree0;
+  // kModeTree: rows redone by k_knn_fallback
+  int32_t* fallback;
+  int32_t* nfallback;
+  int64_t fb_cap;
+  // kModePool
+  const float* boundT;   // leaf-ordered K-th distance bound, [T][n]
+  const int32_t* known;  // [n][K] ids already evaluated (-1 = none)
+  int32_t* pool_cnt;     // [n]
+  int pool_cap;
+  int32_t* ovf_flag;     // [n]
+  int32_t* ovf_list;     // rows whose pool overflowed
+  int32_t* novf;
+};
+// kModeTree: exact per-tree lists (bound pass + filter pass); kModeCand: the
+// k' widened candidates of each row (k' = kprime_eff(p)); kModePool: filter
+// against an external bound, survivors appended to per-row pools
+int launch_lowp_knn(drpt_ctx* ctx, const drpt_params* p, int mode, KnnArgs a);
+int lowp_slots(const drpt_params* p);  // k' actually used (a multiple of 4)
+
+struct RowMergeArgs {
+  const void* X;
+  int64_t n;
+  int d, K;
+  // candidate ids (duplicates, -1 and the row itself allowed): fixed sources
+  // src[t * src_tree_stride + row * S + s] (t < ntsrc, s < S) when cnt == nullptr,
+  // else a pool src[row * S + j], j < min(cnt[row], S)
+  const int32_t* src;
+  int64_t src_tree_stride;
+  int S, ntsrc;
+  const int32_t* cnt;
+  int cnt_lo, cnt_hi;        // pools: rows with cnt_lo < cnt[row] <= cnt_hi (set by the launcher)
+  const int32_t* known_idx;  // optional [n][K] list (sorted, -1 padded) merged in
+  const float* known_dist;
+  const int32_t* skip_flag;  // optional [n]: rows left to the exact row path
+  int32_t* out_idx;          // [n][K]
+  float* out_dist;
+};
+int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a);
+// rows listed in ovf_list: exact K best over every leafmate in every tree
+int launch_row_exact(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
+                     const LeafItem* items, const int64_t* range, int rows_per_item,
+                     const int32_t* ovf_list, const int32_t* novf, int64_t nrows_max,
+                     int32_t* out_idx, float* out_dist);
+// leaf-ordered gather of a per-point value: outT[i] = src[perm[i] * stride + offset]
+int launch_leaf_gather_val(drpt_ctx* ctx, const int32_t* perm, int64_t total, const float* src,
+                           int stride, int offset, float* outT);
+
 int launch_fallback(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
                     const LeafItem* items, const int32_t* fallback, const int32_t* nfallback,
                     int64_t fallback_cap, int32_t* cand_idx, float* cand_dist);

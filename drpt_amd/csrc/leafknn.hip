@@ -25,6 +25,7 @@
 //
 // Roofline: FP32 path vector-FP32 bound (3 flops per dim per pair); BF16 path
 // MFMA bound, algorithmic flops 2*d*sum(m^2).
+#include <algorithm>
 #include <climits>
 
 #include "internal.hpp"
@@ -73,6 +74,14 @@ struct TopK {
         break;
       }
     }
+  }
+  // insert unless the index is already held (equal indices carry equal distances)
+  __device__ __forceinline__ void insert_unique(float dv, int iv) {
+    if (!better(dv, iv, d[KP - 1], i[KP - 1])) return;
+#pragma unroll
+    for (int s = 0; s < KP; ++s)
+      if (i[s] == iv) return;
+    insert(dv, iv);
   }
   // write the first K entries (K <= KP) as (index or -1, distance)
   __device__ __forceinline__ void store(int K, int32_t* oi, float* od) const {
@@ -490,57 +499,116 @@ __device__ __forceinline__ void gram_pass(LowpShared& S, const uint16_t* __restr
   }
 }
 
-template <typename XT, int T>
-__global__ void __launch_bounds__(kThreads, 2)
-k_knn_lowp(const XT* __restrict__ X, const uint16_t* __restrict__ Xb,
-           const float* __restrict__ nb, const float* __restrict__ err, int d,
-           const int32_t* __restrict__ perm, int64_t n, const LeafItem* __restrict__ items,
-           const int64_t* __restrict__ range, int K, int32_t* __restrict__ cand_idx,
-           float* __restrict__ cand_dist, unsigned* __restrict__ queue,
-           int32_t* __restrict__ fallback, int32_t* __restrict__ nfallback, int64_t fb_cap,
-           const float* __restrict__ bound) {
+// exact D of two pairs at once (independent chains, loads interleaved)
+template <typename XT>
+__device__ __forceinline__ void exact_dist4_x2(const XT* __restrict__ X, int64_t a0, int64_t b0,
+                                               int64_t a1, int64_t b1, int d, float& D0,
+                                               float& D1) {
+  const XT* xa0 = X + a0 * d;
+  const XT* xb0 = X + b0 * d;
+  const XT* xa1 = X + a1 * d;
+  const XT* xb1 = X + b1 * d;
+  float acc0 = 0.0f, acc1 = 0.0f;
+#pragma unroll 4
+  for (int k = 0; k < d; k += 4) {
+    const float4 u0 = load4f(xa0 + k), v0 = load4f(xb0 + k);
+    const float4 u1 = load4f(xa1 + k), v1 = load4f(xb1 + k);
+    float t = __fsub_rn(u0.x, v0.x);
+    acc0 = fmaf(t, t, acc0);
+    t = __fsub_rn(u1.x, v1.x);
+    acc1 = fmaf(t, t, acc1);
+    t = __fsub_rn(u0.y, v0.y);
+    acc0 = fmaf(t, t, acc0);
+    t = __fsub_rn(u1.y, v1.y);
+    acc1 = fmaf(t, t, acc1);
+    t = __fsub_rn(u0.z, v0.z);
+    acc0 = fmaf(t, t, acc0);
+    t = __fsub_rn(u1.z, v1.z);
+    acc1 = fmaf(t, t, acc1);
+    t = __fsub_rn(u0.w, v0.w);
+    acc0 = fmaf(t, t, acc0);
+    t = __fsub_rn(u1.w, v1.w);
+    acc1 = fmaf(t, t, acc1);
+  }
+  D0 = acc0;
+  D1 = acc1;
+}
+
+// Work distribution: the item range is cut into kQueues contiguous chunks and
+// block b takes items of chunk b % kQueues from its own ticket counter. Blocks
+// are dispatched round-robin over the 8 XCDs, so neighbouring items (the
+// leaves of one subtree) share an XCD's L2.
+constexpr int kQueues = 8;
+
+// Kernel modes (kMode* in internal.hpp):
+//   kModeTree  exact per-tree lists: bound pass, filter pass, exact D, top K
+//   kModeCand  bound pass only: the row's k' widened candidates (point ids)
+//   kModePool  filter pass against an external bound B (leaf-ordered), skipping
+//              the row's known ids; survivors are appended to the row's pool
+
+
+template <typename XT, int T, int MODE>
+__global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
   constexpr int KP = 4 * T;  // widened candidates per row in the bound pass
   __shared__ __align__(16) LowpShared S;
+  const XT* __restrict__ X = static_cast<const XT*>(A.X);
+  const uint16_t* __restrict__ Xb = A.Xb;
+  const int d = A.d, K = A.K;
+  const int64_t n = A.n;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const bool own_bound = bound == nullptr;
   const bool a_resident = d <= kGramChunk;
   // this thread's query row within the block and quarter of the columns
   const int q = (wave & 1) * 32 + (lane & 31);
   const int sub = (wave >> 1) * 2 + (lane >> 5);
   const int cq = (wave >> 1) * 32 + 4 * (lane >> 5);  // + 8 * (e / 4) + e % 4
-  const int base_slot = own_bound ? KP : 0;
-  const int64_t begin = range[0], end = range[1];
-  if (tid == 0) S.ticket[0] = (int)atomicAdd(queue, 1u);
+  const int qx = blockIdx.x % kQueues;
+  const int64_t chunk = (A.range[1] - A.range[0] + kQueues - 1) / kQueues;
+  const int64_t begin = A.range[0] + qx * chunk;
+  const int64_t end = min(A.range[1], begin + chunk);
+  unsigned* myq = A.queue + qx;
+  if (tid == 0) S.ticket[0] = (int)atomicAdd(myq, 1u);
   __syncthreads();
   int64_t item = begin + S.ticket[0];
   for (int parity = 1;; parity ^= 1) {
     if (item >= end) break;
-    if (tid == 0) S.ticket[parity] = (int)atomicAdd(queue, 1u);  // next item, fetched early
-    const LeafItem it = items[item];
-    const int32_t* pl = perm + (int64_t)it.tree * n + it.start;
+    if (tid == 0) S.ticket[parity] = (int)atomicAdd(myq, 1u);  // next item, fetched early
+    const LeafItem it = A.items[item];
+    const int64_t lofs = (int64_t)it.tree * n + it.start;  // leaf offset in the per-tree arrays
+    const int32_t* pl = A.perm + lofs;
     const int m = it.size;
-    if (m > kLeafStage) {  // too large to stage: exact path for every row
-      for (int r = tid; r < m; r += kThreads) {
-        const int slot = atomicAdd(nfallback, 1);
-        if (slot < fb_cap) {
-          fallback[2 * slot] = (int32_t)item;
-          fallback[2 * slot + 1] = r;
+    const int row_end = (int)min((int64_t)m, (int64_t)it.row0 + A.rows_per_item);
+    if (m > kLeafStage) {  // too large to stage
+      if (MODE == kModeTree) {  // exact path for these rows
+        for (int r = it.row0 + tid; r < row_end; r += kThreads) {
+          const int slot = atomicAdd(A.nfallback, 1);
+          if (slot < A.fb_cap) {
+            A.fallback[2 * slot] = (int32_t)item;
+            A.fallback[2 * slot + 1] = r - it.row0;
+          }
+        }
+      } else if (MODE == kModeCand) {  // no widened candidates
+        for (int r = it.row0 + tid; r < row_end; r += kThreads)
+          for (int s = 0; s < KP; ++s)
+            A.out_idx[((int64_t)(it.tree - A.tree0) * n + pl[r]) * KP + s] = -1;
+      } else {  // every row redone exactly over all trees
+        for (int r = it.row0 + tid; r < row_end; r += kThreads) {
+          const int id = pl[r];
+          if (atomicExch(&A.ovf_flag[id], 1) == 0) A.ovf_list[atomicAdd(A.novf, 1)] = id;
         }
       }
       __syncthreads();
       item = begin + S.ticket[parity];
       continue;
     }
-    // stage ids and norms of the leaf; leaf maxima for the bound
+    // stage ids and norms of the leaf (leaf-ordered arrays: coalesced); leaf maxima
     float mn = 0.0f, me_ = 0.0f;
     for (int j = tid; j < m; j += kThreads) {
-      const int id = pl[j];
-      const float v = nb[id];
-      S.lid[j] = id;
+      const float v = A.lnbT[lofs + j];
+      S.lid[j] = pl[j];
       S.lnb[j] = v;
       mn = fmaxf(mn, v);
-      me_ = fmaxf(me_, err[id]);
+      me_ = fmaxf(me_, A.lerrT[lofs + j]);
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -560,8 +628,8 @@ k_knn_lowp(const XT* __restrict__ X, const uint16_t* __restrict__ Xb,
     }
     const double nm = sqrt((double)nmax * (1.0 + (d + 4.0) * 5.9604644775390625e-08));
 
-    for (int row0 = 0; row0 < m; row0 += kRows) {
-      const int rows = min(kRows, m - row0);
+    for (int row0 = it.row0; row0 < row_end; row0 += kRows) {
+      const int rows = min(kRows, row_end - row0);
       const bool qv = q < rows;
       const int qpos = row0 + q;
       const float rn = qv ? S.lnb[qpos] : 0.0f;
@@ -577,7 +645,7 @@ k_knn_lowp(const XT* __restrict__ X, const uint16_t* __restrict__ Xb,
       uint32_t key[T];
 #pragma unroll
       for (int s = 0; s < T; ++s) key[s] = 0xFFFFFFFFu;
-      if (own_bound) {
+      if (MODE != kModePool) {
         // bound pass: per-thread T smallest packed keys
         gram_pass(S, Xb, d, m, row0, rows, a_resident, [&](int c0, const floatx16& acc) {
 #pragma unroll
@@ -603,16 +671,43 @@ k_knn_lowp(const XT* __restrict__ X, const uint16_t* __restrict__ Xb,
             }
           }
         });
+      }
+      if (MODE == kModeCand) {
+        if (qv) {
+          int32_t* o = A.out_idx + ((int64_t)(it.tree - A.tree0) * n + S.lid[qpos]) * KP + sub * T;
+#pragma unroll
+          for (int s = 0; s < T; ++s)
+            o[s] = key[s] == 0xFFFFFFFFu ? -1 : S.lid[key[s] & 0x7FFu];
+        }
+        __syncthreads();  // As / Bs reuse by the next block
+        continue;
+      }
+      if (MODE == kModeTree) {
         if (qv) {
 #pragma unroll
           for (int s = 0; s < T; ++s)
             S.cpos[q][sub * T + s] = key[s] == 0xFFFFFFFFu ? kNoPos : (uint16_t)(key[s] & 0x7FFu);
         }
         __syncthreads();
-        for (int p = tid; p < rows * KP; p += kThreads) {
-          const int r = p / KP, s = p - r * KP;
-          const uint16_t pos = S.cpos[r][s];
-          S.cdist[r][s] = pos == kNoPos ? INFINITY : exact_dist4(X, S.lid[row0 + r], S.lid[pos], d);
+        const int npair = rows * KP;
+        for (int p = tid; p < npair; p += 2 * kThreads) {
+          const int p1 = p + kThreads;
+          const int r0_ = p / KP, s0 = p - r0_ * KP;
+          const uint16_t pos0 = S.cpos[r0_][s0];
+          const bool two = p1 < npair;
+          const int r1_ = two ? p1 / KP : r0_, s1 = two ? p1 - r1_ * KP : s0;
+          const uint16_t pos1 = S.cpos[r1_][s1];
+          const int a0 = S.lid[row0 + r0_], b0 = pos0 == kNoPos ? a0 : S.lid[pos0];
+          const int a1 = S.lid[row0 + r1_], b1 = pos1 == kNoPos ? a1 : S.lid[pos1];
+          float D0, D1;
+          if ((d & 3) == 0) {
+            exact_dist4_x2(X, a0, b0, a1, b1, d, D0, D1);
+          } else {
+            D0 = exact_dist(X, a0, b0, d);
+            D1 = exact_dist(X, a1, b1, d);
+          }
+          S.cdist[r0_][s0] = pos0 == kNoPos ? INFINITY : D0;
+          if (two) S.cdist[r1_][s1] = pos1 == kNoPos ? INFINITY : D1;
         }
         __syncthreads();
         // B = K-th smallest exact distance of the row's KP candidates
@@ -628,49 +723,73 @@ k_knn_lowp(const XT* __restrict__ X, const uint16_t* __restrict__ Xb,
           const uint64_t hit = __ballot(iv != INT_MAX && rank == K - 1);
           const float B = hit ? lane_read(dv, __builtin_ctzll(hit)) : INFINITY;
           if (lane == 0) {
-            const int id = S.lid[row0 + r];
-            S.cacut[r] = acut_of(B, S.lnb[row0 + r], err[id], nm, emax, d);
+            S.cacut[r] = acut_of(B, S.lnb[row0 + r], A.lerrT[lofs + row0 + r], nm, emax, d);
             S.ccnt[r] = KP;
           }
         }
-      } else {
-        if (tid < rows) {
-          const int id = S.lid[row0 + tid];
-          S.cacut[tid] = acut_of(bound[(int64_t)id * K + (K - 1)], S.lnb[row0 + tid], err[id], nm,
-                                 emax, d);
-          S.ccnt[tid] = 0;
+      } else {  // kModePool: external bound, known ids of the block's rows
+        if (tid < rows)
+          S.cacut[tid] = acut_of(A.boundT[lofs + row0 + tid], S.lnb[row0 + tid],
+                                 A.lerrT[lofs + row0 + tid], nm, emax, d);
+        int* kn = reinterpret_cast<int*>(&S.cdist[0][0]);  // [64][K]
+        for (int j = tid; j < rows * K; j += kThreads) {
+          const int r = j / K;
+          kn[j] = A.known[(int64_t)S.lid[row0 + r] * K + (j - r * K)];
         }
       }
       __syncthreads();
-      // filter pass: append every leafmate that can enter the row's top K
+      // filter pass: every leafmate that can enter the row's top K
       const float acut = qv ? S.cacut[q] : -INFINITY;
+      const int* kn = reinterpret_cast<const int*>(&S.cdist[0][0]) + q * K;
+      const int qid = qv ? S.lid[qpos] : 0;
       gram_pass(S, Xb, d, m, row0, rows, a_resident, [&](int c0, const floatx16& acc) {
+        uint32_t pm = 0;  // elements with a <= A_cut
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-          const int cb = c0 + cq + 8 * g;
-          const float4 cn4 = *reinterpret_cast<const float4*>(&S.lnb[cb]);
+          const float4 cn4 = *reinterpret_cast<const float4*>(&S.lnb[c0 + cq + 8 * g]);
           const float cn[4] = {cn4.x, cn4.y, cn4.z, cn4.w};
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            const int c = cb + u;
             const float a = fmaf(-2.0f, acc[4 * g + u], __fadd_rn(rn, cn[u]));
-            if (a <= acut && c < m && c != qpos) {
-              bool dup = false;
+            pm |= (a <= acut ? 1u : 0u) << (4 * g + u);
+          }
+        }
+        while (pm != 0) {
+          const int e = __builtin_ctz(pm);
+          pm &= pm - 1;
+          const int c = c0 + cq + 8 * (e >> 2) + (e & 3);
+          if (c >= m || c == qpos) continue;
+          if (MODE == kModeTree) {
+            bool dup = false;
 #pragma unroll
-              for (int s = 0; s < T; ++s)
-                dup |= key[s] != 0xFFFFFFFFu && (key[s] & 0x7FFu) == (uint32_t)c;
-              if (!dup) {
-                const int slot = atomicAdd(&S.ccnt[q], 1);
-                if (slot < kCap) S.cpos[q][slot] = (uint16_t)c;
-              }
+            for (int s = 0; s < T; ++s)
+              dup |= key[s] != 0xFFFFFFFFu && (key[s] & 0x7FFu) == (uint32_t)c;
+            if (!dup) {
+              const int slot = atomicAdd(&S.ccnt[q], 1);
+              if (slot < kCap) S.cpos[q][slot] = (uint16_t)c;
+            }
+          } else {
+            const int id = S.lid[c];
+            bool known = false;
+            for (int k = 0; k < K; ++k) known |= kn[k] == id;
+            if (!known) {
+              const int slot = atomicAdd(&A.pool_cnt[qid], 1);
+              if (slot < A.pool_cap)
+                A.out_idx[(int64_t)qid * A.pool_cap + slot] = id;
+              else if (atomicExch(&A.ovf_flag[qid], 1) == 0)
+                A.ovf_list[atomicAdd(A.novf, 1)] = qid;
             }
           }
         }
       });
+      if (MODE == kModePool) {
+        __syncthreads();  // As / Bs reuse by the next block
+        continue;
+      }
       __syncthreads();
-      // exact D of the appended candidates, flattened over the block's rows
+      // kModeTree: exact D of the appended candidates, flattened over the block's rows
       if (wave == 0) {
-        int c = lane < rows ? min(S.ccnt[lane], kCap) - base_slot : 0;
+        int c = lane < rows ? min(S.ccnt[lane], kCap) - KP : 0;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
           const int v = __shfl_up(c, off);
@@ -681,14 +800,30 @@ k_knn_lowp(const XT* __restrict__ X, const uint16_t* __restrict__ Xb,
       }
       __syncthreads();
       const int total = S.cpre[rows];
-      for (int p = tid; p < total; p += kThreads) {
-        int lo = 0, hi = rows;  // largest r with cpre[r] <= p
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (S.cpre[mid] <= p) lo = mid; else hi = mid;
+      for (int p = tid; p < total; p += 2 * kThreads) {
+        int r_[2], s_[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int pp = min(p + h * kThreads, total - 1);
+          int lo = 0, hi = rows;  // largest r with cpre[r] <= pp
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (S.cpre[mid] <= pp) lo = mid; else hi = mid;
+          }
+          r_[h] = lo;
+          s_[h] = KP + pp - S.cpre[lo];
         }
-        const int s = base_slot + p - S.cpre[lo];
-        S.cdist[lo][s] = exact_dist4(X, S.lid[row0 + lo], S.lid[S.cpos[lo][s]], d);
+        const int a0 = S.lid[row0 + r_[0]], b0 = S.lid[S.cpos[r_[0]][s_[0]]];
+        const int a1 = S.lid[row0 + r_[1]], b1 = S.lid[S.cpos[r_[1]][s_[1]]];
+        float D0, D1;
+        if ((d & 3) == 0) {
+          exact_dist4_x2(X, a0, b0, a1, b1, d, D0, D1);
+        } else {
+          D0 = exact_dist(X, a0, b0, d);
+          D1 = exact_dist(X, a1, b1, d);
+        }
+        S.cdist[r_[0]][s_[0]] = D0;
+        if (p + kThreads < total) S.cdist[r_[1]][s_[1]] = D1;
       }
       __syncthreads();
       // the K best of each row's list, or the exact fallback on overflow
@@ -700,10 +835,10 @@ k_knn_lowp(const XT* __restrict__ X, const uint16_t* __restrict__ Xb,
         const int me = S.lid[row0 + r];
         if (cnt > kCap) {
           if (lane == 0) {
-            const int slot = atomicAdd(nfallback, 1);
-            if (slot < fb_cap) {
-              fallback[2 * slot] = (int32_t)item;
-              fallback[2 * slot + 1] = row0 + r;
+            const int slot = atomicAdd(A.nfallback, 1);
+            if (slot < A.fb_cap) {
+              A.fallback[2 * slot] = (int32_t)item;
+              A.fallback[2 * slot + 1] = row0 + r - it.row0;
             }
           }
           continue;
@@ -712,11 +847,296 @@ k_knn_lowp(const XT* __restrict__ X, const uint16_t* __restrict__ Xb,
         const float dv = pos == kNoPos ? INFINITY : S.cdist[r][lane];
         const int iv = pos == kNoPos ? INT_MAX : S.lid[pos];
         const int64_t o = ((int64_t)it.tree * n + me) * K;
-        rank_store(dv, iv, cnt, K, lane, cand_idx + o, cand_dist + o);
+        rank_store(dv, iv, cnt, K, lane, A.out_idx + o, A.out_dist + o);
       }
       __syncthreads();
     }
     item = begin + S.ticket[parity];
+  }
+}
+
+// Leaf-ordered copies of |x~|^2 and |x~ - x| (position j of tree t's leaf order).
+__global__ void k_leaf_gather(const int32_t* __restrict__ perm, int64_t total,
+                              const float* __restrict__ nb, const float* __restrict__ err,
+                              float* __restrict__ lnbT, float* __restrict__ lerrT) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int id = perm[i];
+    lnbT[i] = nb[id];
+    lerrT[i] = err[id];
+  }
+}
+
+// leaf-ordered gather of one value per point: outT[i] = src[perm[i] * stride + offset]
+__global__ void k_leaf_gather_val(const int32_t* __restrict__ perm, int64_t total,
+                                  const float* __restrict__ src, int stride, int offset,
+                                  float* __restrict__ outT) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x)
+    outT[i] = src[(int64_t)perm[i] * stride + offset];
+}
+
+// ---------------------------------------------------------------------------
+// Per-row merge (one wave per row). Candidate ids from up to 64*E slots
+// (duplicates, -1 and the row itself allowed) minus the row's known ids are
+// sorted and de-duplicated (bitonic network over the wave, E elements per
+// lane), their exact D computed, sorted by (D, index) and merged with the
+// known list; the K best are written. Keys (D bits << 32 | index) order like
+// (D, index) because D >= 0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v, int j) { return (uint32_t)__shfl_xor((int)v, j); }
+__device__ __forceinline__ uint64_t xor_lane(uint64_t v, int j) {
+  const uint32_t lo = xor_lane((uint32_t)v, j), hi = xor_lane((uint32_t)(v >> 32), j);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// ascending sort of the 64*E values v[e] (element index e*64 + lane)
+template <typename U, int E>
+__device__ __forceinline__ void bitonic_wave(U (&v)[E], int lane) {
+  constexpr int N = 64 * E;
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= 64) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int e2 = e ^ (j >> 6);
+          if (e2 > e) {
+            const bool asc = (((e << 6) | lane) & k) == 0;
+            const U a = v[e], b = v[e2];
+            const bool sw = asc ? (a > b) : (a < b);
+            v[e] = sw ? b : a;
+            v[e2] = sw ? a : b;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const U o = xor_lane(v[e], j);
+          const bool asc = (((e << 6) | lane) & k) == 0;
+          const bool lower = (lane & j) == 0;
+          const U mn = v[e] < o ? v[e] : o, mx = v[e] < o ? o : v[e];
+          v[e] = (asc == lower) ? mn : mx;
+        }
+      }
+    }
+  }
+}
+
+// exact D of (row, cand): the fma chain, 8 vector loads per batch (one
+// 128-byte line of each row for FP32 input)
+template <typename XT>
+__device__ __forceinline__ float exact_dist8(const XT* __restrict__ xr, const XT* __restrict__ xc,
+                                             int d) {
+  float acc = 0.0f;
+  if ((d & 3) == 0) {
+#pragma unroll 8
+    for (int k = 0; k < d; k += 4) {
+      const float4 a = load4f(xr + k), b = load4f(xc + k);
+      float t = __fsub_rn(a.x, b.x);
+      acc = fmaf(t, t, acc);
+      t = __fsub_rn(a.y, b.y);
+      acc = fmaf(t, t, acc);
+      t = __fsub_rn(a.z, b.z);
+      acc = fmaf(t, t, acc);
+      t = __fsub_rn(a.w, b.w);
+      acc = fmaf(t, t, acc);
+    }
+  } else {
+    for (int k = 0; k < d; ++k) {
+      const float t = __fsub_rn((float)xr[k], (float)xc[k]);
+      acc = fmaf(t, t, acc);
+    }
+  }
+  return acc;
+}
+
+constexpr int kMergeWaves = 4;    // rows in flight per block (one per wave)
+constexpr int kHashBits = 9;      // per-wave open-addressing table for distinct ids:
+constexpr int kHashSlots = 1 << kHashBits;  // more slots than K + 256 insertions, so probing ends
+
+template <typename XT, int E>
+__global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeArgs A) {
+  __shared__ uint32_t tab[kMergeWaves][kHashSlots];
+  __shared__ uint32_t dense[kMergeWaves][64 * E];
+  const XT* __restrict__ X = static_cast<const XT*>(A.X);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int K = A.K, d = A.d;
+  uint32_t* tb = tab[w];
+  uint32_t* dn = dense[w];
+  for (int j = lane; j < kHashSlots; j += 64) tb[j] = 0xFFFFFFFFu;
+  const int64_t waves = (int64_t)gridDim.x * kMergeWaves;
+  for (int64_t row = (int64_t)blockIdx.x * kMergeWaves + w; row < A.n; row += waves) {
+    if (A.skip_flag != nullptr && A.skip_flag[row] != 0) continue;
+    int total = A.S * A.ntsrc;
+    if (A.cnt) {
+      const int c = A.cnt[row];
+      if (c <= A.cnt_lo || c > A.cnt_hi) continue;  // another launch handles this row
+      total = min(c, A.S);
+    }
+    // known list (sorted, -1 padded): lanes < K
+    int kid = -1;
+    float kd = INFINITY;
+    if (A.known_idx != nullptr && lane < K) {
+      kid = A.known_idx[row * K + lane];
+      kd = A.known_dist[row * K + lane];
+    }
+    const int nknown = __popcll(__ballot(kid >= 0));
+    // known ids first, then candidates: the first insertion of an id wins
+    if (kid >= 0) {
+      uint32_t h = ((uint32_t)kid * 2654435761u) >> (32 - kHashBits);
+      while (true) {
+        const uint32_t old = atomicCAS(&tb[h], 0xFFFFFFFFu, (uint32_t)kid);
+        if (old == 0xFFFFFFFFu || old == (uint32_t)kid) break;
+        h = (h + 1) & (kHashSlots - 1);
+      }
+    }
+    int cnt_new = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int j = e * 64 + lane;
+      int id = -1;
+      if (j < total) {
+        if (A.cnt) {
+          id = A.src[row * A.S + j];
+        } else {
+          const int t = j / A.S;
+          id = A.src[t * A.src_tree_stride + row * A.S + (j - t * A.S)];
+        }
+      }
+      bool fresh = false;
+      if (id >= 0 && id != (int)row) {
+        uint32_t h = ((uint32_t)id * 2654435761u) >> (32 - kHashBits);
+        while (true) {
+          const uint32_t old = atomicCAS(&tb[h], 0xFFFFFFFFu, (uint32_t)id);
+          if (old == 0xFFFFFFFFu) { fresh = true; break; }
+          if (old == (uint32_t)id) break;
+          h = (h + 1) & (kHashSlots - 1);
+        }
+      }
+      const uint64_t bal = __ballot(fresh);
+      if (fresh) dn[cnt_new + __popcll(bal & ((1ull << lane) - 1))] = (uint32_t)id;
+      cnt_new += __popcll(bal);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    // reset the table for the next row
+    for (int j = lane; j < kHashSlots; j += 64) tb[j] = 0xFFFFFFFFu;
+    // exact D of the distinct new ids (dense lanes), keys (D bits << 32 | id)
+    uint64_t key[E];
+    const XT* xr = X + row * d;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      key[e] = ~0ull;
+      const int j = e * 64 + lane;
+      if (j < cnt_new) {
+        const uint32_t id = dn[j];
+        key[e] = ((uint64_t)__float_as_uint(exact_dist8(xr, X + (int64_t)id * d, d)) << 32) | id;
+      }
+    }
+    if (E == 1 || cnt_new <= 64) {
+      uint64_t k1[1] = {key[0]};
+      bitonic_wave<uint64_t, 1>(k1, lane);
+      key[0] = k1[0];
+    } else {
+      bitonic_wave<uint64_t, E>(key, lane);
+    }
+    // merge the K best new entries (lanes < K of key[0]) with the known list
+    const uint64_t nk = lane < K ? key[0] : ~0ull;
+    const uint64_t kk = kid >= 0 ? (((uint64_t)__float_as_uint(kd) << 32) | (uint32_t)kid) : ~0ull;
+    int rn = lane, rk = lane;  // rank of my new entry / of my known entry
+    for (int s = 0; s < K; ++s) {
+      const uint64_t o_new = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(nk >> 32), s) << 32) |
+                             (uint32_t)__builtin_amdgcn_readlane((int)nk, s);
+      const uint64_t o_kn = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(kk >> 32), s) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane((int)kk, s);
+      rn += o_kn < nk ? 1 : 0;
+      rk += o_new < kk ? 1 : 0;
+    }
+    int32_t* oi = A.out_idx + row * K;
+    float* od = A.out_dist + row * K;
+    if (lane < K && nk != ~0ull && rn < K) {
+      oi[rn] = (int32_t)(uint32_t)nk;
+      od[rn] = __uint_as_float((uint32_t)(nk >> 32));
+    }
+    if (lane < K && kk != ~0ull && rk < K) {
+      oi[rk] = kid;
+      od[rk] = kd;
+    }
+    const int nv = min(cnt_new, K) + nknown;
+    if (lane < K && lane >= nv) {
+      oi[lane] = -1;
+      od[lane] = INFINITY;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Exact K best distinct leafmates over every tree, for rows whose pool
+// overflowed: one block per row.
+constexpr int kFbThreadsRow = 128;
+template <typename XT, int KP>
+__global__ void __launch_bounds__(kFbThreadsRow)
+k_row_exact(const XT* __restrict__ X, int d, const int32_t* __restrict__ perm, int64_t n, int T,
+            const int32_t* __restrict__ invperm, const int32_t* __restrict__ rowitem,
+            const LeafItem* __restrict__ items, const int32_t* __restrict__ rows,
+            const int32_t* __restrict__ nrows, int64_t nrows_max, int K,
+            int32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+  __shared__ float ld[kFbThreadsRow][KP];
+  __shared__ int li[kFbThreadsRow][KP];
+  const int64_t total = min((int64_t)*nrows, nrows_max);
+  for (int64_t e = blockIdx.x; e < total; e += gridDim.x) {
+    const int me = rows[e];
+    TopK<KP> top;
+    top.init();
+    for (int t = 0; t < T; ++t) {
+      const int pos = invperm[(int64_t)t * n + me];
+      const LeafItem it = items[rowitem[(int64_t)t * n + pos]];
+      const int32_t* pl = perm + (int64_t)t * n + it.start;
+      for (int c = threadIdx.x; c < it.size; c += kFbThreadsRow) {
+        const int id = pl[c];
+        if (id != me) top.insert_unique(exact_dist(X, me, id, d), id);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < KP; ++s) {
+      ld[threadIdx.x][s] = top.d[s];
+      li[threadIdx.x][s] = top.i[s];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int t = 1; t < kFbThreadsRow; ++t)
+        for (int s = 0; s < KP; ++s) {
+          if (li[t][s] == INT_MAX) break;
+          top.insert_unique(ld[t][s], li[t][s]);
+        }
+      top.store(K, out_idx + (int64_t)me * K, out_dist + (int64_t)me * K);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_invperm(const int32_t* __restrict__ perm, int64_t n, int T,
+                          int32_t* __restrict__ inv) {
+  const int64_t total = (int64_t)T * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = i / n;
+    inv[t * n + perm[i]] = (int32_t)(i - t * n);
+  }
+}
+
+// rowitem[tree * n + position] = the item covering that leaf position
+__global__ void k_rowitem(const LeafItem* __restrict__ items, const int64_t* __restrict__ range,
+                          int rows_per_item, int64_t n, int32_t* __restrict__ rowitem) {
+  const int64_t b = range[0], e = range[1];
+  for (int64_t i = b + blockIdx.x; i < e; i += gridDim.x) {
+    const LeafItem it = items[i];
+    const int r1 = (int)min((int64_t)it.size, (int64_t)it.row0 + rows_per_item);
+    for (int r = it.row0 + threadIdx.x; r < r1; r += blockDim.x)
+      rowitem[(int64_t)it.tree * n + it.start + r] = (int32_t)i;
   }
 }
 
@@ -805,55 +1225,48 @@ int persistent_grid(drpt_ctx* ctx, T kernel, int threads, unsigned* grid) {
   return DRPT_OK;
 }
 
-template <typename XT, int T>
-int launch_lowp(drpt_ctx* ctx, const XT* X, const void* Xq, const float* nrm2, const float* err,
-                int d, const int32_t* perm, int64_t n, const LeafItem* items, const int64_t* range,
-                int K, int32_t* ci, float* cd, unsigned* q, int32_t* fb, int32_t* nfb, int64_t cap,
-                const float* bound) {
+template <typename XT, int T, int MODE>
+int launch_lowp_t(drpt_ctx* ctx, const KnnArgs& a) {
   unsigned grid;
-  DRPT_CHECK(persistent_grid(ctx, k_knn_lowp<XT, T>, kThreads, &grid));
-  hipLaunchKernelGGL((k_knn_lowp<XT, T>), dim3(grid), dim3(kThreads), 0, ctx->stream, X,
-                     (const uint16_t*)Xq, nrm2, err, d, perm, n, items, range, K, ci, cd, q, fb,
-                     nfb, cap, bound);
+  DRPT_CHECK(persistent_grid(ctx, k_knn_lowp<XT, T, MODE>, kThreads, &grid));
+  hipLaunchKernelGGL((k_knn_lowp<XT, T, MODE>), dim3(grid), dim3(kThreads), 0, ctx->stream, a);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }
 
+template <typename XT, int T>
+int launch_lowp_m(drpt_ctx* ctx, int mode, const KnnArgs& a) {
+  if (mode == kModeTree) return launch_lowp_t<XT, T, kModeTree>(ctx, a);
+  if (mode == kModeCand) return launch_lowp_t<XT, T, kModeCand>(ctx, a);
+  return launch_lowp_t<XT, T, kModePool>(ctx, a);
+}
+
 template <typename XT>
-int dispatch(drpt_ctx* ctx, const drpt_params* p, const XT* X, const int32_t* perm,
-             const LeafItem* items, const int64_t* range, const void* Xq, const float* nrm2,
-             const float* err, int32_t* ci, float* cd, unsigned* q, int32_t* fb, int32_t* nfb,
-             int64_t cap, const float* bound) {
-  const int K = p->nn, d = p->d;
-  const int64_t n = p->n;
-  if (K < 1 || K > 64) {
-    set_error("leaf_knn: nn must be in [1, 64]");
-    return DRPT_ERR_UNSUPPORTED;
-  }
+int launch_lowp_x(drpt_ctx* ctx, int t, int mode, const KnnArgs& a) {
+  if (t <= 3) return launch_lowp_m<XT, 3>(ctx, mode, a);
+  if (t <= 4) return launch_lowp_m<XT, 4>(ctx, mode, a);
+  if (t <= 6) return launch_lowp_m<XT, 6>(ctx, mode, a);
+  if (t <= 8) return launch_lowp_m<XT, 8>(ctx, mode, a);
+  if (t <= 12) return launch_lowp_m<XT, 12>(ctx, mode, a);
+  return launch_lowp_m<XT, 16>(ctx, mode, a);
+}
+
+// per-thread slots of the bound pass (4 threads per row)
+int slots_of(int kp) {
+  const int t = (kp + 3) / 4;
+  return t <= 3 ? 3 : t <= 4 ? 4 : t <= 6 ? 6 : t <= 8 ? 8 : t <= 12 ? 12 : 16;
+}
+
+template <typename XT>
+int dispatch_exact(drpt_ctx* ctx, const drpt_params* p, const XT* X, const int32_t* perm,
+                   const LeafItem* items, const int64_t* range, int32_t* ci, float* cd,
+                   unsigned* q) {
   unsigned grid;
-  if (p->precision == DRPT_FP32) {
-    DRPT_CHECK(persistent_grid(ctx, k_knn_exact<XT>, kThreads, &grid));
-    hipLaunchKernelGGL((k_knn_exact<XT>), dim3(grid), dim3(kThreads), 0, ctx->stream, X, d, perm, n,
-                       items, range, K, ci, cd, q);
-    DRPT_HIP(hipGetLastError());
-    return DRPT_OK;
-  }
-  if (p->precision == DRPT_BF16) {
-    const int kp = p->kprime;
-    if (kp < K || kp > 4 * 16) {
-      set_error("leaf_knn: kprime must be in [nn, 64] for BF16");
-      return DRPT_ERR_UNSUPPORTED;
-    }
-    const int t = (kp + 3) / 4;  // per-thread slots of the bound pass (4 threads per row)
-    if (t <= 3) return launch_lowp<XT, 3>(ctx, X, Xq, nrm2, err, d, perm, n, items, range, K, ci, cd, q, fb, nfb, cap, bound);
-    if (t <= 4) return launch_lowp<XT, 4>(ctx, X, Xq, nrm2, err, d, perm, n, items, range, K, ci, cd, q, fb, nfb, cap, bound);
-    if (t <= 6) return launch_lowp<XT, 6>(ctx, X, Xq, nrm2, err, d, perm, n, items, range, K, ci, cd, q, fb, nfb, cap, bound);
-    if (t <= 8) return launch_lowp<XT, 8>(ctx, X, Xq, nrm2, err, d, perm, n, items, range, K, ci, cd, q, fb, nfb, cap, bound);
-    if (t <= 12) return launch_lowp<XT, 12>(ctx, X, Xq, nrm2, err, d, perm, n, items, range, K, ci, cd, q, fb, nfb, cap, bound);
-    return launch_lowp<XT, 16>(ctx, X, Xq, nrm2, err, d, perm, n, items, range, K, ci, cd, q, fb, nfb, cap, bound);
-  }
-  set_error("leaf_knn: precision not supported");
-  return DRPT_ERR_UNSUPPORTED;
+  DRPT_CHECK(persistent_grid(ctx, k_knn_exact<XT>, kThreads, &grid));
+  hipLaunchKernelGGL((k_knn_exact<XT>), dim3(grid), dim3(kThreads), 0, ctx->stream, X, p->d, perm,
+                     p->n, items, range, p->nn, ci, cd, q);
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
 }
 
 }  // namespace
@@ -876,18 +1289,151 @@ int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, 
   return DRPT_OK;
 }
 
+int launch_leaf_gather(drpt_ctx* ctx, const int32_t* perm, int64_t total, const float* nrm2,
+                       const float* err, float* lnbT, float* lerrT) {
+  const int64_t blocks = std::min<int64_t>((total + 255) / 256, (int64_t)ctx->num_cus * 16);
+  hipLaunchKernelGGL(k_leaf_gather, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0,
+                     ctx->stream, perm, total, nrm2, err, lnbT, lerrT);
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
+}
+
+int lowp_slots(const drpt_params* p) { return 4 * slots_of(p->kprime); }
+
 int launch_leaf_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
-                    const LeafItem* items, const int64_t* range, const void* Xq, const float* nrm2,
-                    const float* err, int32_t* cand_idx, float* cand_dist, int32_t* fallback,
-                    int32_t* nfallback, int64_t fallback_cap, const float* bound) {
+                    const LeafItem* items, const int64_t* range, int32_t* cand_idx,
+                    float* cand_dist) {
+  if (p->nn < 1 || p->nn > 64) {
+    set_error("leaf_knn: nn must be in [1, 64]");
+    return DRPT_ERR_UNSUPPORTED;
+  }
   unsigned* q = nullptr;
-  DRPT_CHECK(ws(ctx, "knn.queue", 1, &q));
-  DRPT_CHECK(fill_u32(ctx, q, 0u, 1));
+  DRPT_CHECK(ws(ctx, "knn.queue", kQueues, &q));
+  DRPT_CHECK(fill_u32(ctx, q, 0u, kQueues));
   if (p->dtype == DRPT_U8)
-    return dispatch<uint8_t>(ctx, p, (const uint8_t*)X, perm, items, range, Xq, nrm2, err, cand_idx,
-                             cand_dist, q, fallback, nfallback, fallback_cap, bound);
-  return dispatch<float>(ctx, p, (const float*)X, perm, items, range, Xq, nrm2, err, cand_idx,
-                         cand_dist, q, fallback, nfallback, fallback_cap, bound);
+    return dispatch_exact<uint8_t>(ctx, p, (const uint8_t*)X, perm, items, range, cand_idx,
+                                   cand_dist, q);
+  return dispatch_exact<float>(ctx, p, (const float*)X, perm, items, range, cand_idx, cand_dist, q);
+}
+
+int launch_lowp_knn(drpt_ctx* ctx, const drpt_params* p, int mode, KnnArgs a) {
+  const int K = p->nn, kp = p->kprime;
+  if (K < 1 || K > 64) {
+    set_error("leaf_knn: nn must be in [1, 64]");
+    return DRPT_ERR_UNSUPPORTED;
+  }
+  if (p->precision != DRPT_BF16) {
+    set_error("leaf_knn: precision not supported");
+    return DRPT_ERR_UNSUPPORTED;
+  }
+  if (kp < K || kp > 64) {
+    set_error("leaf_knn: kprime must be in [nn, 64] for BF16");
+    return DRPT_ERR_UNSUPPORTED;
+  }
+  DRPT_CHECK(ws(ctx, "knn.queue", kQueues, &a.queue));
+  DRPT_CHECK(fill_u32(ctx, a.queue, 0u, kQueues));
+  const int t = slots_of(kp);
+  if (p->dtype == DRPT_U8) return launch_lowp_x<uint8_t>(ctx, t, mode, a);
+  return launch_lowp_x<float>(ctx, t, mode, a);
+}
+
+int launch_leaf_gather_val(drpt_ctx* ctx, const int32_t* perm, int64_t total, const float* src,
+                           int stride, int offset, float* outT) {
+  const int64_t blocks = std::min<int64_t>((total + 255) / 256, (int64_t)ctx->num_cus * 16);
+  hipLaunchKernelGGL(k_leaf_gather_val, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0,
+                     ctx->stream, perm, total, src, stride, offset, outT);
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
+}
+
+int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a) {
+  if (a.K < 1 || a.K > 64) {
+    set_error("row_merge: nn must be in [1, 64]");
+    return DRPT_ERR_UNSUPPORTED;
+  }
+  const unsigned grid =
+      (unsigned)std::max<int64_t>(1, std::min<int64_t>((a.n + kMergeWaves - 1) / kMergeWaves,
+                                                       (int64_t)ctx->num_cus * 16));
+  const bool u8 = dtype == DRPT_U8;
+  auto go = [&](int E) -> int {
+    if (E == 1) {
+      if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 1>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+      else hipLaunchKernelGGL((k_row_merge<float, 1>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+    } else if (E == 2) {
+      if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 2>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+      else hipLaunchKernelGGL((k_row_merge<float, 2>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+    } else {
+      if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 4>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+      else hipLaunchKernelGGL((k_row_merge<float, 4>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+    }
+    DRPT_HIP(hipGetLastError());
+    return DRPT_OK;
+  };
+  if (a.cnt == nullptr) {  // fixed sources
+    const int slots = a.S * a.ntsrc;
+    if (slots > 256) {
+      set_error("row_merge: more than 256 candidates per row");
+      return DRPT_ERR_UNSUPPORTED;
+    }
+    return go(slots <= 64 ? 1 : slots <= 128 ? 2 : 4);
+  }
+  // pools: rows with at most 64 entries, then the larger ones
+  if (a.S > 256) {
+    set_error("row_merge: pool capacity above 256");
+    return DRPT_ERR_UNSUPPORTED;
+  }
+  a.cnt_lo = -1;
+  a.cnt_hi = 64;
+  DRPT_CHECK(go(1));
+  if (a.S > 64) {
+    a.cnt_lo = 64;
+    a.cnt_hi = a.S;
+    DRPT_CHECK(go(4));
+  }
+  return DRPT_OK;
+}
+
+int launch_row_exact(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
+                     const LeafItem* items, const int64_t* range, int rows_per_item,
+                     const int32_t* ovf_list, const int32_t* novf, int64_t nrows_max,
+                     int32_t* out_idx, float* out_dist) {
+  const int64_t n = p->n;
+  const int T = p->ntrees, K = p->nn;
+  if (K > 32) {
+    set_error("row_exact: nn > 32 is not supported");
+    return DRPT_ERR_UNSUPPORTED;
+  }
+  int32_t *inv, *rowitem;
+  DRPT_CHECK(ws(ctx, "knn.invperm", (size_t)T * n, &inv));
+  DRPT_CHECK(ws(ctx, "knn.rowitem", (size_t)T * n, &rowitem));
+  const unsigned g = (unsigned)std::min<int64_t>(((int64_t)T * n + 255) / 256, (int64_t)ctx->num_cus * 16);
+  hipLaunchKernelGGL(k_invperm, dim3(g), dim3(256), 0, ctx->stream, perm, n, T, inv);
+  DRPT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_rowitem, dim3((unsigned)ctx->num_cus * 8), dim3(256), 0, ctx->stream, items,
+                     range, rows_per_item, n, rowitem);
+  DRPT_HIP(hipGetLastError());
+  const unsigned grid = (unsigned)(ctx->num_cus * 2);
+  if (p->dtype == DRPT_U8) {
+    if (K <= 16)
+      hipLaunchKernelGGL((k_row_exact<uint8_t, 16>), dim3(grid), dim3(kFbThreadsRow), 0, ctx->stream,
+                         (const uint8_t*)X, p->d, perm, n, T, inv, rowitem, items, ovf_list, novf,
+                         nrows_max, K, out_idx, out_dist);
+    else
+      hipLaunchKernelGGL((k_row_exact<uint8_t, 32>), dim3(grid), dim3(kFbThreadsRow), 0, ctx->stream,
+                         (const uint8_t*)X, p->d, perm, n, T, inv, rowitem, items, ovf_list, novf,
+                         nrows_max, K, out_idx, out_dist);
+  } else {
+    if (K <= 16)
+      hipLaunchKernelGGL((k_row_exact<float, 16>), dim3(grid), dim3(kFbThreadsRow), 0, ctx->stream,
+                         (const float*)X, p->d, perm, n, T, inv, rowitem, items, ovf_list, novf,
+                         nrows_max, K, out_idx, out_dist);
+    else
+      hipLaunchKernelGGL((k_row_exact<float, 32>), dim3(grid), dim3(kFbThreadsRow), 0, ctx->stream,
+                         (const float*)X, p->d, perm, n, T, inv, rowitem, items, ovf_list, novf,
+                         nrows_max, K, out_idx, out_dist);
+  }
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
 }
 
 int launch_fallback(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,

@@ -141,23 +141,25 @@ def test_bf16_lists_equal_fp32(ctx, case, T):
     assert tm.n_fallback_rows <= n * T // 100
 
 
-def test_bf16_duplicates_overflow(ctx, orc):
-    """80 copies of each vector: candidate lists overflow (ties at distance 0)
-    and those rows go through the exact fallback; lists still match."""
-    _log("[edge] bf16 duplicate overflow")
+@pytest.mark.parametrize("copies,nbase,T,overflow", [(40, 20, 3, False), (400, 5, 3, True),
+                                                     (400, 5, 6, True)])
+def test_bf16_duplicates_overflow(ctx, orc, copies, nbase, T, overflow):
+    """Each vector repeated: ties at distance 0 fill the candidate pools (40
+    copies: pools of 65..256 entries; 400 copies: pools overflow and those rows
+    are redone exactly over every tree); the lists still equal the oracle's."""
+    _log(f"[edge] bf16 duplicates copies={copies} T={T}")
     rng = np.random.default_rng(4)
-    base = rng.integers(0, 50, size=(20, 16)).astype(np.float32)
-    X = np.repeat(base, 80, axis=0)
+    base = rng.integers(0, 50, size=(nbase, 16)).astype(np.float32)
+    X = np.repeat(base, copies, axis=0)
     n, d = X.shape
-    for T in (3, 6):
-        p = drpt_amd.leafmax_params(n, d, T, 10, leaf_max=300, precision="bf16", seed=8)
-        oi, od, tm = ctx.build_graph(p, _dev(X))
-        ri, rd, _, _ = orc.build_graph(X, T, p.depth, 10, local_mode=1, leaf_max=300, seed=8,
-                                       density=p.density)
-        _log(f"  T={T} fallback rows {tm.n_fallback_rows}")
-        assert tm.n_fallback_rows > 0
-        assert np.array_equal(oi.cpu().numpy(), ri)
-        assert np.array_equal(_bits(od.cpu().numpy()), _bits(rd))
+    p = drpt_amd.leafmax_params(n, d, T, 10, leaf_max=300, precision="bf16", seed=8)
+    oi, od, tm = ctx.build_graph(p, _dev(X))
+    ri, rd, _, _ = orc.build_graph(X, T, p.depth, 10, local_mode=1, leaf_max=300, seed=8,
+                                   density=p.density)
+    _log(f"  overflow rows {tm.n_fallback_rows}")
+    assert (tm.n_fallback_rows > 0) == overflow
+    assert np.array_equal(oi.cpu().numpy(), ri)
+    assert np.array_equal(_bits(od.cpu().numpy()), _bits(rd))
 
 
 def test_merge_bitexact(ctx, orc):
