@@ -542,16 +542,18 @@ static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
 
 // Trees whose merged widened candidates bound the K-th distance of every row.
 constexpr int kFirstTrees = 4;
-// Pool capacity per row (entries beyond it send the row to the exact row path).
+// Pool slots per row (more survivors send the row to the exact row path).
 constexpr int kPoolCap = 256;
 
 // Low-precision graph (build_graph): the final lists straight from the leaves.
-//  1. kModeCand on trees < a: k' widened candidates per row and tree
-//  2. row merge: distinct candidates, exact D, K best -> known list; its K-th
-//     distance B(i) bounds the K-th distance of the final list
+//  1. kModeCand on trees < a: k' widened candidates per row and tree (with
+//     their approximate distances)
+//  2. row merge: distinct candidates, exact D of the 2K best approximate ones,
+//     K best -> known list; its K-th distance B(i) bounds the K-th distance of
+//     the final list (any K exact distances do)
 //  3. kModePool on every tree: leafmates with LB <= B(i) not in the known
 //     list go to the row's pool (every member of the final top K passes)
-//  4. row merge of pool and known list -> final lists; rows whose pool
+//  4. row merge of pools and known list -> final lists; rows whose pool
 //     overflowed are redone exactly over every tree
 static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
                           const uint32_t* leaf_heap, int32_t* out_idx, float* out_dist,
@@ -573,7 +575,8 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   DRPT_CHECK(group_impl(ctx, p, leaf_heap, rows_per_item, first, &G));
   KnnArgs a{};
   DRPT_CHECK(prep_lowp(ctx, &pp, X, G.perm, &a));
-  int32_t *s1, *ki, *pool, *pcnt, *oflag, *olist, *novf;
+  uint64_t* s1;
+  int32_t *ki, *pool, *pcnt, *oflag, *olist, *novf;
   float *kd, *boundT;
   DRPT_CHECK(ws(ctx, "knn.s1", (size_t)first * n * KP, &s1));
   DRPT_CHECK(ws(ctx, "knn.known_idx", (size_t)n * K, &ki));
@@ -581,10 +584,10 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   DRPT_CHECK(ws(ctx, "knn.boundT", (size_t)T * n, &boundT));
   DRPT_CHECK(ws(ctx, "knn.pool", (size_t)n * kPoolCap, &pool));
   DRPT_CHECK(ws(ctx, "knn.pool_cnt", n, &pcnt));
+  DRPT_CHECK(fill_u32(ctx, (uint32_t*)pcnt, 0u, n));
   DRPT_CHECK(ws(ctx, "knn.ovf_flag", n, &oflag));
   DRPT_CHECK(ws(ctx, "knn.ovf_list", n, &olist));
   DRPT_CHECK(ws(ctx, "knn.novf", 1, &novf));
-  DRPT_CHECK(fill_u32(ctx, (uint32_t*)pcnt, 0u, n));
   DRPT_CHECK(fill_u32(ctx, (uint32_t*)oflag, 0u, n));
   DRPT_CHECK(fill_u32(ctx, (uint32_t*)novf, 0u, 1));
   a.items = G.items;
@@ -592,7 +595,7 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   DRPT_HIP(hipEventRecord(ctx->ev[7], s));
   // 1. widened candidates of the first trees
   a.range = G.range + 0;
-  a.out_idx = s1;
+  a.out_key = s1;
   a.tree0 = 0;
   DRPT_CHECK(launch_lowp_knn(ctx, &pp, kModeCand, a));
   // 2. known lists and bounds
@@ -601,12 +604,14 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   m.n = n;
   m.d = p->d;
   m.K = K;
-  m.src = s1;
+  m.src_key = s1;
   m.src_tree_stride = n * KP;
   m.S = KP;
   m.ntsrc = first;
+  m.rerank = std::min(2 * K, first * KP);
   m.out_idx = ki;
   m.out_dist = kd;
+  m.order = G.perm;  // tree 0's leaf order
   DRPT_CHECK(launch_row_merge(ctx, p->dtype, m));
   DRPT_CHECK(launch_leaf_gather_val(ctx, G.perm, (int64_t)T * n, kd, K, K - 1, boundT));
   // 3. pools over every tree
@@ -630,14 +635,16 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   f.d = p->d;
   f.K = K;
   f.src = pool;
+  f.src_tree_stride = 0;
   f.S = kPoolCap;
   f.ntsrc = 1;
-  f.cnt = pcnt;
+  f.cnt_tree = pcnt;
   f.known_idx = ki;
   f.known_dist = kd;
   f.skip_flag = oflag;
   f.out_idx = out_idx;
   f.out_dist = out_dist;
+  f.order = G.perm;
   DRPT_CHECK(launch_row_merge(ctx, p->dtype, f));
   int32_t nov = 0;
   DRPT_HIP(hipMemcpyAsync(&nov, novf, 4, hipMemcpyDeviceToHost, s));

@@ -121,9 +121,10 @@ struct KnnArgs {
   int d, K;
   int rows_per_item;  // rows of a leaf one item covers (from row0)
   unsigned* queue;    // ticket counters (set by launch_lowp_knn)
-  // kModeTree: per-tree lists [T][n][K]; kModeCand: ids [tree - tree0][n][k'];
-  // kModePool: pool ids [n][pool_cap]
+  // kModeTree: per-tree lists [T][n][K]; kModePool: pool ids [n][pool_cap]
   int32_t* out_idx;
+  // kModeCand: keys approx_bits << 32 | id, [tree - tree0][n][k']
+  uint64_t* out_key;
   float* out_dist;
   int tree0;
   // kModeTree: rows redone by k_knn_fallback
@@ -133,8 +134,8 @@ struct KnnArgs {
   // kModePool
   const float* boundT;   // leaf-ordered K-th distance bound, [T][n]
   const int32_t* known;  // [n][K] ids already evaluated (-1 = none)
-  int32_t* pool_cnt;     // [n]
-  int pool_cap;
+  int32_t* pool_cnt;     // [n] entries of each row's pool (zeroed by the caller)
+  int pool_cap;          // slots per row
   int32_t* ovf_flag;     // [n]
   int32_t* ovf_list;     // rows whose pool overflowed
   int32_t* novf;
@@ -149,17 +150,20 @@ struct RowMergeArgs {
   const void* X;
   int64_t n;
   int d, K;
-  // candidate ids (duplicates, -1 and the row itself allowed): fixed sources
-  // src[t * src_tree_stride + row * S + s] (t < ntsrc, s < S) when cnt == nullptr,
-  // else a pool src[row * S + j], j < min(cnt[row], S)
+  // candidates (duplicates, -1 and the row itself allowed) from ntsrc sources
+  // of S slots each: slot s of source t of a row is at (t * src_tree_stride +
+  // row * S + s), valid for s < cnt_tree[t * n + row] (all S when cnt_tree is
+  // null); as ids (src) or as keys approx_bits << 32 | id (src_key)
   const int32_t* src;
+  const uint64_t* src_key;
   int64_t src_tree_stride;
   int S, ntsrc;
-  const int32_t* cnt;
-  int cnt_lo, cnt_hi;        // pools: rows with cnt_lo < cnt[row] <= cnt_hi (set by the launcher)
+  const int32_t* cnt_tree;
+  int rerank;                // src_key: exact D for the rerank best approximate candidates only
   const int32_t* known_idx;  // optional [n][K] list (sorted, -1 padded) merged in
   const float* known_dist;
   const int32_t* skip_flag;  // optional [n]: rows left to the exact row path
+  const int32_t* order;      // optional permutation of the rows (processing order)
   int32_t* out_idx;          // [n][K]
   float* out_dist;
 };

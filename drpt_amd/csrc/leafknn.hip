@@ -416,50 +416,74 @@ __device__ __forceinline__ float acut_of(float B, float nb_i, float e_i, double 
 }
 
 struct LowpShared {
-  uint16_t As[kRows][kTilePitch];   // query rows, bf16 (one chunk of dims)
-  uint16_t Bs[kCols][kTilePitch];   // leafmate tile, bf16
+  // leafmate tiles, bf16, double-buffered; for d > kGramChunk buffer 1 holds
+  // the query rows' chunk and buffer 0 the leafmates' (single-buffered)
+  uint16_t Bs[2][kCols][kTilePitch];
   int lid[kLeafStage];              // point id of every leaf position
   float lnb[kLeafStage];            // |x~|^2 of every leaf position
   uint16_t cpos[kRows][kCap];       // candidate leaf positions per query row
   float cdist[kRows][kCap];         // their exact distances
-  int ccnt[kRows];
+  int ccnt[kRows * 4];              // per row (tree mode) or per (row, quarter) (pool mode)
   float cacut[kRows];
   int cpre[kRows + 1];
   float red_n[kThreads / 64], red_e[kThreads / 64];
   int ticket[2];
 };
 
+// Query operand of the block (d <= kGramChunk): lane l of wave w holds, for
+// every k-step, the 8 bf16 of query row (w & 1) * 32 + l % 32 at dims
+// 16 ks + 8 (l / 32) .. +7, i.e. the MFMA's B operand, for all passes.
+struct QueryOp {
+  bf16x8 v[kGramChunk / 16];
+};
+
+__device__ __forceinline__ void load_query(LowpShared& S, const uint16_t* __restrict__ Xb, int d,
+                                           int row0, int rows, QueryOp& qo) {
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  uint16_t(*As)[kTilePitch] = S.Bs[1];
+#pragma unroll
+  for (int s4 = 0; s4 < 4; ++s4) {
+    const int c = tid + kThreads * s4;
+    const int r = c >> 4, part = c & 15;
+    *reinterpret_cast<uint4*>(&As[r][part * 8]) =
+        load_piece(Xb, r < rows ? S.lid[row0 + r] : -1, d, part * 8);
+  }
+  __syncthreads();
+  const int qh = (wave & 1) * 32;
+#pragma unroll
+  for (int ks = 0; ks < kGramChunk / 16; ++ks)
+    qo.v[ks] = *reinterpret_cast<const bf16x8*>(&As[qh + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
+  // buffer 1 is next written after gram_pass's first barrier
+}
+
 // One pass of Gram tiles of the query block [row0, row0 + 64) against every
-// leafmate tile; epi(c0, acc) sees each thread's 16 accumulators.
+// leafmate tile; epi(c0, acc) sees each thread's 16 accumulators. With the
+// query operand in registers (d <= kGramChunk) the leafmate tiles are
+// double-buffered: one barrier per tile, the next tile's loads in flight
+// during this tile's MFMAs and epilogue.
 template <typename Epi>
 __device__ __forceinline__ void gram_pass(LowpShared& S, const uint16_t* __restrict__ Xb, int d,
-                                          int m, int row0, int rows, bool a_resident, Epi&& epi) {
+                                          int m, int row0, int rows, const QueryOp& qo,
+                                          Epi&& epi) {
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int qh = (wave & 1) * 32, ch = (wave >> 1) * 32;
-  const int ksteps_res = (d + 15) / 16;
-  uint4 rb[4];
-  if (a_resident) {
+  if (d <= kGramChunk) {
+    const int ksteps = (d + 15) / 16;
+    uint4 rb[4];
 #pragma unroll
     for (int s4 = 0; s4 < 4; ++s4) {
       const int c = tid + kThreads * s4;
       const int r = c >> 4, part = c & 15;
-      rb[s4] = load_piece(Xb, r < m ? S.lid[r] : -1, d, part * 8);
+      *reinterpret_cast<uint4*>(&S.Bs[0][r][part * 8]) =
+          load_piece(Xb, r < m ? S.lid[r] : -1, d, part * 8);
     }
-  }
-  for (int c0 = 0; c0 < m; c0 += kCols) {
-    floatx16 acc;
-#pragma unroll
-    for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
-    if (a_resident) {
-      __syncthreads();  // previous tile's operand reads are done
-#pragma unroll
-      for (int s4 = 0; s4 < 4; ++s4) {
-        const int c = tid + kThreads * s4;
-        *reinterpret_cast<uint4*>(&S.Bs[c >> 4][(c & 15) * 8]) = rb[s4];
-      }
-      __syncthreads();
-      if (c0 + kCols < m) {  // next leafmate tile in flight during this one
+    __syncthreads();
+    int b = 0;
+    for (int c0 = 0; c0 < m; c0 += kCols, b ^= 1) {
+      const bool more = c0 + kCols < m;
+      if (more) {  // next leafmate tile in flight during this one
 #pragma unroll
         for (int s4 = 0; s4 < 4; ++s4) {
           const int c = tid + kThreads * s4;
@@ -468,35 +492,59 @@ __device__ __forceinline__ void gram_pass(LowpShared& S, const uint16_t* __restr
           rb[s4] = load_piece(Xb, col < m ? S.lid[col] : -1, d, part * 8);
         }
       }
-      for (int ks = 0; ks < ksteps_res; ++ks) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&S.Bs[ch + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(&S.As[qh + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+      floatx16 acc;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+#pragma unroll
+      for (int ks = 0; ks < kGramChunk / 16; ++ks) {
+        if (ks < ksteps) {
+          const bf16x8 a =
+              *reinterpret_cast<const bf16x8*>(&S.Bs[b][ch + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qo.v[ks], acc, 0, 0, 0);
+        }
       }
-    } else {
-      for (int k0 = 0; k0 < d; k0 += kGramChunk) {
-        __syncthreads();
+      epi(c0, acc);
+      if (more) {
 #pragma unroll
         for (int s4 = 0; s4 < 4; ++s4) {
           const int c = tid + kThreads * s4;
-          const int r = c >> 4, part = c & 15;
-          const int col = c0 + r;
-          const uint4 va = load_piece(Xb, r < rows ? S.lid[row0 + r] : -1, d, k0 + part * 8);
-          const uint4 vb = load_piece(Xb, col < m ? S.lid[col] : -1, d, k0 + part * 8);
-          *reinterpret_cast<uint4*>(&S.As[r][part * 8]) = va;
-          *reinterpret_cast<uint4*>(&S.Bs[r][part * 8]) = vb;
+          *reinterpret_cast<uint4*>(&S.Bs[b ^ 1][c >> 4][(c & 15) * 8]) = rb[s4];
         }
-        __syncthreads();
-        const int ksteps = min(kGramChunk, d - k0 + 15) / 16;
-        for (int ks = 0; ks < ksteps; ++ks) {
-          const bf16x8 a = *reinterpret_cast<const bf16x8*>(&S.Bs[ch + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
-          const bf16x8 b = *reinterpret_cast<const bf16x8*>(&S.As[qh + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-        }
+      }
+      __syncthreads();
+    }
+    return;
+  }
+  // d > kGramChunk: chunks of dims, query chunk in buffer 1, leafmates in buffer 0
+  uint16_t(*As)[kTilePitch] = S.Bs[1];
+  uint16_t(*Bs)[kTilePitch] = S.Bs[0];
+  for (int c0 = 0; c0 < m; c0 += kCols) {
+    floatx16 acc;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+    for (int k0 = 0; k0 < d; k0 += kGramChunk) {
+      __syncthreads();
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        const int c = tid + kThreads * s4;
+        const int r = c >> 4, part = c & 15;
+        const int col = c0 + r;
+        const uint4 va = load_piece(Xb, r < rows ? S.lid[row0 + r] : -1, d, k0 + part * 8);
+        const uint4 vb = load_piece(Xb, col < m ? S.lid[col] : -1, d, k0 + part * 8);
+        *reinterpret_cast<uint4*>(&As[r][part * 8]) = va;
+        *reinterpret_cast<uint4*>(&Bs[r][part * 8]) = vb;
+      }
+      __syncthreads();
+      const int ksteps = min(kGramChunk, d - k0 + 15) / 16;
+      for (int ks = 0; ks < ksteps; ++ks) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&Bs[ch + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
+        const bf16x8 bq = *reinterpret_cast<const bf16x8*>(&As[qh + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq, acc, 0, 0, 0);
       }
     }
     epi(c0, acc);
   }
+  __syncthreads();
 }
 
 // exact D of two pairs at once (independent chains, loads interleaved)
@@ -539,13 +587,12 @@ __device__ __forceinline__ void exact_dist4_x2(const XT* __restrict__ X, int64_t
 // are dispatched round-robin over the 8 XCDs, so neighbouring items (the
 // leaves of one subtree) share an XCD's L2.
 constexpr int kQueues = 8;
-
+constexpr int kSubCap = 16;  // pool mode: list slots per thread (4 threads per row)
 // Kernel modes (kMode* in internal.hpp):
 //   kModeTree  exact per-tree lists: bound pass, filter pass, exact D, top K
 //   kModeCand  bound pass only: the row's k' widened candidates (point ids)
 //   kModePool  filter pass against an external bound B (leaf-ordered), skipping
 //              the row's known ids; survivors are appended to the row's pool
-
 
 template <typename XT, int T, int MODE>
 __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
@@ -590,7 +637,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
       } else if (MODE == kModeCand) {  // no widened candidates
         for (int r = it.row0 + tid; r < row_end; r += kThreads)
           for (int s = 0; s < KP; ++s)
-            A.out_idx[((int64_t)(it.tree - A.tree0) * n + pl[r]) * KP + s] = -1;
+            A.out_key[((int64_t)(it.tree - A.tree0) * n + pl[r]) * KP + s] = ~0ull;
       } else {  // every row redone exactly over all trees
         for (int r = it.row0 + tid; r < row_end; r += kThreads) {
           const int id = pl[r];
@@ -633,21 +680,14 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
       const bool qv = q < rows;
       const int qpos = row0 + q;
       const float rn = qv ? S.lnb[qpos] : 0.0f;
-      if (a_resident) {  // query rows, all dims (visible after gram_pass's first barrier)
-#pragma unroll
-        for (int s4 = 0; s4 < 4; ++s4) {
-          const int c = tid + kThreads * s4;
-          const int r = c >> 4, part = c & 15;
-          *reinterpret_cast<uint4*>(&S.As[r][part * 8]) =
-              load_piece(Xb, r < rows ? S.lid[row0 + r] : -1, d, part * 8);
-        }
-      }
+      QueryOp qo;
+      if (a_resident) load_query(S, Xb, d, row0, rows, qo);
       uint32_t key[T];
 #pragma unroll
       for (int s = 0; s < T; ++s) key[s] = 0xFFFFFFFFu;
       if (MODE != kModePool) {
         // bound pass: per-thread T smallest packed keys
-        gram_pass(S, Xb, d, m, row0, rows, a_resident, [&](int c0, const floatx16& acc) {
+        gram_pass(S, Xb, d, m, row0, rows, qo, [&](int c0, const floatx16& acc) {
 #pragma unroll
           for (int g = 0; g < 4; ++g) {
             const int cb = c0 + cq + 8 * g;
@@ -674,10 +714,12 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
       }
       if (MODE == kModeCand) {
         if (qv) {
-          int32_t* o = A.out_idx + ((int64_t)(it.tree - A.tree0) * n + S.lid[qpos]) * KP + sub * T;
+          uint64_t* o = A.out_key + ((int64_t)(it.tree - A.tree0) * n + S.lid[qpos]) * KP + sub * T;
 #pragma unroll
           for (int s = 0; s < T; ++s)
-            o[s] = key[s] == 0xFFFFFFFFu ? -1 : S.lid[key[s] & 0x7FFu];
+            o[s] = key[s] == 0xFFFFFFFFu
+                       ? ~0ull
+                       : ((uint64_t)(key[s] & 0xFFFFF800u) << 32) | (uint32_t)S.lid[key[s] & 0x7FFu];
         }
         __syncthreads();  // As / Bs reuse by the next block
         continue;
@@ -728,9 +770,11 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
           }
         }
       } else {  // kModePool: external bound, known ids of the block's rows
-        if (tid < rows)
+        if (tid < rows) {
           S.cacut[tid] = acut_of(A.boundT[lofs + row0 + tid], S.lnb[row0 + tid],
                                  A.lerrT[lofs + row0 + tid], nm, emax, d);
+          S.ccnt[tid] = 0;
+        }
         int* kn = reinterpret_cast<int*>(&S.cdist[0][0]);  // [64][K]
         for (int j = tid; j < rows * K; j += kThreads) {
           const int r = j / K;
@@ -740,9 +784,8 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
       __syncthreads();
       // filter pass: every leafmate that can enter the row's top K
       const float acut = qv ? S.cacut[q] : -INFINITY;
-      const int* kn = reinterpret_cast<const int*>(&S.cdist[0][0]) + q * K;
-      const int qid = qv ? S.lid[qpos] : 0;
-      gram_pass(S, Xb, d, m, row0, rows, a_resident, [&](int c0, const floatx16& acc) {
+      int nsub = 0;  // kModePool: entries of this thread's list
+      gram_pass(S, Xb, d, m, row0, rows, qo, [&](int c0, const floatx16& acc) {
         uint32_t pm = 0;  // elements with a <= A_cut
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
@@ -768,22 +811,55 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
               const int slot = atomicAdd(&S.ccnt[q], 1);
               if (slot < kCap) S.cpos[q][slot] = (uint16_t)c;
             }
-          } else {
-            const int id = S.lid[c];
-            bool known = false;
-            for (int k = 0; k < K; ++k) known |= kn[k] == id;
-            if (!known) {
-              const int slot = atomicAdd(&A.pool_cnt[qid], 1);
-              if (slot < A.pool_cap)
-                A.out_idx[(int64_t)qid * A.pool_cap + slot] = id;
-              else if (atomicExch(&A.ovf_flag[qid], 1) == 0)
-                A.ovf_list[atomicAdd(A.novf, 1)] = qid;
-            }
+          } else {  // this thread's list (known ids dropped at the flush)
+            if (nsub < kSubCap) S.cpos[q][sub * kSubCap + nsub] = (uint16_t)c;
+            ++nsub;
           }
         }
       });
       if (MODE == kModePool) {
-        __syncthreads();  // As / Bs reuse by the next block
+        // flush: thread (row r = tid / 4, quarter tid % 4) drops the row's known
+        // ids from the list of the thread that owned (r, quarter) in the tiles,
+        // the 4 lanes of a row reserve pool slots with one atomic
+        S.ccnt[q * 4 + sub] = nsub;  // the owner's count, visible after the barrier
+        __syncthreads();
+        const int r = tid >> 2, qs = tid & 3;
+        const int* kn = reinterpret_cast<const int*>(&S.cdist[0][0]) + r * K;
+        const int cnt = r < rows ? S.ccnt[r * 4 + qs] : 0;
+        uint16_t* lst = &S.cpos[r][qs * kSubCap];
+        int keep = 0;
+        if (cnt > kSubCap) {
+          keep = A.pool_cap + 1;  // forces the row's overflow
+        } else {
+          for (int j = 0; j < cnt; ++j) {
+            const uint16_t c = lst[j];
+            const int id = S.lid[c];
+            bool known = false;
+            for (int k = 0; k < K; ++k) known |= kn[k] == id;
+            if (!known) lst[keep++] = c;
+          }
+        }
+        int pre = keep;  // inclusive prefix over the row's 4 lanes
+#pragma unroll
+        for (int off = 1; off < 4; off <<= 1) {
+          const int v = __shfl_up(pre, off, 4);
+          if (qs >= off) pre += v;
+        }
+        const int total = __shfl(pre, 3, 4);
+        int base = 0;
+        if (qs == 0 && total > 0 && total <= A.pool_cap) base = atomicAdd(&A.pool_cnt[S.lid[row0 + r]], total);
+        base = __shfl(base, 0, 4);
+        if (total > 0) {
+          const int id = S.lid[row0 + r];
+          if (total <= A.pool_cap && base + total <= A.pool_cap) {
+            const int o = base + pre - keep;
+            for (int j = 0; j < keep; ++j)
+              A.out_idx[(int64_t)id * A.pool_cap + o + j] = S.lid[lst[j]];
+          } else if (qs == 0 && atomicExch(&A.ovf_flag[id], 1) == 0) {
+            A.ovf_list[atomicAdd(A.novf, 1)] = id;
+          }
+        }
+        __syncthreads();  // lists and tiles reused by the next block
         continue;
       }
       __syncthreads();
@@ -959,22 +1035,24 @@ constexpr int kHashSlots = 1 << kHashBits;  // more slots than K + 256 insertion
 template <typename XT, int E>
 __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeArgs A) {
   __shared__ uint32_t tab[kMergeWaves][kHashSlots];
-  __shared__ uint32_t dense[kMergeWaves][64 * E];
+  __shared__ uint64_t dense[kMergeWaves][64 * E];
   const XT* __restrict__ X = static_cast<const XT*>(A.X);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int K = A.K, d = A.d;
   uint32_t* tb = tab[w];
-  uint32_t* dn = dense[w];
+  uint64_t* dn = dense[w];
   for (int j = lane; j < kHashSlots; j += 64) tb[j] = 0xFFFFFFFFu;
-  const int64_t waves = (int64_t)gridDim.x * kMergeWaves;
-  for (int64_t row = (int64_t)blockIdx.x * kMergeWaves + w; row < A.n; row += waves) {
+  // rows in A.order (leaf order of one tree: neighbouring rows share most
+  // candidates), one contiguous chunk per XCD (blocks go round-robin over 8)
+  const int xcd = blockIdx.x & 7;
+  const int64_t nblk = (gridDim.x - xcd + 7) >> 3;  // blocks on this XCD
+  const int64_t chunk = (A.n + 7) >> 3;
+  const int64_t lo = xcd * chunk, hi = min(A.n, lo + chunk);
+  const int nslots = A.S * A.ntsrc;
+  for (int64_t i = lo + (int64_t)(blockIdx.x >> 3) * kMergeWaves + w; i < hi;
+       i += nblk * kMergeWaves) {
+    const int64_t row = A.order != nullptr ? (int64_t)A.order[i] : i;
     if (A.skip_flag != nullptr && A.skip_flag[row] != 0) continue;
-    int total = A.S * A.ntsrc;
-    if (A.cnt) {
-      const int c = A.cnt[row];
-      if (c <= A.cnt_lo || c > A.cnt_hi) continue;  // another launch handles this row
-      total = min(c, A.S);
-    }
     // known list (sorted, -1 padded): lanes < K
     int kid = -1;
     float kd = INFINITY;
@@ -992,17 +1070,30 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
         h = (h + 1) & (kHashSlots - 1);
       }
     }
+    // valid slots per source (lane t holds source t's count; ntsrc <= 64)
+    const int ct = (A.cnt_tree != nullptr && lane < A.ntsrc)
+                       ? min(A.cnt_tree[(int64_t)lane * A.n + row], A.S) : A.S;
     int cnt_new = 0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int j = e * 64 + lane;
+      const int t = min(j / A.S, 63), sl = j - t * A.S;
+      const int cti = __shfl(ct, t);
       int id = -1;
-      if (j < total) {
-        if (A.cnt) {
-          id = A.src[row * A.S + j];
-        } else {
-          const int t = j / A.S;
-          id = A.src[t * A.src_tree_stride + row * A.S + (j - t * A.S)];
+      uint32_t ap = 0;  // approximate distance bits (src_key)
+      if (j < nslots) {
+        const bool ok = sl < cti;
+        if (ok) {
+          const int64_t at = t * A.src_tree_stride + row * A.S + sl;
+          if (A.src_key != nullptr) {
+            const uint64_t kv = A.src_key[at];
+            if (kv != ~0ull) {
+              id = (int)(uint32_t)kv;
+              ap = (uint32_t)(kv >> 32);
+            }
+          } else {
+            id = A.src[at];
+          }
         }
       }
       bool fresh = false;
@@ -1016,26 +1107,40 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
         }
       }
       const uint64_t bal = __ballot(fresh);
-      if (fresh) dn[cnt_new + __popcll(bal & ((1ull << lane) - 1))] = (uint32_t)id;
+      if (fresh) dn[cnt_new + __popcll(bal & ((1ull << lane) - 1))] = ((uint64_t)ap << 32) | (uint32_t)id;
       cnt_new += __popcll(bal);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     // reset the table for the next row
     for (int j = lane; j < kHashSlots; j += 64) tb[j] = 0xFFFFFFFFu;
-    // exact D of the distinct new ids (dense lanes), keys (D bits << 32 | id)
+    // distinct new candidates in dense lanes; keep the rerank best approximate ones
     uint64_t key[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) key[e] = (e * 64 + lane < cnt_new) ? dn[e * 64 + lane] : ~0ull;
+    int neval = cnt_new;
+    if (A.rerank > 0 && cnt_new > A.rerank) {
+      if (cnt_new <= 64) {
+        uint64_t k1[1] = {key[0]};
+        bitonic_wave<uint64_t, 1>(k1, lane);
+        key[0] = k1[0];
+      } else {
+        bitonic_wave<uint64_t, E>(key, lane);
+      }
+      neval = A.rerank;
+    }
+    // exact D, keys (D bits << 32 | id)
     const XT* xr = X + row * d;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      key[e] = ~0ull;
-      const int j = e * 64 + lane;
-      if (j < cnt_new) {
-        const uint32_t id = dn[j];
+      if (e * 64 + lane < neval) {
+        const uint32_t id = (uint32_t)key[e];
         key[e] = ((uint64_t)__float_as_uint(exact_dist8(xr, X + (int64_t)id * d, d)) << 32) | id;
+      } else {
+        key[e] = ~0ull;
       }
     }
-    if (E == 1 || cnt_new <= 64) {
+    if (E == 1 || neval <= 64) {
       uint64_t k1[1] = {key[0]};
       bitonic_wave<uint64_t, 1>(k1, lane);
       key[0] = k1[0];
@@ -1064,7 +1169,7 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
       oi[rk] = kid;
       od[rk] = kd;
     }
-    const int nv = min(cnt_new, K) + nknown;
+    const int nv = min(neval, K) + nknown;
     if (lane < K && lane >= nv) {
       oi[lane] = -1;
       od[lane] = INFINITY;
@@ -1347,49 +1452,28 @@ int launch_leaf_gather_val(drpt_ctx* ctx, const int32_t* perm, int64_t total, co
 }
 
 int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a) {
-  if (a.K < 1 || a.K > 64) {
-    set_error("row_merge: nn must be in [1, 64]");
+  const int slots = a.S * a.ntsrc;
+  if (a.K < 1 || a.K > 64 || slots > 256 || a.rerank > 64 * 4 || a.ntsrc > 64) {
+    set_error("row_merge: unsupported sizes (nn in [1, 64], at most 256 candidate slots)");
     return DRPT_ERR_UNSUPPORTED;
   }
-  const unsigned grid =
-      (unsigned)std::max<int64_t>(1, std::min<int64_t>((a.n + kMergeWaves - 1) / kMergeWaves,
-                                                       (int64_t)ctx->num_cus * 16));
+  // a multiple of 8 blocks: every XCD chunk of rows has blocks (k_row_merge)
+  const int64_t want = std::min<int64_t>((a.n + kMergeWaves - 1) / kMergeWaves,
+                                         (int64_t)ctx->num_cus * 16);
+  const unsigned grid = (unsigned)(((std::max<int64_t>(want, 8) + 7) / 8) * 8);
   const bool u8 = dtype == DRPT_U8;
-  auto go = [&](int E) -> int {
-    if (E == 1) {
-      if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 1>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
-      else hipLaunchKernelGGL((k_row_merge<float, 1>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
-    } else if (E == 2) {
-      if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 2>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
-      else hipLaunchKernelGGL((k_row_merge<float, 2>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
-    } else {
-      if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 4>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
-      else hipLaunchKernelGGL((k_row_merge<float, 4>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
-    }
-    DRPT_HIP(hipGetLastError());
-    return DRPT_OK;
-  };
-  if (a.cnt == nullptr) {  // fixed sources
-    const int slots = a.S * a.ntsrc;
-    if (slots > 256) {
-      set_error("row_merge: more than 256 candidates per row");
-      return DRPT_ERR_UNSUPPORTED;
-    }
-    return go(slots <= 64 ? 1 : slots <= 128 ? 2 : 4);
+  const int E = slots <= 64 ? 1 : slots <= 128 ? 2 : 4;
+  if (E == 1) {
+    if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 1>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+    else hipLaunchKernelGGL((k_row_merge<float, 1>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+  } else if (E == 2) {
+    if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 2>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+    else hipLaunchKernelGGL((k_row_merge<float, 2>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+  } else {
+    if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 4>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+    else hipLaunchKernelGGL((k_row_merge<float, 4>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
   }
-  // pools: rows with at most 64 entries, then the larger ones
-  if (a.S > 256) {
-    set_error("row_merge: pool capacity above 256");
-    return DRPT_ERR_UNSUPPORTED;
-  }
-  a.cnt_lo = -1;
-  a.cnt_hi = 64;
-  DRPT_CHECK(go(1));
-  if (a.S > 64) {
-    a.cnt_lo = 64;
-    a.cnt_hi = a.S;
-    DRPT_CHECK(go(4));
-  }
+  DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }
 

@@ -141,12 +141,13 @@ def test_bf16_lists_equal_fp32(ctx, case, T):
     assert tm.n_fallback_rows <= n * T // 100
 
 
-@pytest.mark.parametrize("copies,nbase,T,overflow", [(40, 20, 3, False), (400, 5, 3, True),
+@pytest.mark.parametrize("copies,nbase,T,overflow", [(20, 20, 8, False), (400, 5, 3, True),
                                                      (400, 5, 6, True)])
 def test_bf16_duplicates_overflow(ctx, orc, copies, nbase, T, overflow):
-    """Each vector repeated: ties at distance 0 fill the candidate pools (40
-    copies: pools of 65..256 entries; 400 copies: pools overflow and those rows
-    are redone exactly over every tree); the lists still equal the oracle's."""
+    """Each vector repeated: ties at distance 0 fill the candidate pools (20
+    copies over 8 trees: the same duplicates in every tree's pool segment,
+    merged once; 400 copies: segments overflow and those rows are redone
+    exactly over every tree); the lists still equal the oracle's."""
     _log(f"[edge] bf16 duplicates copies={copies} T={T}")
     rng = np.random.default_rng(4)
     base = rng.integers(0, 50, size=(nbase, 16)).astype(np.float32)
