@@ -415,20 +415,46 @@ __device__ __forceinline__ float acut_of(float B, float nb_i, float e_i, double 
   return (float)((sb * sb + eta) * (1.0 + 1e-6) + 1e-30);
 }
 
-struct LowpShared {
+constexpr int kRing = 3;          // leafmate tile stages in flight (ring path)
+constexpr int kRingRow = 256;     // bytes per leafmate row in a ring stage (d <= 128)
+
+// LDS of the low-precision kernel. Tree mode: register-staged double buffer
+// plus the candidates' exact distances. Candidate / pool modes: the tile ring
+// (also holds the double buffer for inputs the ring does not cover).
+template <bool kTree>
+struct LowpShared;
+
+#define DRPT_LOWP_COMMON                                                              \
+  int lid[kLeafStage];              /* point id of every leaf position */           \
+  float lnb[kLeafStage];            /* |x~|^2 of every leaf position */             \
+  uint16_t cpos[kRows][kCap];       /* candidate leaf positions per query row */    \
+  int ccnt[kRows * 4];              /* per row (tree) or per (row, quarter) (pool) */ \
+  float cacut[kRows];                                                               \
+  float red_n[kThreads / 64], red_e[kThreads / 64];                                 \
+  int ticket[2];
+
+template <>
+struct LowpShared<true> {
   // leafmate tiles, bf16, double-buffered; for d > kGramChunk buffer 1 holds
   // the query rows' chunk and buffer 0 the leafmates' (single-buffered)
   uint16_t Bs[2][kCols][kTilePitch];
-  int lid[kLeafStage];              // point id of every leaf position
-  float lnb[kLeafStage];            // |x~|^2 of every leaf position
-  uint16_t cpos[kRows][kCap];       // candidate leaf positions per query row
-  float cdist[kRows][kCap];         // their exact distances
-  int ccnt[kRows * 4];              // per row (tree mode) or per (row, quarter) (pool mode)
-  float cacut[kRows];
+  DRPT_LOWP_COMMON
+  float cdist[kRows][kCap];         // exact distances of the candidates
   int cpre[kRows + 1];
-  float red_n[kThreads / 64], red_e[kThreads / 64];
-  int ticket[2];
 };
+
+constexpr int kKnownLds = 16;  // pool mode: known ids per row staged in LDS (larger K: global)
+
+template <>
+struct LowpShared<false> {
+  union {
+    uint16_t Bs[2][kCols][kTilePitch];
+    unsigned char ring[kRing][kCols * kRingRow];
+  };
+  DRPT_LOWP_COMMON
+  int known[kRows * kKnownLds];   // pool mode: the block's known ids
+};
+#undef DRPT_LOWP_COMMON
 
 // Query operand of the block (d <= kGramChunk): lane l of wave w holds, for
 // every k-step, the 8 bf16 of query row (w & 1) * 32 + l % 32 at dims
@@ -437,7 +463,8 @@ struct QueryOp {
   bf16x8 v[kGramChunk / 16];
 };
 
-__device__ __forceinline__ void load_query(LowpShared& S, const uint16_t* __restrict__ Xb, int d,
+template <typename SH>
+__device__ __forceinline__ void load_query(SH& S, const uint16_t* __restrict__ Xb, int d,
                                            int row0, int rows, QueryOp& qo) {
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -462,8 +489,8 @@ __device__ __forceinline__ void load_query(LowpShared& S, const uint16_t* __rest
 // query operand in registers (d <= kGramChunk) the leafmate tiles are
 // double-buffered: one barrier per tile, the next tile's loads in flight
 // during this tile's MFMAs and epilogue.
-template <typename Epi>
-__device__ __forceinline__ void gram_pass(LowpShared& S, const uint16_t* __restrict__ Xb, int d,
+template <typename SH, typename Epi>
+__device__ __forceinline__ void gram_pass(SH& S, const uint16_t* __restrict__ Xb, int d,
                                           int m, int row0, int rows, const QueryOp& qo,
                                           Epi&& epi) {
   const int tid = threadIdx.x;
@@ -547,6 +574,103 @@ __device__ __forceinline__ void gram_pass(LowpShared& S, const uint16_t* __restr
   __syncthreads();
 }
 
+// vmcnt-only wait (expcnt and lgkmcnt left at their maxima), gfx9 encoding
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8) | ((N >> 4) << 14));
+}
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+// Workgroup barrier for the ring: this wave's LDS operations complete, then
+// s_barrier; global loads stay in flight (__syncthreads would wait for all of
+// them). The memory clobber keeps the compiler from moving memory accesses
+// across it.
+// 16 bytes per lane from global memory straight into LDS at lds_base + 16 *
+// lane (global_load_lds_dwordx4, base in M0). Issued from asm so that the
+// compiler does not wait for it before unrelated LDS reads; completion is
+// awaited explicitly (wait_vmcnt). No register is written by the load; its
+// order against the ring barrier (also volatile asm) is kept.
+__device__ __forceinline__ void lds_dma16(const void* gptr, uint32_t lds_base) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off"
+               :
+               : "v"(gptr), "s"(lds_base)
+               : "m0");
+}
+
+__device__ __forceinline__ void ring_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Ring path of the Gram pass (d % 8 == 0, d <= 128): leafmate tiles go
+// straight from HBM into LDS (global_load_lds_dwordx4, 16 B per lane, lane l
+// of an instruction writes 16 l bytes after its wave-uniform base), kRing
+// stages, two tiles in flight during each tile's MFMAs. Stage layout: row r
+// of the tile at r * 256 bytes, its 16-byte chunk c at physical chunk
+// c ^ (r & 15) (bank spread for the MFMA operand reads); chunks at or past
+// d / 8 stay zero. Every wave issues exactly 4 load instructions per tile
+// (each covers 4 whole rows, at least one valid chunk per row), so
+// vmcnt <= 4 means "all but the newest tile".
+template <typename Epi>
+__device__ __forceinline__ void gram_pass_ring(LowpShared<false>& S,
+                                               const uint16_t* __restrict__ Xb, int d, int m,
+                                               const QueryOp& qo, Epi&& epi) {
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int ch = (wave >> 1) * 32;
+  const int L = d >> 3;            // 16-byte chunks per row
+  const int ksteps = (d + 15) / 16;
+  const int ntiles = (m + kCols - 1) / kCols;
+  __syncthreads();  // query staging reads done (they share the ring's bytes)
+  if (L < 16) {     // zero the chunks no load writes
+    for (int j = tid; j < kRing * kCols * 16; j += kThreads) {
+      const int st = j / (kCols * 16), slot = j % (kCols * 16);
+      const int r = slot >> 4, c = (slot & 15) ^ (r & 15);
+      if (c >= L) *reinterpret_cast<uint4*>(&S.ring[st][slot * 16]) = make_uint4(0, 0, 0, 0);
+    }
+  }
+  auto issue = [&](int j) {
+    unsigned char* base = S.ring[j % kRing];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int blk = i * 4 + wave;  // 64 consecutive slots = rows 4 blk .. 4 blk + 3
+      const int slot = blk * 64 + lane;
+      const int r = slot >> 4, c = (slot & 15) ^ (r & 15);
+      const int col = min(j * kCols + r, m - 1);  // rows past the leaf: any valid row
+      const uint32_t dst = __builtin_amdgcn_readfirstlane(
+          (uint32_t)(uintptr_t)(lds_ptr_t)(base + blk * 64 * 16));
+      if (c < L) lds_dma16(Xb + (int64_t)S.lid[col] * d + c * 8, dst);
+    }
+  };
+  issue(0);
+  if (ntiles > 1) issue(1);
+  for (int t = 0; t < ntiles; ++t) {
+    if (t + 1 < ntiles) wait_vmcnt<4>(); else wait_vmcnt<0>();
+    ring_barrier();  // tile t visible; every wave is done with tile t - 1's stage
+    if (t + 2 < ntiles) issue(t + 2);
+    const unsigned char* st = S.ring[t % kRing];
+    floatx16 acc;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+    const int r = ch + (lane & 31);
+#pragma unroll
+    for (int k4 = 0; k4 < kGramChunk / 16; k4 += 4) {  // 4 operands read, then 4 MFMAs
+      bf16x8 a[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int pc = (2 * (k4 + u) + (lane >> 5)) ^ (r & 15);
+        a[u] = *reinterpret_cast<const bf16x8*>(st + r * kRingRow + pc * 16);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (k4 + u < ksteps) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u], qo.v[k4 + u], acc, 0, 0, 0);
+    }
+    epi(t * kCols, acc);
+  }
+  __syncthreads();  // ring reused by the next pass
+}
+
 // exact D of two pairs at once (independent chains, loads interleaved)
 template <typename XT>
 __device__ __forceinline__ void exact_dist4_x2(const XT* __restrict__ X, int64_t a0, int64_t b0,
@@ -594,10 +718,10 @@ constexpr int kSubCap = 16;  // pool mode: list slots per thread (4 threads per 
 //   kModePool  filter pass against an external bound B (leaf-ordered), skipping
 //              the row's known ids; survivors are appended to the row's pool
 
-template <typename XT, int T, int MODE>
+template <typename XT, int T, int MODE, bool RING>
 __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
   constexpr int KP = 4 * T;  // widened candidates per row in the bound pass
-  __shared__ __align__(16) LowpShared S;
+  __shared__ __align__(16) LowpShared<MODE == kModeTree> S;
   const XT* __restrict__ X = static_cast<const XT*>(A.X);
   const uint16_t* __restrict__ Xb = A.Xb;
   const int d = A.d, K = A.K;
@@ -626,7 +750,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
     const int m = it.size;
     const int row_end = (int)min((int64_t)m, (int64_t)it.row0 + A.rows_per_item);
     if (m > kLeafStage) {  // too large to stage
-      if (MODE == kModeTree) {  // exact path for these rows
+      if constexpr (MODE == kModeTree) {  // exact path for these rows
         for (int r = it.row0 + tid; r < row_end; r += kThreads) {
           const int slot = atomicAdd(A.nfallback, 1);
           if (slot < A.fb_cap) {
@@ -634,7 +758,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
             A.fallback[2 * slot + 1] = r - it.row0;
           }
         }
-      } else if (MODE == kModeCand) {  // no widened candidates
+      } else if constexpr (MODE == kModeCand) {  // no widened candidates
         for (int r = it.row0 + tid; r < row_end; r += kThreads)
           for (int s = 0; s < KP; ++s)
             A.out_key[((int64_t)(it.tree - A.tree0) * n + pl[r]) * KP + s] = ~0ull;
@@ -682,12 +806,19 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
       const float rn = qv ? S.lnb[qpos] : 0.0f;
       QueryOp qo;
       if (a_resident) load_query(S, Xb, d, row0, rows, qo);
+      // the ring path is a separate instantiation (RING: d % 8 == 0, d <= 128,
+      // candidate / pool modes), so no register-staged tile load is pending
+      // in its loop
+      auto gram = [&](auto&& epi) {
+        if constexpr (RING) gram_pass_ring(S, Xb, d, m, qo, epi);
+        else gram_pass(S, Xb, d, m, row0, rows, qo, epi);
+      };
       uint32_t key[T];
 #pragma unroll
       for (int s = 0; s < T; ++s) key[s] = 0xFFFFFFFFu;
-      if (MODE != kModePool) {
+      if constexpr (MODE != kModePool) {
         // bound pass: per-thread T smallest packed keys
-        gram_pass(S, Xb, d, m, row0, rows, qo, [&](int c0, const floatx16& acc) {
+        gram([&](int c0, const floatx16& acc) {
 #pragma unroll
           for (int g = 0; g < 4; ++g) {
             const int cb = c0 + cq + 8 * g;
@@ -712,7 +843,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
           }
         });
       }
-      if (MODE == kModeCand) {
+      if constexpr (MODE == kModeCand) {
         if (qv) {
           uint64_t* o = A.out_key + ((int64_t)(it.tree - A.tree0) * n + S.lid[qpos]) * KP + sub * T;
 #pragma unroll
@@ -724,7 +855,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
         __syncthreads();  // As / Bs reuse by the next block
         continue;
       }
-      if (MODE == kModeTree) {
+      if constexpr (MODE == kModeTree) {
         if (qv) {
 #pragma unroll
           for (int s = 0; s < T; ++s)
@@ -775,17 +906,19 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
                                  A.lerrT[lofs + row0 + tid], nm, emax, d);
           S.ccnt[tid] = 0;
         }
-        int* kn = reinterpret_cast<int*>(&S.cdist[0][0]);  // [64][K]
-        for (int j = tid; j < rows * K; j += kThreads) {
-          const int r = j / K;
-          kn[j] = A.known[(int64_t)S.lid[row0 + r] * K + (j - r * K)];
+        if constexpr (MODE == kModePool) {
+          if (K <= kKnownLds)  // read at the flush, after the Gram pass
+            for (int j = tid; j < rows * K; j += kThreads) {
+              const int r = j / K;
+              S.known[r * kKnownLds + (j - r * K)] = A.known[(int64_t)S.lid[row0 + r] * K + (j - r * K)];
+            }
         }
       }
       __syncthreads();
       // filter pass: every leafmate that can enter the row's top K
       const float acut = qv ? S.cacut[q] : -INFINITY;
       int nsub = 0;  // kModePool: entries of this thread's list
-      gram_pass(S, Xb, d, m, row0, rows, qo, [&](int c0, const floatx16& acc) {
+      gram([&](int c0, const floatx16& acc) {
         uint32_t pm = 0;  // elements with a <= A_cut
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
@@ -802,7 +935,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
           pm &= pm - 1;
           const int c = c0 + cq + 8 * (e >> 2) + (e & 3);
           if (c >= m || c == qpos) continue;
-          if (MODE == kModeTree) {
+          if constexpr (MODE == kModeTree) {
             bool dup = false;
 #pragma unroll
             for (int s = 0; s < T; ++s)
@@ -817,14 +950,15 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
           }
         }
       });
-      if (MODE == kModePool) {
+      if constexpr (MODE == kModePool) {
         // flush: thread (row r = tid / 4, quarter tid % 4) drops the row's known
         // ids from the list of the thread that owned (r, quarter) in the tiles,
         // the 4 lanes of a row reserve pool slots with one atomic
         S.ccnt[q * 4 + sub] = nsub;  // the owner's count, visible after the barrier
         __syncthreads();
         const int r = tid >> 2, qs = tid & 3;
-        const int* kn = reinterpret_cast<const int*>(&S.cdist[0][0]) + r * K;
+        const int* kn = K <= kKnownLds ? &S.known[r * kKnownLds]
+                                       : A.known + (int64_t)S.lid[row0 + min(r, rows - 1)] * K;
         const int cnt = r < rows ? S.ccnt[r * 4 + qs] : 0;
         uint16_t* lst = &S.cpos[r][qs * kSubCap];
         int keep = 0;
@@ -862,68 +996,71 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
         __syncthreads();  // lists and tiles reused by the next block
         continue;
       }
-      __syncthreads();
-      // kModeTree: exact D of the appended candidates, flattened over the block's rows
-      if (wave == 0) {
-        int c = lane < rows ? min(S.ccnt[lane], kCap) - KP : 0;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          const int v = __shfl_up(c, off);
-          if (lane >= off) c += v;
-        }
-        S.cpre[lane + 1] = c;
-        if (lane == 0) S.cpre[0] = 0;
-      }
-      __syncthreads();
-      const int total = S.cpre[rows];
-      for (int p = tid; p < total; p += 2 * kThreads) {
-        int r_[2], s_[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int pp = min(p + h * kThreads, total - 1);
-          int lo = 0, hi = rows;  // largest r with cpre[r] <= pp
-          while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (S.cpre[mid] <= pp) lo = mid; else hi = mid;
+      if constexpr (MODE == kModeTree) {
+        __syncthreads();
+        // kModeTree: exact D of the appended candidates, flattened over the block's rows
+        if (wave == 0) {
+          int c = lane < rows ? min(S.ccnt[lane], kCap) - KP : 0;
+  #pragma unroll
+          for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(c, off);
+            if (lane >= off) c += v;
           }
-          r_[h] = lo;
-          s_[h] = KP + pp - S.cpre[lo];
+          S.cpre[lane + 1] = c;
+          if (lane == 0) S.cpre[0] = 0;
         }
-        const int a0 = S.lid[row0 + r_[0]], b0 = S.lid[S.cpos[r_[0]][s_[0]]];
-        const int a1 = S.lid[row0 + r_[1]], b1 = S.lid[S.cpos[r_[1]][s_[1]]];
-        float D0, D1;
-        if ((d & 3) == 0) {
-          exact_dist4_x2(X, a0, b0, a1, b1, d, D0, D1);
-        } else {
-          D0 = exact_dist(X, a0, b0, d);
-          D1 = exact_dist(X, a1, b1, d);
-        }
-        S.cdist[r_[0]][s_[0]] = D0;
-        if (p + kThreads < total) S.cdist[r_[1]][s_[1]] = D1;
-      }
-      __syncthreads();
-      // the K best of each row's list, or the exact fallback on overflow
-#pragma unroll 1
-      for (int rr = 0; rr < kRowsPerWave; ++rr) {
-        const int r = wave * kRowsPerWave + rr;
-        if (r >= rows) break;
-        const int cnt = S.ccnt[r];
-        const int me = S.lid[row0 + r];
-        if (cnt > kCap) {
-          if (lane == 0) {
-            const int slot = atomicAdd(A.nfallback, 1);
-            if (slot < A.fb_cap) {
-              A.fallback[2 * slot] = (int32_t)item;
-              A.fallback[2 * slot + 1] = row0 + r - it.row0;
+        __syncthreads();
+        const int total = S.cpre[rows];
+        for (int p = tid; p < total; p += 2 * kThreads) {
+          int r_[2], s_[2];
+  #pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int pp = min(p + h * kThreads, total - 1);
+            int lo = 0, hi = rows;  // largest r with cpre[r] <= pp
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (S.cpre[mid] <= pp) lo = mid; else hi = mid;
             }
+            r_[h] = lo;
+            s_[h] = KP + pp - S.cpre[lo];
           }
-          continue;
+          const int a0 = S.lid[row0 + r_[0]], b0 = S.lid[S.cpos[r_[0]][s_[0]]];
+          const int a1 = S.lid[row0 + r_[1]], b1 = S.lid[S.cpos[r_[1]][s_[1]]];
+          float D0, D1;
+          if ((d & 3) == 0) {
+            exact_dist4_x2(X, a0, b0, a1, b1, d, D0, D1);
+          } else {
+            D0 = exact_dist(X, a0, b0, d);
+            D1 = exact_dist(X, a1, b1, d);
+          }
+          S.cdist[r_[0]][s_[0]] = D0;
+          if (p + kThreads < total) S.cdist[r_[1]][s_[1]] = D1;
         }
-        const uint16_t pos = lane < cnt ? S.cpos[r][lane] : kNoPos;
-        const float dv = pos == kNoPos ? INFINITY : S.cdist[r][lane];
-        const int iv = pos == kNoPos ? INT_MAX : S.lid[pos];
-        const int64_t o = ((int64_t)it.tree * n + me) * K;
-        rank_store(dv, iv, cnt, K, lane, A.out_idx + o, A.out_dist + o);
+        __syncthreads();
+        // the K best of each row's list, or the exact fallback on overflow
+  #pragma unroll 1
+        for (int rr = 0; rr < kRowsPerWave; ++rr) {
+          const int r = wave * kRowsPerWave + rr;
+          if (r >= rows) break;
+          const int cnt = S.ccnt[r];
+          const int me = S.lid[row0 + r];
+          if (cnt > kCap) {
+            if (lane == 0) {
+              const int slot = atomicAdd(A.nfallback, 1);
+              if (slot < A.fb_cap) {
+                A.fallback[2 * slot] = (int32_t)item;
+                A.fallback[2 * slot + 1] = row0 + r - it.row0;
+              }
+            }
+            continue;
+          }
+          const uint16_t pos = lane < cnt ? S.cpos[r][lane] : kNoPos;
+          const float dv = pos == kNoPos ? INFINITY : S.cdist[r][lane];
+          const int iv = pos == kNoPos ? INT_MAX : S.lid[pos];
+          const int64_t o = ((int64_t)it.tree * n + me) * K;
+          rank_store(dv, iv, cnt, K, lane, A.out_idx + o, A.out_dist + o);
+        }
+
       }
       __syncthreads();
     }
@@ -1061,6 +1198,17 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
       kd = A.known_dist[row * K + lane];
     }
     const int nknown = __popcll(__ballot(kid >= 0));
+    // valid slots per source (lane t holds source t's count; ntsrc <= 64)
+    const int ct = (A.cnt_tree != nullptr && lane < A.ntsrc)
+                       ? min(A.cnt_tree[(int64_t)lane * A.n + row], A.S) : A.S;
+    if (A.cnt_tree != nullptr && __ballot(lane < A.ntsrc && ct > 0) == 0) {
+      // no candidates: the known list is the result
+      if (lane < K) {
+        A.out_idx[row * K + lane] = kid;
+        A.out_dist[row * K + lane] = kd;
+      }
+      continue;
+    }
     // known ids first, then candidates: the first insertion of an id wins
     if (kid >= 0) {
       uint32_t h = ((uint32_t)kid * 2654435761u) >> (32 - kHashBits);
@@ -1070,9 +1218,6 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
         h = (h + 1) & (kHashSlots - 1);
       }
     }
-    // valid slots per source (lane t holds source t's count; ntsrc <= 64)
-    const int ct = (A.cnt_tree != nullptr && lane < A.ntsrc)
-                       ? min(A.cnt_tree[(int64_t)lane * A.n + row], A.S) : A.S;
     int cnt_new = 0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -1181,7 +1326,7 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
 
 // Exact K best distinct leafmates over every tree, for rows whose pool
 // overflowed: one block per row.
-constexpr int kFbThreadsRow = 128;
+constexpr int kFbThreadsRow = 256;
 template <typename XT, int KP>
 __global__ void __launch_bounds__(kFbThreadsRow)
 k_row_exact(const XT* __restrict__ X, int d, const int32_t* __restrict__ perm, int64_t n, int T,
@@ -1202,7 +1347,7 @@ k_row_exact(const XT* __restrict__ X, int d, const int32_t* __restrict__ perm, i
       const int32_t* pl = perm + (int64_t)t * n + it.start;
       for (int c = threadIdx.x; c < it.size; c += kFbThreadsRow) {
         const int id = pl[c];
-        if (id != me) top.insert_unique(exact_dist(X, me, id, d), id);
+        if (id != me) top.insert_unique(exact_dist8(X + (int64_t)me * d, X + (int64_t)id * d, d), id);
       }
     }
 #pragma unroll
@@ -1330,13 +1475,21 @@ int persistent_grid(drpt_ctx* ctx, T kernel, int threads, unsigned* grid) {
   return DRPT_OK;
 }
 
-template <typename XT, int T, int MODE>
-int launch_lowp_t(drpt_ctx* ctx, const KnnArgs& a) {
+template <typename XT, int T, int MODE, bool RING>
+int launch_lowp_r(drpt_ctx* ctx, const KnnArgs& a) {
   unsigned grid;
-  DRPT_CHECK(persistent_grid(ctx, k_knn_lowp<XT, T, MODE>, kThreads, &grid));
-  hipLaunchKernelGGL((k_knn_lowp<XT, T, MODE>), dim3(grid), dim3(kThreads), 0, ctx->stream, a);
+  DRPT_CHECK(persistent_grid(ctx, k_knn_lowp<XT, T, MODE, RING>, kThreads, &grid));
+  hipLaunchKernelGGL((k_knn_lowp<XT, T, MODE, RING>), dim3(grid), dim3(kThreads), 0, ctx->stream, a);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
+}
+
+template <typename XT, int T, int MODE>
+int launch_lowp_t(drpt_ctx* ctx, const KnnArgs& a) {
+  if constexpr (MODE != kModeTree) {
+    if ((a.d & 7) == 0 && a.d <= kGramChunk) return launch_lowp_r<XT, T, MODE, true>(ctx, a);
+  }
+  return launch_lowp_r<XT, T, MODE, false>(ctx, a);
 }
 
 template <typename XT, int T>
