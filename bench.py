@@ -87,6 +87,21 @@ def recall_fp64(X_dev, idx, K, nq=2000, seed=0):
     return hits / (len(q) * K), hits_ref / (len(q) * (K - 1))
 
 
+def pmc_traffic(config, prec, kernel):
+    """HBM bytes per launch of the dominant kernel, from the FETCH_SIZE / WRITE_SIZE
+    counter runs summarised under profiles/ (scaled by the copy-probe factors), or None
+    when no such measurement exists for this configuration."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("config") != config or t.get("precision") != prec or t.get("kernel") != kernel:
+        return None
+    return t["traffic_bytes"]
+
+
 def cpu_baseline(w, seed):
     import oracle
     import drpt_amd
@@ -195,16 +210,18 @@ def main():
     kms = phases["knn_kernel_ms"]
     flops = tm_last["gram_flops"]
     if prec == "fp32":
-        # vector FP32: 3 flops (sub, fma) per dimension per pair
-        achieved = 1.5 * flops / (kms / 1e3) / 1e12
+        # exact FP32 kernel on the vector ALU: 3 flops (sub, fma) per dimension per pair
+        fl = 1.5 * flops
+        kernel = "k_knn_exact"
     else:
-        achieved = flops / (kms / 1e3) / 1e12
-    res["roofline"] = {"bound": "mfma" if prec != "fp32" else "valu",
-                       "kernel": "k_knn_bf16" if prec == "bf16" else "k_knn_exact",
-                       "achieved": achieved, "peak": PEAK[prec], "unit": "TFLOP/s",
-                       "frac": achieved / PEAK[prec], "traffic": None,
-                       "flops_per_launch": flops if prec != "fp32" else 1.5 * flops,
-                       "kernel_ms": kms}
+        # the pool pass: the BF16 Gram of every leaf of every tree, once (MFMA)
+        fl = flops
+        kernel = "k_knn_lowp<float, 6, kModePool, ring>"
+    achieved = fl / (kms / 1e3) / 1e12
+    res["roofline"] = {"bound": "mfma", "kernel": kernel, "achieved": achieved,
+                       "peak": PEAK[prec], "unit": "TFLOP/s", "frac": achieved / PEAK[prec],
+                       "traffic": pmc_traffic(a.config, prec, kernel),
+                       "flops_per_launch": fl, "kernel_ms": kms}
     if not a.no_cpu:
         log("[bench] cpu baseline (oracle)")
         res["cpu_baseline"] = cpu_baseline(w, a.seed)

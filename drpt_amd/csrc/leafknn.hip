@@ -4,27 +4,27 @@
 // (cpp/src/drpt/algo/mdrpt.cpp:123-212, math_operations.cpp:385-396): every
 // ordered pair of leafmates, per-row ordering, head kept.
 //
-// Work list: one item per (leaf, block of 64 rows); a persistent grid pulls
-// items from an atomic queue (largest leaves first) so uneven leaves balance.
-//
 // Exact distance (the ranking every path ends with):
 //   D(x, y) = fma-chain over k = 0..d-1 of t_k = fl(x_k - y_k): acc = fma(t_k, t_k, acc)
 // Results per row: the K nearest non-self leafmates ordered by (D, index).
 //
 // Paths:
-//   FP32   D for all m^2 pairs, 64x64 tiles, 4x4 register blocking (vector ALU).
-//   BF16   Gram X~ X~^T of the bf16-rounded rows on the MFMA cores
-//          (v_mfma_f32_32x32x16_bf16, FP32 accumulate), approximate distance
-//          a = |x~_i|^2 + |x~_j|^2 - 2 G_ij, widened top-k' by (a, index),
-//          exact D for the k' candidates, top-K by (D, index). A row is
-//          certified when its K-th exact distance is below a lower bound of
-//          every non-candidate's D (derived from the (k'+1)-th approximate
-//          value, the bf16 rounding error norms and the FP32 accumulation
-//          error); uncertified rows are redone exactly (k_knn_fallback), so
-//          the lists equal the FP32 path's.
+//   FP32 (k_knn_exact)  D for all m^2 pairs, 64x64 tiles, 4x4 register
+//         blocking (vector ALU), one 64-row block per work item.
+//   BF16 (k_knn_lowp)   Gram X~ X~^T of the bf16-rounded rows on the MFMA cores
+//         (v_mfma_f32_32x32x16_bf16, FP32 accumulate) gives approximate distances
+//         a; a certified lower bound LB(a) <= D filters the leafmates, and
+//         every distance that reaches a list is the exact D above:
+//           tree mode  per-tree exact lists (bound pass + filter pass);
+//           cand mode  the k' widened candidates of each row (first trees);
+//           pool mode  filter against an external bound, survivors to pools.
+//   k_row_merge         per row: distinct candidate ids, exact D, K best merged
+//         with a known list (the graph build's first and last steps).
+//   k_knn_fallback / k_row_exact  exact redo of single rows (list or pool
+//         overflow, leaves too large to stage).
 //
 // Roofline: FP32 path vector-FP32 bound (3 flops per dim per pair); BF16 path
-// MFMA bound, algorithmic flops 2*d*sum(m^2).
+// MFMA bound, algorithmic flops 2*d*sum(m^2) per pass.
 #include <algorithm>
 #include <climits>
 
