@@ -430,13 +430,17 @@ int grow_trees_impl(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64
     DRPT_CHECK(fill_u32(ctx, (uint32_t*)S1, 0u, 2 * TN));
     DRPT_CHECK(fill_u32(ctx, (uint32_t*)S2, 0u, 2 * TN));
     DRPT_CHECK(fill_u32(ctx, H, 0u, 16 * TN));
-    // points per block: enough work to amortise flushing the per-node LDS arrays
-    int64_t ppb = std::max<int64_t>(8192, NN * 32);
-    int64_t bx = (n + ppb - 1) / ppb;
-    dim3 pgrid((unsigned)bx, (unsigned)T);
     unsigned ngrid = (unsigned)((TN + 255) / 256);
     bool lds_sum = NN <= kLdsSumNodes;
     bool lds_hist = NN <= kLdsHistNodes;
+    // points per block: with per-node LDS arrays, enough points to amortise their
+    // flush; with global atomics, small blocks so every CU has work
+    const int64_t ppb_sum = lds_sum ? std::max<int64_t>(8192, NN * 8) : 8192;
+    const int64_t ppb_hist = lds_hist ? std::max<int64_t>(8192, NN * 16) : 8192;
+    const dim3 sgrid((unsigned)((n + ppb_sum - 1) / ppb_sum), (unsigned)T);
+    const dim3 hgrid((unsigned)((n + ppb_hist - 1) / ppb_hist), (unsigned)T);
+    const int64_t ppb = ppb_sum;
+    const dim3 pgrid = sgrid;
     if (lds_sum)
       hipLaunchKernelGGL(k_sum1<true>, pgrid, dim3(kThreads), NN * 12, s, P, node, n, D, l, colmax,
                          bitlen, cnt, S1, ppb);
@@ -458,11 +462,11 @@ int grow_trees_impl(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64
                        split, E);
     DRPT_HIP(hipGetLastError());
     if (lds_hist)
-      hipLaunchKernelGGL(k_hist<true>, pgrid, dim3(kThreads), NN * 64, s, P, node, n, D, l, E,
-                         split, H, ppb);
+      hipLaunchKernelGGL(k_hist<true>, hgrid, dim3(kThreads), NN * 64, s, P, node, n, D, l, E,
+                         split, H, ppb_hist);
     else
-      hipLaunchKernelGGL(k_hist<false>, pgrid, dim3(kThreads), 0, s, P, node, n, D, l, E, split, H,
-                         ppb);
+      hipLaunchKernelGGL(k_hist<false>, hgrid, dim3(kThreads), 0, s, P, node, n, D, l, E, split, H,
+                         ppb_hist);
     DRPT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_median, dim3(ngrid), dim3(256), 0, s, T, l, cnt, H, E, split, med);
     DRPT_HIP(hipGetLastError());
