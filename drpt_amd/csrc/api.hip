@@ -239,7 +239,7 @@ int drpt_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, cons
   DRPT_CHECK(upload(ctx, "proj.colptr", colptr, &dcp));
   DRPT_CHECK(upload(ctx, "proj.rowidx", rowidx, &dri));
   DRPT_CHECK(upload(ctx, "proj.w", w, &dw));
-  return launch_project(ctx, X, dtype, n, d, dcp, dri, dw, L, P, colmax);
+  return launch_project(ctx, X, dtype, n, d, dcp, dri, dw, L, colptr[L], P, colmax);
 }
 
 int drpt_grow_trees(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t n, int ntrees,
@@ -651,7 +651,7 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   DRPT_HIP(hipStreamSynchronize(s));
   if (nov > 0)
     DRPT_CHECK(launch_row_exact(ctx, &pp, X, G.perm, G.items, G.range + 4, rows_per_item, olist,
-                                novf, n, out_idx, out_dist));
+                                novf, nov, out_idx, out_dist));
   DRPT_HIP(hipEventRecord(ctx->ev[5], s));
   DRPT_CHECK(leaf_stats(ctx, p, G, timings));
   if (timings) {

@@ -65,7 +65,8 @@ int fill_u32(drpt_ctx* ctx, uint32_t* p, uint32_t v, size_t count);
 
 // project.hip
 int launch_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, const int32_t* colptr,
-                   const uint16_t* rowidx, const float* w, int L, float* P, uint32_t* colmax);
+                   const uint16_t* rowidx, const float* w, int L, int nnz, float* P,
+                   uint32_t* colmax);
 
 // split.hip
 int grow_trees_impl(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t n, int T, int D,

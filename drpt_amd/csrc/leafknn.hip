@@ -27,6 +27,7 @@
 // MFMA bound, algorithmic flops 2*d*sum(m^2) per pass.
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "internal.hpp"
 
@@ -1266,9 +1267,10 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
     int neval = cnt_new;
     if (A.rerank > 0 && cnt_new > A.rerank) {
       if (cnt_new <= 64) {
-        uint64_t k1[1] = {key[0]};
-        bitonic_wave<uint64_t, 1>(k1, lane);
-        key[0] = k1[0];
+        // approximate bits (low 11 bits zero) | dense index: a 32-bit sort
+        uint32_t k1[1] = {lane < cnt_new ? (uint32_t)(key[0] >> 32) | (uint32_t)lane : 0xFFFFFFFFu};
+        bitonic_wave<uint32_t, 1>(k1, lane);
+        key[0] = lane < A.rerank ? (uint64_t)dn[k1[0] & 0x7FFu] : ~0ull;
       } else {
         bitonic_wave<uint64_t, E>(key, lane);
       }
@@ -1325,46 +1327,83 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
 }
 
 // Exact K best distinct leafmates over every tree, for rows whose pool
-// overflowed: one block per row.
+// overflowed. k_row_exact_tree: one block per (row, tree) pair, the row's
+// exact KP best leafmates in that tree (per-thread lists, pairwise merged in
+// LDS); k_row_exact_merge: per row, the T sorted lists merged without
+// duplicates.
 constexpr int kFbThreadsRow = 256;
 template <typename XT, int KP>
 __global__ void __launch_bounds__(kFbThreadsRow)
-k_row_exact(const XT* __restrict__ X, int d, const int32_t* __restrict__ perm, int64_t n, int T,
-            const int32_t* __restrict__ invperm, const int32_t* __restrict__ rowitem,
-            const LeafItem* __restrict__ items, const int32_t* __restrict__ rows,
-            const int32_t* __restrict__ nrows, int64_t nrows_max, int K,
-            int32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+k_row_exact_tree(const XT* __restrict__ X, int d, const int32_t* __restrict__ perm, int64_t n,
+                 int T, const int32_t* __restrict__ invperm, const int32_t* __restrict__ rowitem,
+                 const LeafItem* __restrict__ items, const int32_t* __restrict__ rows,
+                 const int32_t* __restrict__ nrows, int64_t nrows_max,
+                 int32_t* __restrict__ tmp_i, float* __restrict__ tmp_d) {
   __shared__ float ld[kFbThreadsRow][KP];
   __shared__ int li[kFbThreadsRow][KP];
-  const int64_t total = min((int64_t)*nrows, nrows_max);
-  for (int64_t e = blockIdx.x; e < total; e += gridDim.x) {
+  const int tid = threadIdx.x;
+  const int64_t total = min((int64_t)*nrows, nrows_max) * T;
+  for (int64_t p = blockIdx.x; p < total; p += gridDim.x) {
+    const int64_t e = p / T;
+    const int t = (int)(p - e * T);
     const int me = rows[e];
+    const int pos = invperm[(int64_t)t * n + me];
+    const LeafItem it = items[rowitem[(int64_t)t * n + pos]];
+    const int32_t* pl = perm + (int64_t)t * n + it.start;
     TopK<KP> top;
     top.init();
-    for (int t = 0; t < T; ++t) {
-      const int pos = invperm[(int64_t)t * n + me];
-      const LeafItem it = items[rowitem[(int64_t)t * n + pos]];
-      const int32_t* pl = perm + (int64_t)t * n + it.start;
-      for (int c = threadIdx.x; c < it.size; c += kFbThreadsRow) {
-        const int id = pl[c];
-        if (id != me) top.insert_unique(exact_dist8(X + (int64_t)me * d, X + (int64_t)id * d, d), id);
-      }
+    for (int c = tid; c < it.size; c += kFbThreadsRow) {  // leafmates of one tree are distinct
+      const int id = pl[c];
+      if (id != me) top.insert(exact_dist8(X + (int64_t)me * d, X + (int64_t)id * d, d), id);
     }
 #pragma unroll
     for (int s = 0; s < KP; ++s) {
-      ld[threadIdx.x][s] = top.d[s];
-      li[threadIdx.x][s] = top.i[s];
+      ld[tid][s] = top.d[s];
+      li[tid][s] = top.i[s];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int t = 1; t < kFbThreadsRow; ++t)
-        for (int s = 0; s < KP; ++s) {
-          if (li[t][s] == INT_MAX) break;
-          top.insert_unique(ld[t][s], li[t][s]);
+    for (int h = kFbThreadsRow / 2; h > 0; h >>= 1) {
+      if (tid < h) {
+        for (int s = 0; s < KP; ++s) {  // partner's list is sorted
+          const int pi = li[tid + h][s];
+          if (pi == INT_MAX || !better(ld[tid + h][s], pi, top.d[KP - 1], top.i[KP - 1])) break;
+          top.insert(ld[tid + h][s], pi);
         }
-      top.store(K, out_idx + (int64_t)me * K, out_dist + (int64_t)me * K);
+#pragma unroll
+        for (int s = 0; s < KP; ++s) {
+          ld[tid][s] = top.d[s];
+          li[tid][s] = top.i[s];
+        }
+      }
+      __syncthreads();
+    }
+    if (tid < KP) {
+      tmp_i[p * KP + tid] = li[0][tid];
+      tmp_d[p * KP + tid] = ld[0][tid];
     }
     __syncthreads();
+  }
+}
+
+template <int KP>
+__global__ void k_row_exact_merge(int T, int K, const int32_t* __restrict__ rows,
+                                  const int32_t* __restrict__ nrows, int64_t nrows_max,
+                                  const int32_t* __restrict__ tmp_i, const float* __restrict__ tmp_d,
+                                  int32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+  const int64_t total = min((int64_t)*nrows, nrows_max);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    TopK<KP> top;
+    top.init();
+    for (int t = 0; t < T; ++t)
+      for (int s = 0; s < KP; ++s) {
+        const int64_t o = (e * T + t) * KP + s;
+        const int id = tmp_i[o];
+        if (id == INT_MAX) break;
+        top.insert_unique(tmp_d[o], id);
+      }
+    const int me = rows[e];
+    top.store(K, out_idx + (int64_t)me * K, out_dist + (int64_t)me * K);
   }
 }
 
@@ -1649,28 +1688,30 @@ int launch_row_exact(drpt_ctx* ctx, const drpt_params* p, const void* X, const i
   hipLaunchKernelGGL(k_rowitem, dim3((unsigned)ctx->num_cus * 8), dim3(256), 0, ctx->stream, items,
                      range, rows_per_item, n, rowitem);
   DRPT_HIP(hipGetLastError());
-  const unsigned grid = (unsigned)(ctx->num_cus * 2);
-  if (p->dtype == DRPT_U8) {
-    if (K <= 16)
-      hipLaunchKernelGGL((k_row_exact<uint8_t, 16>), dim3(grid), dim3(kFbThreadsRow), 0, ctx->stream,
-                         (const uint8_t*)X, p->d, perm, n, T, inv, rowitem, items, ovf_list, novf,
-                         nrows_max, K, out_idx, out_dist);
+  int32_t* ti;
+  float* td;
+  DRPT_CHECK(ws(ctx, "knn.rowx_i", (size_t)nrows_max * T * 32, &ti));
+  DRPT_CHECK(ws(ctx, "knn.rowx_d", (size_t)nrows_max * T * 32, &td));
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nrows_max * T, (int64_t)ctx->num_cus * 4));
+  const unsigned mgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nrows_max + 255) / 256, 1024));
+  auto go = [&](auto kp) -> int {
+    constexpr int KP = decltype(kp)::value;
+    if (p->dtype == DRPT_U8)
+      hipLaunchKernelGGL((k_row_exact_tree<uint8_t, KP>), dim3(grid), dim3(kFbThreadsRow), 0,
+                         ctx->stream, (const uint8_t*)X, p->d, perm, n, T, inv, rowitem, items,
+                         ovf_list, novf, nrows_max, ti, td);
     else
-      hipLaunchKernelGGL((k_row_exact<uint8_t, 32>), dim3(grid), dim3(kFbThreadsRow), 0, ctx->stream,
-                         (const uint8_t*)X, p->d, perm, n, T, inv, rowitem, items, ovf_list, novf,
-                         nrows_max, K, out_idx, out_dist);
-  } else {
-    if (K <= 16)
-      hipLaunchKernelGGL((k_row_exact<float, 16>), dim3(grid), dim3(kFbThreadsRow), 0, ctx->stream,
-                         (const float*)X, p->d, perm, n, T, inv, rowitem, items, ovf_list, novf,
-                         nrows_max, K, out_idx, out_dist);
-    else
-      hipLaunchKernelGGL((k_row_exact<float, 32>), dim3(grid), dim3(kFbThreadsRow), 0, ctx->stream,
-                         (const float*)X, p->d, perm, n, T, inv, rowitem, items, ovf_list, novf,
-                         nrows_max, K, out_idx, out_dist);
-  }
-  DRPT_HIP(hipGetLastError());
-  return DRPT_OK;
+      hipLaunchKernelGGL((k_row_exact_tree<float, KP>), dim3(grid), dim3(kFbThreadsRow), 0,
+                         ctx->stream, (const float*)X, p->d, perm, n, T, inv, rowitem, items,
+                         ovf_list, novf, nrows_max, ti, td);
+    DRPT_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_row_exact_merge<KP>), dim3(mgrid), dim3(256), 0, ctx->stream, T, K,
+                       ovf_list, novf, nrows_max, ti, td, out_idx, out_dist);
+    DRPT_HIP(hipGetLastError());
+    return DRPT_OK;
+  };
+  if (K <= 16) return go(std::integral_constant<int, 16>());
+  return go(std::integral_constant<int, 32>());
 }
 
 int launch_fallback(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,

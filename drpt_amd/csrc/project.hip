@@ -23,6 +23,11 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// bytes of the LDS header before the X tile (16-byte aligned)
+__host__ __device__ inline size_t r_bytes(int L, int nnz) {
+  return ((size_t)L * 4 + (size_t)(L + 1) * 4 + (size_t)nnz * 6 + 15) & ~(size_t)15;
+}
+
 // TR rows per tile; lane -> (row = lane % TR, column slot = lane / TR).
 // Grid-stride over row tiles; the per-column max |P| is kept in LDS and
 // flushed once per block.
@@ -32,9 +37,20 @@ k_project(const XT* __restrict__ X, int64_t n, int d, int L, const int32_t* __re
           const uint16_t* __restrict__ rowidx, const float* __restrict__ w, float* __restrict__ P,
           uint32_t* __restrict__ colmax, int pitch) {
   extern __shared__ __align__(16) unsigned char smem[];
+  // LDS: column maxima [L], R's compressed columns (colptr [L+1], weights, row
+  // indices), then the X tile
+  const int nnz = colptr[L];
   uint32_t* lmax = reinterpret_cast<uint32_t*>(smem);
-  XT* tile = reinterpret_cast<XT*>(smem + (((size_t)L * 4 + 15) & ~(size_t)15));
+  int32_t* lcp = reinterpret_cast<int32_t*>(lmax + L);
+  float* lw = reinterpret_cast<float*>(lcp + L + 1);
+  uint16_t* lri = reinterpret_cast<uint16_t*>(lw + nnz);
+  XT* tile = reinterpret_cast<XT*>(smem + r_bytes(L, nnz));
   for (int c = threadIdx.x; c < L; c += kThreads) lmax[c] = 0u;
+  for (int c = threadIdx.x; c <= L; c += kThreads) lcp[c] = colptr[c];
+  for (int k = threadIdx.x; k < nnz; k += kThreads) {
+    lw[k] = w[k];
+    lri[k] = rowidx[k];
+  }
   constexpr int kSlots = 64 / TR;  // columns a wave works on at once
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -70,14 +86,19 @@ k_project(const XT* __restrict__ X, int64_t n, int d, int L, const int32_t* __re
       const int c = cb + slot;
       float pv = 0.0f;
       if (c < L) {
-        const int beg = colptr[c], end = colptr[c + 1];
+        const int beg = lcp[c], end = lcp[c + 1];
         double acc = 0.0;
         const XT* xr = tile + r * pitch;
-        for (int k = beg; k < end; ++k) {
-          double x = (double)xr[rowidx[k]];
-          double prod = __dmul_rn(x, (double)w[k]);
-          acc = __dadd_rn(acc, prod);
+        int k = beg;
+        for (; k + 4 <= end; k += 4) {  // independent loads, sums still in ascending k
+          const double x0 = (double)xr[lri[k]], x1 = (double)xr[lri[k + 1]];
+          const double x2 = (double)xr[lri[k + 2]], x3 = (double)xr[lri[k + 3]];
+          acc = __dadd_rn(acc, __dmul_rn(x0, (double)lw[k]));
+          acc = __dadd_rn(acc, __dmul_rn(x1, (double)lw[k + 1]));
+          acc = __dadd_rn(acc, __dmul_rn(x2, (double)lw[k + 2]));
+          acc = __dadd_rn(acc, __dmul_rn(x3, (double)lw[k + 3]));
         }
+        for (; k < end; ++k) acc = __dadd_rn(acc, __dmul_rn((double)xr[lri[k]], (double)lw[k]));
         pv = (float)acc;
         if (valid_row) P[(int64_t)c * n + row0 + r] = pv;
       }
@@ -94,7 +115,7 @@ k_project(const XT* __restrict__ X, int64_t n, int d, int L, const int32_t* __re
 
 template <typename XT, int TR>
 int run(drpt_ctx* ctx, const XT* X, int64_t n, int d, const int32_t* colptr, const uint16_t* rowidx,
-        const float* w, int L, float* P, uint32_t* colmax) {
+        const float* w, int L, int nnz, float* P, uint32_t* colmax) {
   int pitch;
   if (sizeof(XT) == 1) {
     int words = (d + 3) / 4;
@@ -103,7 +124,7 @@ int run(drpt_ctx* ctx, const XT* X, int64_t n, int d, const int32_t* colptr, con
   } else {
     pitch = d | 1;
   }
-  size_t lds = (((size_t)L * 4 + 15) & ~(size_t)15) + (size_t)TR * pitch * sizeof(XT);
+  size_t lds = r_bytes(L, nnz) + (size_t)TR * pitch * sizeof(XT);
   if (lds > 64 * 1024) {
     set_error("project: LDS tile too large for this dimension / column count");
     return DRPT_ERR_UNSUPPORTED;
@@ -118,19 +139,24 @@ int run(drpt_ctx* ctx, const XT* X, int64_t n, int d, const int32_t* colptr, con
 }  // namespace
 
 int launch_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, const int32_t* colptr,
-                   const uint16_t* rowidx, const float* w, int L, float* P, uint32_t* colmax) {
+                   const uint16_t* rowidx, const float* w, int L, int nnz, float* P,
+                   uint32_t* colmax) {
   DRPT_CHECK(fill_u32(ctx, colmax, 0u, (size_t)L));
   if (n <= 0) return DRPT_OK;
-  const size_t kLdsBudget = 64 * 1024 - (((size_t)L * 4 + 15) & ~(size_t)15);
+  const size_t kLdsBudget = 64 * 1024 - r_bytes(L, nnz);
+  if (r_bytes(L, nnz) >= 64 * 1024) {
+    set_error("project: projection matrix too large for LDS");
+    return DRPT_ERR_UNSUPPORTED;
+  }
   if (dtype == DRPT_U8) {
-    return run<uint8_t, 64>(ctx, (const uint8_t*)X, n, d, colptr, rowidx, w, L, P, colmax);
+    return run<uint8_t, 64>(ctx, (const uint8_t*)X, n, d, colptr, rowidx, w, L, nnz, P, colmax);
   }
   const float* Xf = (const float*)X;
   size_t row_bytes = (size_t)((d | 1)) * 4;
-  if (64 * row_bytes <= kLdsBudget) return run<float, 64>(ctx, Xf, n, d, colptr, rowidx, w, L, P, colmax);
-  if (32 * row_bytes <= kLdsBudget) return run<float, 32>(ctx, Xf, n, d, colptr, rowidx, w, L, P, colmax);
-  if (16 * row_bytes <= kLdsBudget) return run<float, 16>(ctx, Xf, n, d, colptr, rowidx, w, L, P, colmax);
-  if (8 * row_bytes <= kLdsBudget) return run<float, 8>(ctx, Xf, n, d, colptr, rowidx, w, L, P, colmax);
+  if (64 * row_bytes <= kLdsBudget) return run<float, 64>(ctx, Xf, n, d, colptr, rowidx, w, L, nnz, P, colmax);
+  if (32 * row_bytes <= kLdsBudget) return run<float, 32>(ctx, Xf, n, d, colptr, rowidx, w, L, nnz, P, colmax);
+  if (16 * row_bytes <= kLdsBudget) return run<float, 16>(ctx, Xf, n, d, colptr, rowidx, w, L, nnz, P, colmax);
+  if (8 * row_bytes <= kLdsBudget) return run<float, 8>(ctx, Xf, n, d, colptr, rowidx, w, L, nnz, P, colmax);
   set_error("project: dimension too large for the LDS tile");
   return DRPT_ERR_UNSUPPORTED;
 }
