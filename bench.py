@@ -9,8 +9,8 @@ input (its lists must equal the BF16 ones bit for bit), recall@10 against an
 FP64 brute force on sampled queries, the roofline of the leaf k-NN kernel and
 the oracle (CPU restatement, OpenMP) on a bounded sample of the same workload.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c3]
-                    [--precision bf16|fp32] [--no-cpu] [--no-fp32]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c3|c4]
+                    [--precision bf16|fp8|fp32] [--no-cpu] [--no-fp32]
 """
 import argparse
 import json
@@ -47,6 +47,11 @@ def workload(name):
         return dict(desc="1000000x960 float32 1000-centre Gaussian mixture, T=12, nn=10, "
                          "leaf<=2048", gen=lambda n: datagen.gauss_mixture(n), n=1_000_000,
                     ntrees=12, nn=10, leaf_max=2048, kprime=24, mode="leafmax", cpu_n=20_000)
+    if name == "c4":
+        return dict(desc="10000000x96 float32 unit-L2 4096-centre mixture, T=32, nn=32, "
+                         "leaf<=1024, FP8 k'=64", gen=lambda n: datagen.sphere_mixture(n),
+                    n=10_000_000, ntrees=32, nn=32, leaf_max=1024, kprime=64, mode="leafmax",
+                    cpu_n=50_000)
     raise SystemExit(f"unknown config {name}")
 
 
@@ -64,8 +69,10 @@ def make_params(w, n, d, dtype_u8, precision, seed=1):
     return p
 
 
-def recall_fp64(X_dev, idx, K, nq=2000, seed=0):
-    """Tie-aware recall@K of idx (n x K, non-self) against FP64 brute force."""
+def recall_fp64(X_dev, idx, K, nq=2000, seed=0, dist=None):
+    """Tie-aware recall@K of idx (n x K, non-self) against FP64 brute force; with dist,
+    also the largest relative error of the returned distances against the FP64
+    squared distance of the same pairs."""
     import torch
     n = X_dev.shape[0]
     rng = np.random.default_rng(seed)
@@ -73,6 +80,7 @@ def recall_fp64(X_dev, idx, K, nq=2000, seed=0):
     X64 = X_dev.double()
     x2 = (X64 * X64).sum(1)
     hits = hits_ref = 0
+    err = 0.0
     for s in range(0, len(q), 256):
         qq = q[s:s + 256]
         d = x2[qq][:, None] + x2[None, :] - 2.0 * (X64[qq] @ X64.T)
@@ -84,6 +92,15 @@ def recall_fp64(X_dev, idx, K, nq=2000, seed=0):
         hits += int(((dg <= kth[:, None]) & valid).sum())
         kth9 = torch.topk(d, K - 1, dim=1, largest=False).values[:, K - 2]
         hits_ref += int(((dg[:, :K - 1] <= kth9[:, None]) & valid[:, :K - 1]).sum())
+        if dist is not None:
+            diff = X64[qq][:, None, :] - X64[got.clamp(min=0)]
+            d64 = (diff * diff).sum(2)
+            got_d = dist[qq].double()
+            rel = ((got_d - d64).abs() / d64.clamp(min=1e-300))[valid]
+            if rel.numel():
+                err = max(err, float(rel.max()))
+    if dist is not None:
+        return hits / (len(q) * K), hits_ref / (len(q) * (K - 1)), err
     return hits / (len(q) * K), hits_ref / (len(q) * (K - 1))
 
 
@@ -151,7 +168,7 @@ def main():
         return
 
     w = workload(a.config)
-    prec = a.precision or ("fp32" if a.config == "c1" else "bf16")
+    prec = a.precision or {"c1": "fp32", "c4": "fp8"}.get(a.config, "bf16")
     log(f"[bench] generating {w['desc']}")
     X = w["gen"](w["n"])
     n, d = X.shape
@@ -195,9 +212,10 @@ def main():
            "fallback_rows": tm_last["n_fallback_rows"]}
 
     log("[bench] recall")
-    r, rref = recall_fp64(Xd, idx_main, p.nn)
+    r, rref, derr = recall_fp64(Xd, idx_main, p.nn, dist=dist_main)
     res["recall_at_10"] = r
     res["recall_ref_style_nn_minus_1"] = rref
+    res["dist_max_rel_err_vs_fp64"] = derr
 
     if prec != "fp32" and not a.no_fp32:
         p32 = make_params(w, n, d, X.dtype == np.uint8, "fp32", a.seed)
@@ -214,9 +232,14 @@ def main():
         fl = 1.5 * flops
         kernel = "k_knn_exact"
     else:
-        # the pool pass: the BF16 Gram of every leaf of every tree, once (MFMA)
+        # the pool pass: the BF16 / FP8 Gram of every leaf of every tree, once (MFMA)
         fl = flops
-        kernel = "k_knn_lowp<float, 6, kModePool, ring>"
+        t = (p.kprime + 3) // 4
+        t = next(s for s in (3, 4, 6, 8, 12, 16) if t <= s)
+        el = "ElFp8" if prec == "fp8" else "ElBf16"
+        rb = d * (1 if prec == "fp8" else 2)  # bytes per quantised row
+        path = "ring" if rb % 16 == 0 and rb <= 256 else "regs"
+        kernel = f"k_knn_lowp<float, {el}, {t}, kModePool, {path}>"
     achieved = fl / (kms / 1e3) / 1e12
     res["roofline"] = {"bound": "mfma", "kernel": kernel, "achieved": achieved,
                        "peak": PEAK[prec], "unit": "TFLOP/s", "frac": achieved / PEAK[prec],

@@ -44,9 +44,11 @@ def derive_params(n, d, ntrees, nn, tree_depth_ratio=0.6):
     return p
 
 
-def leafmax_params(n, d, ntrees, nn, leaf_max, precision="fp32", kprime=0, seed=1, depth=None):
+def leafmax_params(n, d, ntrees, nn, leaf_max, precision="fp32", kprime=0, seed=1, depth=None,
+                   tree_group=0):
     """This build's leaf-size-capped configuration: nodes larger than leaf_max
-    are split with the reference split rule; depth = projection columns per tree."""
+    are split with the reference split rule; depth = projection columns per tree;
+    tree_group = trees per pooled filter pass of BF16/FP8 builds (0 = automatic)."""
     p = derive_params(n, d, ntrees, nn)
     p.local_mode = DRPT_LOCAL_LEAFMAX
     p.leaf_max = leaf_max
@@ -56,6 +58,7 @@ def leafmax_params(n, d, ntrees, nn, leaf_max, precision="fp32", kprime=0, seed=
     p.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
     p.kprime = kprime
     p.seed = seed
+    p.tree_group = tree_group
     return p
 
 
@@ -112,10 +115,16 @@ class Context:
         self.sync()
         return P, cm
 
-    def grow_trees(self, P, colmax, ntrees, depth, tree_mode=DRPT_TREE_FIXED, leaf_max=0):
+    def grow_trees(self, P, colmax, ntrees, depth, tree_mode=DRPT_TREE_FIXED, leaf_max=0,
+                   out=None):
+        """Leaf heap index of every point in every tree, [ntrees][n] (into `out` when
+        given: the same buffer lets the context replay its captured split graph)."""
         torch = _torch()
         L, n = P.shape
-        heap = torch.empty((ntrees, n), dtype=torch.int32, device=P.device)
+        heap = out if out is not None else torch.empty((ntrees, n), dtype=torch.int32,
+                                                       device=P.device)
+        if heap.shape != (ntrees, n) or heap.dtype != torch.int32 or not heap.is_contiguous():
+            raise ValueError("out must be a contiguous int32 tensor of shape (ntrees, n)")
         torch.cuda.synchronize(P.device)
         _lib.check(self.lib.drpt_grow_trees(self.h, _ptr(P), _ptr(colmax), n, ntrees, depth,
                                             tree_mode, leaf_max, _ptr(heap)))

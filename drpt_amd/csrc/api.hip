@@ -6,6 +6,7 @@
 // every MPI rank; here one context drives one GPU and one stream.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -91,9 +92,13 @@ int validate(const drpt_params* p) {
     set_error("leaf_max must be >= 1 in leaf-max mode");
     return DRPT_ERR_ARG;
   }
-  if (p->precision != DRPT_FP32 && p->precision != DRPT_BF16) {
+  if (p->precision != DRPT_FP32 && p->precision != DRPT_BF16 && p->precision != DRPT_FP8) {
     set_error("precision not supported by this build");
     return DRPT_ERR_UNSUPPORTED;
+  }
+  if (p->tree_group < 0) {
+    set_error("tree_group must be >= 0");
+    return DRPT_ERR_ARG;
   }
   return DRPT_OK;
 }
@@ -141,6 +146,7 @@ int drpt_create(drpt_ctx** out, int device) {
   ctx->device = device;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  if (const char* g = getenv("DRPT_GRAPHS")) ctx->use_graphs = strcmp(g, "0") != 0;
   if (hipStreamCreate(&ctx->stream) != hipSuccess) {
     delete ctx;
     set_error("hipStreamCreate failed");
@@ -160,9 +166,12 @@ int drpt_destroy(drpt_ctx* ctx) {
   if (!ctx) return DRPT_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->graphs)
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
   for (auto& kv : ctx->bufs) (void)hipFree(kv.second.first);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->ev_pass) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return DRPT_OK;
@@ -264,6 +273,7 @@ struct Grouping {
   const int32_t* perm = nullptr;
   LeafItem* items = nullptr;
   int64_t* range = nullptr;  // device {0, first, first, all, 0, all}
+  const int64_t* base = nullptr;  // device [T + 1]: first work item of each tree
   unsigned long long* stats = nullptr;
   double host_gram = -1.0;
   int64_t host_leaves = 0, host_max = 0;
@@ -406,6 +416,10 @@ static int group_impl(drpt_ctx* ctx, const drpt_params* p, const uint32_t* leaf_
     }
     const int64_t ni = (int64_t)hitems.size();
     std::vector<int64_t> hr = {0, nfirst, nfirst, ni, 0, ni};
+    std::vector<int64_t> hbase((size_t)T + 1, 0);
+    for (auto& it : hitems) ++hbase[(size_t)it.tree + 1];
+    for (int t = 0; t < T; ++t) hbase[(size_t)t + 1] += hbase[(size_t)t];
+    DRPT_HIP(hipMemcpyAsync(base, hbase.data(), hbase.size() * 8, hipMemcpyHostToDevice, s));
     DRPT_HIP(hipMemcpyAsync(items, hitems.data(), hitems.size() * sizeof(LeafItem),
                             hipMemcpyHostToDevice, s));
     DRPT_HIP(hipMemcpyAsync(range, hr.data(), 6 * 8, hipMemcpyHostToDevice, s));
@@ -415,6 +429,7 @@ static int group_impl(drpt_ctx* ctx, const drpt_params* p, const uint32_t* leaf_
   G->perm = perm_final;
   G->items = items;
   G->range = range;
+  G->base = base;
   G->stats = stats;
   G->host_gram = host_gram;
   G->host_leaves = host_leaves;
@@ -440,21 +455,29 @@ static int leaf_stats(drpt_ctx* ctx, const drpt_params* p, const Grouping& G, dr
   return DRPT_OK;
 }
 
-// bf16 copy, |x~|^2 and |x~ - x| per point, then their leaf-ordered copies
+// quantised copy (bf16, or fp8 e4m3 with a power-of-two scale per row),
+// |x~|^2 and |x~ - x| per point, then their leaf-ordered copies
 static int prep_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
                      KnnArgs* a) {
   const int64_t n = p->n;
-  uint16_t* xb;
+  const bool fp8 = p->precision == DRPT_FP8;
+  uint8_t* xq;
   float *nb, *er, *lnbT, *lerrT;
-  DRPT_CHECK(ws(ctx, "lowp.x", n * p->d, &xb));
+  int8_t *pexp = nullptr, *lexpT = nullptr;
+  DRPT_CHECK(ws(ctx, "lowp.x", (size_t)n * p->d * (fp8 ? 1 : 2), &xq));
   DRPT_CHECK(ws(ctx, "lowp.nrm2", n, &nb));
   DRPT_CHECK(ws(ctx, "lowp.err", n, &er));
-  DRPT_CHECK(launch_prep_lowp(ctx, X, p->dtype, n, p->d, p->precision, xb, nb, er, nullptr));
+  if (fp8) DRPT_CHECK(ws(ctx, "lowp.pexp", n, &pexp));
+  DRPT_CHECK(launch_prep_lowp(ctx, X, p->dtype, n, p->d, p->precision, xq, nb, er, pexp));
   DRPT_CHECK(ws(ctx, "lowp.lnbT", (size_t)p->ntrees * n, &lnbT));
   DRPT_CHECK(ws(ctx, "lowp.lerrT", (size_t)p->ntrees * n, &lerrT));
-  DRPT_CHECK(launch_leaf_gather(ctx, perm, (int64_t)p->ntrees * n, nb, er, lnbT, lerrT));
+  if (fp8) DRPT_CHECK(ws(ctx, "lowp.lexpT", (size_t)p->ntrees * n, &lexpT));
+  DRPT_CHECK(launch_leaf_gather(ctx, perm, (int64_t)p->ntrees * n, nb, er, pexp, lnbT, lerrT,
+                                lexpT));
   a->X = X;
-  a->Xb = xb;
+  a->Xq = xq;
+  a->precision = p->precision;
+  a->lexpT = lexpT;
   a->lnbT = lnbT;
   a->lerrT = lerrT;
   a->perm = perm;
@@ -542,8 +565,11 @@ static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
 
 // Trees whose merged widened candidates bound the K-th distance of every row.
 constexpr int kFirstTrees = 4;
-// Pool slots per row (more survivors send the row to the exact row path).
-constexpr int kPoolCap = 256;
+// Pool slots per row (more survivors send the row to the exact row path): the
+// pool pass drops a row's known ids when K <= 16 (ids staged in LDS); with
+// larger K they stay in the pool and the row merge drops them, so the pool
+// is larger.
+inline int pool_cap_of(int K) { return K <= 16 ? 256 : 512; }
 
 // Low-precision graph (build_graph): the final lists straight from the leaves.
 //  1. kModeCand on trees < a: k' widened candidates per row and tree (with
@@ -555,6 +581,13 @@ constexpr int kPoolCap = 256;
 //     list go to the row's pool (every member of the final top K passes)
 //  4. row merge of pools and known list -> final lists; rows whose pool
 //     overflowed are redone exactly over every tree
+// Trees per pooled filter pass: a row's pool holds the survivors of every
+// tree of the group (a neighbour repeats once per tree whose leaf holds it),
+// so larger K takes smaller groups; one group when K <= 10.
+static int tree_group_auto(int T, int K) {
+  return std::max(1, std::min(T, std::max(4, 160 / std::max(K, 1))));
+}
+
 static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
                           const uint32_t* leaf_heap, int32_t* out_idx, float* out_dist,
                           drpt_timings* timings) {
@@ -582,7 +615,8 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   DRPT_CHECK(ws(ctx, "knn.known_idx", (size_t)n * K, &ki));
   DRPT_CHECK(ws(ctx, "knn.known_dist", (size_t)n * K, &kd));
   DRPT_CHECK(ws(ctx, "knn.boundT", (size_t)T * n, &boundT));
-  DRPT_CHECK(ws(ctx, "knn.pool", (size_t)n * kPoolCap, &pool));
+  const int pool_cap = pool_cap_of(K);
+  DRPT_CHECK(ws(ctx, "knn.pool", (size_t)n * pool_cap, &pool));
   DRPT_CHECK(ws(ctx, "knn.pool_cnt", n, &pcnt));
   DRPT_CHECK(fill_u32(ctx, (uint32_t*)pcnt, 0u, n));
   DRPT_CHECK(ws(ctx, "knn.ovf_flag", n, &oflag));
@@ -613,39 +647,67 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   m.out_dist = kd;
   m.order = G.perm;  // tree 0's leaf order
   DRPT_CHECK(launch_row_merge(ctx, p->dtype, m));
-  DRPT_CHECK(launch_leaf_gather_val(ctx, G.perm, (int64_t)T * n, kd, K, K - 1, boundT));
-  // 3. pools over every tree
-  DRPT_HIP(hipEventRecord(ctx->ev[9], s));
-  a.range = G.range + 4;
+  // 3. pools over groups of trees, each merged into the known lists before
+  // the next group (duplicates across the group's trees removed, the bound
+  // tightened); the last merge writes the output
+  const int gsz = p->tree_group > 0 ? std::min(p->tree_group, T) : tree_group_auto(T, K);
+  const int ngroups = (T + gsz - 1) / gsz;
+  int64_t* granges;
+  int32_t* ki2 = nullptr;
+  float* kd2 = nullptr;
+  DRPT_CHECK(ws(ctx, "knn.granges", 2 * (size_t)ngroups, &granges));
+  if (ngroups > 1) {
+    DRPT_CHECK(ws(ctx, "knn.known_idx2", (size_t)n * K, &ki2));
+    DRPT_CHECK(ws(ctx, "knn.known_dist2", (size_t)n * K, &kd2));
+  }
+  while ((int)ctx->ev_pass.size() < 2 * ngroups) {
+    hipEvent_t e;
+    DRPT_HIP(hipEventCreate(&e));
+    ctx->ev_pass.push_back(e);
+  }
+  for (int g = 0; g < ngroups; ++g)
+    DRPT_CHECK(launch_tree_range(ctx, G.base, g * gsz, std::min(T, (g + 1) * gsz), granges + 2 * g));
   a.out_idx = pool;
   a.out_dist = nullptr;
   a.boundT = boundT;
-  a.known = ki;
   a.pool_cnt = pcnt;
-  a.pool_cap = kPoolCap;
+  a.pool_cap = pool_cap;
   a.ovf_flag = oflag;
   a.ovf_list = olist;
   a.novf = novf;
-  DRPT_CHECK(launch_lowp_knn(ctx, &pp, kModePool, a));
-  DRPT_HIP(hipEventRecord(ctx->ev[8], s));
-  // 4. final lists
-  RowMergeArgs f{};
-  f.X = X;
-  f.n = n;
-  f.d = p->d;
-  f.K = K;
-  f.src = pool;
-  f.src_tree_stride = 0;
-  f.S = kPoolCap;
-  f.ntsrc = 1;
-  f.cnt_tree = pcnt;
-  f.known_idx = ki;
-  f.known_dist = kd;
-  f.skip_flag = oflag;
-  f.out_idx = out_idx;
-  f.out_dist = out_dist;
-  f.order = G.perm;
-  DRPT_CHECK(launch_row_merge(ctx, p->dtype, f));
+  int32_t* kin = ki;  // known lists in, merged lists out (ping-pong)
+  float* kdn = kd;
+  for (int g = 0; g < ngroups; ++g) {
+    const int t0 = g * gsz, t1 = std::min(T, t0 + gsz);
+    const bool last = g + 1 == ngroups;
+    DRPT_CHECK(launch_leaf_gather_val(ctx, G.perm + (int64_t)t0 * n, (int64_t)(t1 - t0) * n, kdn, K,
+                                      K - 1, boundT + (int64_t)t0 * n));
+    if (g > 0) DRPT_CHECK(fill_u32(ctx, (uint32_t*)pcnt, 0u, n));
+    DRPT_HIP(hipEventRecord(ctx->ev_pass[2 * g], s));
+    a.range = granges + 2 * g;
+    a.known = kin;
+    DRPT_CHECK(launch_lowp_knn(ctx, &pp, kModePool, a));
+    DRPT_HIP(hipEventRecord(ctx->ev_pass[2 * g + 1], s));
+    RowMergeArgs f{};
+    f.X = X;
+    f.n = n;
+    f.d = p->d;
+    f.K = K;
+    f.src = pool;
+    f.src_tree_stride = 0;
+    f.S = pool_cap;
+    f.ntsrc = 1;
+    f.cnt_tree = pcnt;
+    f.known_idx = kin;
+    f.known_dist = kdn;
+    f.skip_flag = oflag;
+    f.out_idx = last ? out_idx : (kin == ki ? ki2 : ki);
+    f.out_dist = last ? out_dist : (kdn == kd ? kd2 : kd);
+    f.order = G.perm;
+    DRPT_CHECK(launch_row_merge(ctx, p->dtype, f));
+    kin = f.out_idx;
+    kdn = f.out_dist;
+  }
   int32_t nov = 0;
   DRPT_HIP(hipMemcpyAsync(&nov, novf, 4, hipMemcpyDeviceToHost, s));
   DRPT_HIP(hipStreamSynchronize(s));
@@ -657,7 +719,9 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   if (timings) {
     timings->group_ms = ms_between(ctx->ev[6], ctx->ev[7]);
     timings->knn_ms = ms_between(ctx->ev[7], ctx->ev[5]);
-    timings->knn_kernel_ms = ms_between(ctx->ev[9], ctx->ev[8]);
+    timings->knn_kernel_ms = 0.0;
+    for (int g = 0; g < ngroups; ++g)
+      timings->knn_kernel_ms += ms_between(ctx->ev_pass[2 * g], ctx->ev_pass[2 * g + 1]);
     timings->n_fallback_rows = nov;
   }
   return DRPT_OK;

@@ -181,6 +181,14 @@ __global__ void k_tree_bases(const int64_t* __restrict__ tree_items, int T, int 
   range[5] = acc;
 }
 
+__global__ void k_tree_range(const int64_t* __restrict__ base, int t0, int t1,
+                             int64_t* __restrict__ range) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    range[0] = base[t0];
+    range[1] = base[t1];
+  }
+}
+
 // Per tree (one block): the work items (blocks of rpi rows) of every non-empty slot.
 __global__ void __launch_bounds__(1024)
 k_emit_items(const uint32_t* __restrict__ hcount, const int64_t* __restrict__ hoff, int HS, int rpi,
@@ -282,6 +290,16 @@ int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, int
   DRPT_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_emit_items, dim3(T), dim3(1024), 0, ctx->stream, hcount, hoff, HS,
                      rows_per_item, base, items);
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
+}
+
+int launch_tree_range(drpt_ctx* ctx, const int64_t* base, int t0, int t1, int64_t* range) {
+  if (t0 < 0 || t1 < t0) {
+    set_error("tree_range: invalid tree interval");
+    return DRPT_ERR_ARG;
+  }
+  hipLaunchKernelGGL(k_tree_range, dim3(1), dim3(64), 0, ctx->stream, base, t0, t1, range);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -39,6 +40,15 @@ struct drpt_ctx {
   hipStream_t stream = nullptr;
   std::map<std::string, std::pair<void*, size_t>> bufs;
   hipEvent_t ev[12] = {};
+  std::vector<hipEvent_t> ev_pass;  // per-pass timing events (grown on demand)
+  // captured launch sequences (hipGraph), replayed while their key (sizes,
+  // parameters and every buffer address they touch) is unchanged
+  struct Graph {
+    std::string key;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::map<std::string, Graph> graphs;
+  bool use_graphs = true;  // environment DRPT_GRAPHS=0: launch eagerly
 };
 
 namespace drpt {
@@ -62,6 +72,12 @@ int host_local_depth(int64_t gmin, int nn, int offset);
 // ---- launchers (one per .hip unit) ------------------------------------------
 // util.hip: capture-safe fills (a kernel, not hipMemsetAsync)
 int fill_u32(drpt_ctx* ctx, uint32_t* p, uint32_t v, size_t count);
+// Issues record() on the context's stream: eagerly when graphs are off, else
+// captured into a hipGraph the first time `key` is seen under `name` and
+// replayed from then on. record() may only launch kernels on ctx->stream (no
+// allocation, no synchronisation, no host reads of device data).
+int run_graph(drpt_ctx* ctx, const char* name, const std::string& key,
+              const std::function<int()>& record);
 
 // project.hip
 int launch_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, const int32_t* colptr,
@@ -91,28 +107,36 @@ int launch_scatter_leaves(drpt_ctx* ctx, const uint32_t* heap, int64_t n, int T,
 int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, int rows_per_item,
                         int first_trees, int64_t* hoff, int64_t* tree_items, int64_t* base,
                         int64_t* range, LeafItem* items, unsigned long long* stats);
+// range[0] = base[t0], range[1] = base[t1]: the work items of trees [t0, t1)
+int launch_tree_range(drpt_ctx* ctx, const int64_t* base, int t0, int t1, int64_t* range);
 int launch_sort_segments(drpt_ctx* ctx, int32_t* perm, int64_t n, const int64_t* seg_start,
                          const int32_t* seg_size, const int32_t* seg_tree, int nseg, int max_seg);
 int launch_apply_local(drpt_ctx* ctx, const int32_t* perm_in, int32_t* perm_out, int64_t n,
                        const int64_t* seg_start, const int32_t* seg_tree, const int64_t* seg_tbl,
                        const int32_t* seg_size, int nseg, const int32_t* tables);
 
+// quantised rows (bf16: n x d halves; fp8: n x d bytes with a power-of-two
+// scale 2^pexp[i] per row), |x~|^2 and |x~ - x| of the dequantised rows
 int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, int precision,
-                     void* Xq, float* nrm2, float* err, float* scale);
-// leaf-ordered |x~|^2 and |x~ - x|: lnbT[i] = nrm2[perm[i]], lerrT[i] = err[perm[i]]
+                     void* Xq, float* nrm2, float* err, int8_t* pexp);
+// leaf-ordered |x~|^2, |x~ - x| and (fp8) scale exponents: lnbT[i] =
+// nrm2[perm[i]], lerrT[i] = err[perm[i]], lexpT[i] = pexp[perm[i]] (pexp may be null)
 int launch_leaf_gather(drpt_ctx* ctx, const int32_t* perm, int64_t total, const float* nrm2,
-                       const float* err, float* lnbT, float* lerrT);
+                       const float* err, const int8_t* pexp, float* lnbT, float* lerrT,
+                       int8_t* lexpT);
 // FP32 exact per-tree lists for items[range[0] .. range[1]) (range in device memory)
 int launch_leaf_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
                     const LeafItem* items, const int64_t* range, int32_t* cand_idx,
                     float* cand_dist);
-// ---- low-precision (BF16) leaf k-NN, leafknn.hip -----------------------------
+// ---- low-precision (BF16 / FP8) leaf k-NN, leafknn.hip -----------------------
 // modes of launch_lowp_knn
 enum { kModeTree = 0, kModeCand = 1, kModePool = 2 };
 
 struct KnnArgs {
   const void* X;
-  const uint16_t* Xb;
+  const void* Xq;        // quantised rows (bf16 or fp8 e4m3), n x d
+  int precision;         // DRPT_BF16 | DRPT_FP8
+  const int8_t* lexpT;   // fp8: leaf-ordered scale exponents, [T][n]
   const float* lnbT;     // leaf-ordered |x~|^2, [T][n]
   const float* lerrT;    // leaf-ordered |x~ - x|, [T][n]
   const int32_t* perm;   // [T][n] points in leaf order

@@ -360,21 +360,54 @@ k_knn_exact(const XT* __restrict__ X, int d, const int32_t* __restrict__ perm, i
 // the K best of the leafmates that can still enter the merged top K, which
 // leaves the merged result unchanged.
 // ---------------------------------------------------------------------------
-constexpr int kGramChunk = 128;             // dims per staged Gram chunk
-constexpr int kTilePitch = kGramChunk + 8;  // bf16 elements per LDS row (16-byte aligned rows)
+constexpr int kChunkBytes = 256;                  // bytes of a row per staged Gram chunk
+constexpr int kTilePitchB = kChunkBytes + 16;     // LDS row pitch in bytes (16-byte aligned rows)
 constexpr int kLeafStage = 2048;            // leaves above this go to the exact fallback
 constexpr int kCap = 64;                    // candidate slots per query row
 constexpr uint16_t kNoPos = 0xFFFF;
 
-// 8 consecutive bf16 of row id from dim k (zeros past d, or for id < 0)
-__device__ __forceinline__ uint4 load_piece(const uint16_t* __restrict__ Xb, int id, int d, int k) {
+// Element types of the candidate Gram: bf16 (v_mfma_f32_32x32x16_bf16, 16 B
+// operand per lane per k-step) and fp8 e4m3 with a power-of-two scale per row
+// (v_mfma_f32_32x32x16_fp8_fp8, 8 B operand). Lane l supplies row l % 32 and
+// elements 16 ks + 8 (l / 32) .. +7 of k-step ks.
+struct ElBf16 {
+  static constexpr int kBytes = 2;
+  static constexpr int kOpBytes = 16;
+  typedef uint4 op_t;
+  __device__ static op_t lds(const unsigned char* p) { return *reinterpret_cast<const uint4*>(p); }
+  __device__ static floatx16 mma(op_t a, op_t b, floatx16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+  }
+};
+struct ElFp8 {
+  static constexpr int kBytes = 1;
+  static constexpr int kOpBytes = 8;
+  typedef uint2 op_t;
+  __device__ static op_t lds(const unsigned char* p) { return *reinterpret_cast<const uint2*>(p); }
+  __device__ static floatx16 mma(op_t a, op_t b, floatx16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(__builtin_bit_cast(long, a),
+                                                      __builtin_bit_cast(long, b), c, 0, 0, 0);
+  }
+};
+
+// byte offset, within a row, of lane's operand for k-step ks
+template <typename EL>
+__device__ __forceinline__ int op_offset(int ks, int lane) {
+  return ks * 2 * EL::kOpBytes + (lane >> 5) * EL::kOpBytes;
+}
+
+// 16 bytes of row id (rb bytes per row) from byte o; zeros past the row or for id < 0
+__device__ __forceinline__ uint4 load16(const uint8_t* __restrict__ Xq, int id, int rb, int o) {
   uint4 v = make_uint4(0, 0, 0, 0);
-  if (id < 0 || k >= d) return v;
-  if ((d & 7) == 0) return *reinterpret_cast<const uint4*>(Xb + (int64_t)id * d + k);
-  uint16_t t[8];
+  if (id < 0 || o >= rb) return v;
+  const uint8_t* src = Xq + (int64_t)id * rb + o;
+  if ((rb & 15) == 0) return *reinterpret_cast<const uint4*>(src);
+  uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
-  for (int u = 0; u < 8; ++u) t[u] = (k + u < d) ? Xb[(int64_t)id * d + k + u] : (uint16_t)0;
-  return make_uint4(t[0] | (t[1] << 16), t[2] | (t[3] << 16), t[4] | (t[5] << 16), t[6] | (t[7] << 16));
+  for (int u = 0; u < 16; ++u)
+    if (o + u < rb) w[u >> 2] |= (uint32_t)src[u] << (8 * (u & 3));
+  return make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 __device__ __forceinline__ float4 load4f(const float* p) { return *reinterpret_cast<const float4*>(p); }
@@ -428,6 +461,7 @@ struct LowpShared;
 #define DRPT_LOWP_COMMON                                                              \
   int lid[kLeafStage];              /* point id of every leaf position */           \
   float lnb[kLeafStage];            /* |x~|^2 of every leaf position */             \
+  int8_t lexp[kLeafStage];          /* fp8: scale exponent of every leaf position */ \
   uint16_t cpos[kRows][kCap];       /* candidate leaf positions per query row */    \
   int ccnt[kRows * 4];              /* per row (tree) or per (row, quarter) (pool) */ \
   float cacut[kRows];                                                               \
@@ -436,9 +470,10 @@ struct LowpShared;
 
 template <>
 struct LowpShared<true> {
-  // leafmate tiles, bf16, double-buffered; for d > kGramChunk buffer 1 holds
-  // the query rows' chunk and buffer 0 the leafmates' (single-buffered)
-  uint16_t Bs[2][kCols][kTilePitch];
+  // leafmate tiles (row bytes), double-buffered; for rows longer than
+  // kChunkBytes buffer 1 holds the query rows' chunk and buffer 0 the
+  // leafmates' (single-buffered)
+  unsigned char Bs[2][kCols][kTilePitchB];
   DRPT_LOWP_COMMON
   float cdist[kRows][kCap];         // exact distances of the candidates
   int cpre[kRows + 1];
@@ -449,63 +484,67 @@ constexpr int kKnownLds = 16;  // pool mode: known ids per row staged in LDS (la
 template <>
 struct LowpShared<false> {
   union {
-    uint16_t Bs[2][kCols][kTilePitch];
+    unsigned char Bs[2][kCols][kTilePitchB];
     unsigned char ring[kRing][kCols * kRingRow];
   };
   DRPT_LOWP_COMMON
   int known[kRows * kKnownLds];   // pool mode: the block's known ids
+  int flush_flag;                 // pool mode: some list may fill up in the next tile
 };
 #undef DRPT_LOWP_COMMON
 
-// Query operand of the block (d <= kGramChunk): lane l of wave w holds, for
-// every k-step, the 8 bf16 of query row (w & 1) * 32 + l % 32 at dims
-// 16 ks + 8 (l / 32) .. +7, i.e. the MFMA's B operand, for all passes.
+// Query operand of the block (rows of at most kChunkBytes): lane l of wave w
+// holds, for every k-step, its operand of query row (w & 1) * 32 + l % 32,
+// i.e. the MFMA's B operand, for all passes (32 VGPRs for either element type).
+template <typename EL>
 struct QueryOp {
-  bf16x8 v[kGramChunk / 16];
+  typename EL::op_t v[kChunkBytes / (2 * EL::kOpBytes)];
 };
 
-template <typename SH>
-__device__ __forceinline__ void load_query(SH& S, const uint16_t* __restrict__ Xb, int d,
-                                           int row0, int rows, QueryOp& qo) {
+template <typename EL, typename SH>
+__device__ __forceinline__ void load_query(SH& S, const uint8_t* __restrict__ Xq, int rb,
+                                           int row0, int rows, QueryOp<EL>& qo) {
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  uint16_t(*As)[kTilePitch] = S.Bs[1];
+  unsigned char(*As)[kTilePitchB] = S.Bs[1];
 #pragma unroll
   for (int s4 = 0; s4 < 4; ++s4) {
     const int c = tid + kThreads * s4;
     const int r = c >> 4, part = c & 15;
-    *reinterpret_cast<uint4*>(&As[r][part * 8]) =
-        load_piece(Xb, r < rows ? S.lid[row0 + r] : -1, d, part * 8);
+    *reinterpret_cast<uint4*>(&As[r][part * 16]) =
+        load16(Xq, r < rows ? S.lid[row0 + r] : -1, rb, part * 16);
   }
   __syncthreads();
   const int qh = (wave & 1) * 32;
 #pragma unroll
-  for (int ks = 0; ks < kGramChunk / 16; ++ks)
-    qo.v[ks] = *reinterpret_cast<const bf16x8*>(&As[qh + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
+  for (int ks = 0; ks < kChunkBytes / (2 * EL::kOpBytes); ++ks)
+    qo.v[ks] = EL::lds(&As[qh + (lane & 31)][op_offset<EL>(ks, lane)]);
   // buffer 1 is next written after gram_pass's first barrier
 }
 
 // One pass of Gram tiles of the query block [row0, row0 + 64) against every
-// leafmate tile; epi(c0, acc) sees each thread's 16 accumulators. With the
-// query operand in registers (d <= kGramChunk) the leafmate tiles are
-// double-buffered: one barrier per tile, the next tile's loads in flight
-// during this tile's MFMAs and epilogue.
-template <typename SH, typename Epi>
-__device__ __forceinline__ void gram_pass(SH& S, const uint16_t* __restrict__ Xb, int d,
-                                          int m, int row0, int rows, const QueryOp& qo,
-                                          Epi&& epi) {
+// leafmate tile; epi(c0, acc) sees each thread's 16 accumulators; hook() runs
+// on every thread between tiles, after a barrier. With the
+// query operand in registers (rows of at most kChunkBytes) the leafmate tiles
+// are double-buffered: one barrier per tile, the next tile's loads in flight
+// during this tile's MFMAs and epilogue. rb = bytes per row, d = elements.
+template <typename EL, typename SH, typename Epi, typename Hook>
+__device__ __forceinline__ void gram_pass(SH& S, const uint8_t* __restrict__ Xq, int d, int rb,
+                                          int m, int row0, int rows, const QueryOp<EL>& qo,
+                                          Epi&& epi, Hook&& hook) {
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int qh = (wave & 1) * 32, ch = (wave >> 1) * 32;
-  if (d <= kGramChunk) {
+  constexpr int kStepBytes = 2 * EL::kOpBytes;  // bytes of a row per k-step
+  if (rb <= kChunkBytes) {
     const int ksteps = (d + 15) / 16;
-    uint4 rb[4];
+    uint4 rbv[4];
 #pragma unroll
     for (int s4 = 0; s4 < 4; ++s4) {
       const int c = tid + kThreads * s4;
       const int r = c >> 4, part = c & 15;
-      *reinterpret_cast<uint4*>(&S.Bs[0][r][part * 8]) =
-          load_piece(Xb, r < m ? S.lid[r] : -1, d, part * 8);
+      *reinterpret_cast<uint4*>(&S.Bs[0][r][part * 16]) =
+          load16(Xq, r < m ? S.lid[r] : -1, rb, part * 16);
     }
     __syncthreads();
     int b = 0;
@@ -517,18 +556,17 @@ __device__ __forceinline__ void gram_pass(SH& S, const uint16_t* __restrict__ Xb
           const int c = tid + kThreads * s4;
           const int r = c >> 4, part = c & 15;
           const int col = c0 + kCols + r;
-          rb[s4] = load_piece(Xb, col < m ? S.lid[col] : -1, d, part * 8);
+          rbv[s4] = load16(Xq, col < m ? S.lid[col] : -1, rb, part * 16);
         }
       }
       floatx16 acc;
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
 #pragma unroll
-      for (int ks = 0; ks < kGramChunk / 16; ++ks) {
+      for (int ks = 0; ks < kChunkBytes / kStepBytes; ++ks) {
         if (ks < ksteps) {
-          const bf16x8 a =
-              *reinterpret_cast<const bf16x8*>(&S.Bs[b][ch + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qo.v[ks], acc, 0, 0, 0);
+          const typename EL::op_t a = EL::lds(&S.Bs[b][ch + (lane & 31)][op_offset<EL>(ks, lane)]);
+          acc = EL::mma(a, qo.v[ks], acc);
         }
       }
       epi(c0, acc);
@@ -536,41 +574,47 @@ __device__ __forceinline__ void gram_pass(SH& S, const uint16_t* __restrict__ Xb
 #pragma unroll
         for (int s4 = 0; s4 < 4; ++s4) {
           const int c = tid + kThreads * s4;
-          *reinterpret_cast<uint4*>(&S.Bs[b ^ 1][c >> 4][(c & 15) * 8]) = rb[s4];
+          *reinterpret_cast<uint4*>(&S.Bs[b ^ 1][c >> 4][(c & 15) * 16]) = rbv[s4];
         }
       }
       __syncthreads();
+      if (more) hook();
     }
     return;
   }
-  // d > kGramChunk: chunks of dims, query chunk in buffer 1, leafmates in buffer 0
-  uint16_t(*As)[kTilePitch] = S.Bs[1];
-  uint16_t(*Bs)[kTilePitch] = S.Bs[0];
+  // longer rows: chunks of kChunkBytes, query chunk in buffer 1, leafmates in buffer 0
+  unsigned char(*As)[kTilePitchB] = S.Bs[1];
+  unsigned char(*Bs)[kTilePitchB] = S.Bs[0];
   for (int c0 = 0; c0 < m; c0 += kCols) {
     floatx16 acc;
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
-    for (int k0 = 0; k0 < d; k0 += kGramChunk) {
+    for (int o0 = 0; o0 < rb; o0 += kChunkBytes) {
       __syncthreads();
 #pragma unroll
       for (int s4 = 0; s4 < 4; ++s4) {
         const int c = tid + kThreads * s4;
         const int r = c >> 4, part = c & 15;
         const int col = c0 + r;
-        const uint4 va = load_piece(Xb, r < rows ? S.lid[row0 + r] : -1, d, k0 + part * 8);
-        const uint4 vb = load_piece(Xb, col < m ? S.lid[col] : -1, d, k0 + part * 8);
-        *reinterpret_cast<uint4*>(&As[r][part * 8]) = va;
-        *reinterpret_cast<uint4*>(&Bs[r][part * 8]) = vb;
+        const uint4 va = load16(Xq, r < rows ? S.lid[row0 + r] : -1, rb, o0 + part * 16);
+        const uint4 vb = load16(Xq, col < m ? S.lid[col] : -1, rb, o0 + part * 16);
+        *reinterpret_cast<uint4*>(&As[r][part * 16]) = va;
+        *reinterpret_cast<uint4*>(&Bs[r][part * 16]) = vb;
       }
       __syncthreads();
-      const int ksteps = min(kGramChunk, d - k0 + 15) / 16;
+      // k-steps covering the chunk's elements (zeros past the row end)
+      const int ksteps = (min(kChunkBytes, rb - o0) + kStepBytes - 1) / kStepBytes;
       for (int ks = 0; ks < ksteps; ++ks) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&Bs[ch + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
-        const bf16x8 bq = *reinterpret_cast<const bf16x8*>(&As[qh + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq, acc, 0, 0, 0);
+        const typename EL::op_t a = EL::lds(&Bs[ch + (lane & 31)][op_offset<EL>(ks, lane)]);
+        const typename EL::op_t bq = EL::lds(&As[qh + (lane & 31)][op_offset<EL>(ks, lane)]);
+        acc = EL::mma(a, bq, acc);
       }
     }
     epi(c0, acc);
+    if (c0 + kCols < m) {
+      __syncthreads();
+      hook();
+    }
   }
   __syncthreads();
 }
@@ -609,20 +653,22 @@ __device__ __forceinline__ void ring_barrier() {
 // of an instruction writes 16 l bytes after its wave-uniform base), kRing
 // stages, two tiles in flight during each tile's MFMAs. Stage layout: row r
 // of the tile at r * 256 bytes, its 16-byte chunk c at physical chunk
-// c ^ (r & 15) (bank spread for the MFMA operand reads); chunks at or past
-// d / 8 stay zero. Every wave issues exactly 4 load instructions per tile
+// c ^ (r & 15) (bank spread for the MFMA operand reads); chunks past the
+// row's bytes stay zero. Every wave issues exactly 4 load instructions per tile
 // (each covers 4 whole rows, at least one valid chunk per row), so
 // vmcnt <= 4 means "all but the newest tile".
-template <typename Epi>
+template <typename EL, typename Epi, typename Hook>
 __device__ __forceinline__ void gram_pass_ring(LowpShared<false>& S,
-                                               const uint16_t* __restrict__ Xb, int d, int m,
-                                               const QueryOp& qo, Epi&& epi) {
+                                               const uint8_t* __restrict__ Xq, int d, int rb,
+                                               int m, const QueryOp<EL>& qo, Epi&& epi,
+                                               Hook&& hook) {
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int ch = (wave >> 1) * 32;
-  const int L = d >> 3;            // 16-byte chunks per row
+  const int L = rb >> 4;           // 16-byte chunks per row
   const int ksteps = (d + 15) / 16;
   const int ntiles = (m + kCols - 1) / kCols;
+  constexpr int kSteps = kChunkBytes / (2 * EL::kOpBytes);
   __syncthreads();  // query staging reads done (they share the ring's bytes)
   if (L < 16) {     // zero the chunks no load writes
     for (int j = tid; j < kRing * kCols * 16; j += kThreads) {
@@ -641,7 +687,7 @@ __device__ __forceinline__ void gram_pass_ring(LowpShared<false>& S,
       const int col = min(j * kCols + r, m - 1);  // rows past the leaf: any valid row
       const uint32_t dst = __builtin_amdgcn_readfirstlane(
           (uint32_t)(uintptr_t)(lds_ptr_t)(base + blk * 64 * 16));
-      if (c < L) lds_dma16(Xb + (int64_t)S.lid[col] * d + c * 8, dst);
+      if (c < L) lds_dma16(Xq + (int64_t)S.lid[col] * rb + c * 16, dst);
     }
   };
   issue(0);
@@ -650,22 +696,25 @@ __device__ __forceinline__ void gram_pass_ring(LowpShared<false>& S,
     if (t + 1 < ntiles) wait_vmcnt<4>(); else wait_vmcnt<0>();
     ring_barrier();  // tile t visible; every wave is done with tile t - 1's stage
     if (t + 2 < ntiles) issue(t + 2);
+    if (t > 0) hook();  // touches neither the ring nor the operand registers
     const unsigned char* st = S.ring[t % kRing];
     floatx16 acc;
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
     const int r = ch + (lane & 31);
 #pragma unroll
-    for (int k4 = 0; k4 < kGramChunk / 16; k4 += 4) {  // 4 operands read, then 4 MFMAs
-      bf16x8 a[4];
+    for (int k4 = 0; k4 < kSteps; k4 += 4) {  // 4 operands read, then 4 MFMAs
+      if (k4 >= ksteps) break;
+      typename EL::op_t a[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int pc = (2 * (k4 + u) + (lane >> 5)) ^ (r & 15);
-        a[u] = *reinterpret_cast<const bf16x8*>(st + r * kRingRow + pc * 16);
+        const int o = op_offset<EL>(k4 + u, lane);
+        const int pc = (o >> 4) ^ (r & 15);
+        a[u] = EL::lds(st + r * kRingRow + pc * 16 + (o & 15));
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        if (k4 + u < ksteps) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u], qo.v[k4 + u], acc, 0, 0, 0);
+        if (k4 + u < ksteps) acc = EL::mma(a[u], qo.v[k4 + u], acc);
     }
     epi(t * kCols, acc);
   }
@@ -719,17 +768,19 @@ constexpr int kSubCap = 16;  // pool mode: list slots per thread (4 threads per 
 //   kModePool  filter pass against an external bound B (leaf-ordered), skipping
 //              the row's known ids; survivors are appended to the row's pool
 
-template <typename XT, int T, int MODE, bool RING>
+template <typename XT, typename EL, int T, int MODE, bool RING>
 __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
   constexpr int KP = 4 * T;  // widened candidates per row in the bound pass
+  constexpr bool kScaled = EL::kBytes == 1;  // fp8: per-row power-of-two scales
   __shared__ __align__(16) LowpShared<MODE == kModeTree> S;
   const XT* __restrict__ X = static_cast<const XT*>(A.X);
-  const uint16_t* __restrict__ Xb = A.Xb;
+  const uint8_t* __restrict__ Xq = static_cast<const uint8_t*>(A.Xq);
   const int d = A.d, K = A.K;
+  const int rb = d * EL::kBytes;  // bytes per quantised row
   const int64_t n = A.n;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const bool a_resident = d <= kGramChunk;
+  const bool a_resident = rb <= kChunkBytes;
   // this thread's query row within the block and quarter of the columns
   const int q = (wave & 1) * 32 + (lane & 31);
   const int sub = (wave >> 1) * 2 + (lane >> 5);
@@ -740,6 +791,9 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
   const int64_t end = min(A.range[1], begin + chunk);
   unsigned* myq = A.queue + qx;
   if (tid == 0) S.ticket[0] = (int)atomicAdd(myq, 1u);
+  if constexpr (MODE == kModePool) {
+    if (tid == 0) S.flush_flag = 0;
+  }
   __syncthreads();
   int64_t item = begin + S.ticket[0];
   for (int parity = 1;; parity ^= 1) {
@@ -779,6 +833,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
       const float v = A.lnbT[lofs + j];
       S.lid[j] = pl[j];
       S.lnb[j] = v;
+      if constexpr (kScaled) S.lexp[j] = A.lexpT[lofs + j];
       mn = fmaxf(mn, v);
       me_ = fmaxf(me_, A.lerrT[lofs + j]);
     }
@@ -805,14 +860,32 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
       const bool qv = q < rows;
       const int qpos = row0 + q;
       const float rn = qv ? S.lnb[qpos] : 0.0f;
-      QueryOp qo;
-      if (a_resident) load_query(S, Xb, d, row0, rows, qo);
-      // the ring path is a separate instantiation (RING: d % 8 == 0, d <= 128,
-      // candidate / pool modes), so no register-staged tile load is pending
-      // in its loop
-      auto gram = [&](auto&& epi) {
-        if constexpr (RING) gram_pass_ring(S, Xb, d, m, qo, epi);
-        else gram_pass(S, Xb, d, m, row0, rows, qo, epi);
+      QueryOp<EL> qo;
+      if (a_resident) load_query<EL>(S, Xq, rb, row0, rows, qo);
+      // fp8: the tile's Gram values scaled back by 2^(e_query + e_leafmate)
+      // (exact: powers of two), so the epilogues see x~_i . x~_j as for bf16
+      const int eq = kScaled && qv ? (int)S.lexp[qpos] : 0;
+      auto scaled = [&](auto& epi) {
+        return [&](int c0, floatx16 acc) {
+          if constexpr (kScaled) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+              const uint32_t w = *reinterpret_cast<const uint32_t*>(&S.lexp[c0 + cq + 8 * g]);
+#pragma unroll
+              for (int u = 0; u < 4; ++u)
+                acc[4 * g + u] = ldexpf(acc[4 * g + u], eq + (int)(int8_t)(w >> (8 * u)));
+            }
+          }
+          epi(c0, acc);
+        };
+      };
+      // the ring path is a separate instantiation (RING: rows of 16-byte
+      // multiples up to 256 bytes, candidate / pool modes), so no
+      // register-staged tile load is pending in its loop
+      auto gram = [&](auto&& epi, auto&& hook) {
+        auto e2 = scaled(epi);
+        if constexpr (RING) gram_pass_ring<EL>(S, Xq, d, rb, m, qo, e2, hook);
+        else gram_pass<EL>(S, Xq, d, rb, m, row0, rows, qo, e2, hook);
       };
       uint32_t key[T];
 #pragma unroll
@@ -842,7 +915,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
               }
             }
           }
-        });
+        }, [] {});
       }
       if constexpr (MODE == kModeCand) {
         if (qv) {
@@ -919,6 +992,68 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
       // filter pass: every leafmate that can enter the row's top K
       const float acut = qv ? S.cacut[q] : -INFINITY;
       int nsub = 0;  // kModePool: entries of this thread's list
+      // kModePool flush: thread (row r = tid / 4, quarter tid % 4) drops the
+      // row's known ids from the list of the thread that owned (r, quarter) in
+      // the tiles, the 4 lanes of a row reserve pool slots with one atomic.
+      // Runs at the end of the pass, and between tiles once some list could
+      // overflow during the next tile (flag set in the epilogue, read after
+      // a barrier, so every thread takes the same branch).
+      auto flush = [&]() {
+        if constexpr (MODE == kModePool) {
+          S.ccnt[q * 4 + sub] = nsub;  // the owner's count, visible after the barrier
+          __syncthreads();
+          const int r = tid >> 2, qs = tid & 3;
+          const int* kn = &S.known[r * kKnownLds];
+          const int cnt = r < rows ? S.ccnt[r * 4 + qs] : 0;
+          uint16_t* lst = &S.cpos[r][qs * kSubCap];
+          int keep = 0;
+          if (cnt > kSubCap) {
+            keep = A.pool_cap + 1;  // forces the row's overflow
+          } else {
+            if (K <= kKnownLds) {
+              for (int j = 0; j < cnt; ++j) {
+                const uint16_t c = lst[j];
+                const int id = S.lid[c];
+                bool known = false;
+                for (int k = 0; k < K; ++k) known |= kn[k] == id;
+                if (!known) lst[keep++] = c;
+              }
+            } else {
+              keep = cnt;  // larger K: the row merge drops known ids (hash)
+            }
+          }
+          int pre = keep;  // inclusive prefix over the row's 4 lanes
+#pragma unroll
+          for (int off = 1; off < 4; off <<= 1) {
+            const int v = __shfl_up(pre, off, 4);
+            if (qs >= off) pre += v;
+          }
+          const int total = __shfl(pre, 3, 4);
+          int base = 0;
+          if (qs == 0 && total > 0 && total <= A.pool_cap) base = atomicAdd(&A.pool_cnt[S.lid[row0 + r]], total);
+          base = __shfl(base, 0, 4);
+          if (total > 0) {
+            const int id = S.lid[row0 + r];
+            if (total <= A.pool_cap && base + total <= A.pool_cap) {
+              const int o = base + pre - keep;
+              for (int j = 0; j < keep; ++j)
+                A.out_idx[(int64_t)id * A.pool_cap + o + j] = S.lid[lst[j]];
+            } else if (qs == 0 && atomicExch(&A.ovf_flag[id], 1) == 0) {
+              A.ovf_list[atomicAdd(A.novf, 1)] = id;
+            }
+          }
+          nsub = 0;
+        }
+      };
+      auto hook = [&]() {
+        if constexpr (MODE == kModePool) {
+          if (S.flush_flag) {
+            flush();
+            if (tid == 0) S.flush_flag = 0;  // no epilogue runs before the barrier
+            __syncthreads();
+          }
+        }
+      };
       gram([&](int c0, const floatx16& acc) {
         uint32_t pm = 0;  // elements with a <= A_cut
 #pragma unroll
@@ -950,50 +1085,13 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
             ++nsub;
           }
         }
-      });
+        if constexpr (MODE == kModePool) {
+          // a tile adds at most 16 entries per thread
+          if (nsub > kSubCap - 16) S.flush_flag = 1;
+        }
+      }, hook);
       if constexpr (MODE == kModePool) {
-        // flush: thread (row r = tid / 4, quarter tid % 4) drops the row's known
-        // ids from the list of the thread that owned (r, quarter) in the tiles,
-        // the 4 lanes of a row reserve pool slots with one atomic
-        S.ccnt[q * 4 + sub] = nsub;  // the owner's count, visible after the barrier
-        __syncthreads();
-        const int r = tid >> 2, qs = tid & 3;
-        const int* kn = K <= kKnownLds ? &S.known[r * kKnownLds]
-                                       : A.known + (int64_t)S.lid[row0 + min(r, rows - 1)] * K;
-        const int cnt = r < rows ? S.ccnt[r * 4 + qs] : 0;
-        uint16_t* lst = &S.cpos[r][qs * kSubCap];
-        int keep = 0;
-        if (cnt > kSubCap) {
-          keep = A.pool_cap + 1;  // forces the row's overflow
-        } else {
-          for (int j = 0; j < cnt; ++j) {
-            const uint16_t c = lst[j];
-            const int id = S.lid[c];
-            bool known = false;
-            for (int k = 0; k < K; ++k) known |= kn[k] == id;
-            if (!known) lst[keep++] = c;
-          }
-        }
-        int pre = keep;  // inclusive prefix over the row's 4 lanes
-#pragma unroll
-        for (int off = 1; off < 4; off <<= 1) {
-          const int v = __shfl_up(pre, off, 4);
-          if (qs >= off) pre += v;
-        }
-        const int total = __shfl(pre, 3, 4);
-        int base = 0;
-        if (qs == 0 && total > 0 && total <= A.pool_cap) base = atomicAdd(&A.pool_cnt[S.lid[row0 + r]], total);
-        base = __shfl(base, 0, 4);
-        if (total > 0) {
-          const int id = S.lid[row0 + r];
-          if (total <= A.pool_cap && base + total <= A.pool_cap) {
-            const int o = base + pre - keep;
-            for (int j = 0; j < keep; ++j)
-              A.out_idx[(int64_t)id * A.pool_cap + o + j] = S.lid[lst[j]];
-          } else if (qs == 0 && atomicExch(&A.ovf_flag[id], 1) == 0) {
-            A.ovf_list[atomicAdd(A.novf, 1)] = id;
-          }
-        }
+        flush();
         __syncthreads();  // lists and tiles reused by the next block
         continue;
       }
@@ -1072,12 +1170,14 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
 // Leaf-ordered copies of |x~|^2 and |x~ - x| (position j of tree t's leaf order).
 __global__ void k_leaf_gather(const int32_t* __restrict__ perm, int64_t total,
                               const float* __restrict__ nb, const float* __restrict__ err,
-                              float* __restrict__ lnbT, float* __restrict__ lerrT) {
+                              const int8_t* __restrict__ pexp, float* __restrict__ lnbT,
+                              float* __restrict__ lerrT, int8_t* __restrict__ lexpT) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int id = perm[i];
     lnbT[i] = nb[id];
     lerrT[i] = err[id];
+    if (pexp) lexpT[i] = pexp[id];
   }
 }
 
@@ -1167,11 +1267,13 @@ __device__ __forceinline__ float exact_dist8(const XT* __restrict__ xr, const XT
 }
 
 constexpr int kMergeWaves = 4;    // rows in flight per block (one per wave)
-constexpr int kHashBits = 9;      // per-wave open-addressing table for distinct ids:
-constexpr int kHashSlots = 1 << kHashBits;  // more slots than K + 256 insertions, so probing ends
 
 template <typename XT, int E>
 __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeArgs A) {
+  // per-wave open-addressing table for distinct ids: more slots than the K
+  // known ids + 64 E insertions, so probing ends
+  constexpr int kHashBits = E <= 4 ? 9 : 10;
+  constexpr int kHashSlots = 1 << kHashBits;
   __shared__ uint32_t tab[kMergeWaves][kHashSlots];
   __shared__ uint64_t dense[kMergeWaves][64 * E];
   const XT* __restrict__ X = static_cast<const XT*>(A.X);
@@ -1190,7 +1292,15 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
   for (int64_t i = lo + (int64_t)(blockIdx.x >> 3) * kMergeWaves + w; i < hi;
        i += nblk * kMergeWaves) {
     const int64_t row = A.order != nullptr ? (int64_t)A.order[i] : i;
-    if (A.skip_flag != nullptr && A.skip_flag[row] != 0) continue;
+    if (A.skip_flag != nullptr && A.skip_flag[row] != 0) {
+      // left to the exact row path; the known list passes through (a valid
+      // bound for later passes)
+      if (A.known_idx != nullptr && lane < K) {
+        A.out_idx[row * K + lane] = A.known_idx[row * K + lane];
+        A.out_dist[row * K + lane] = A.known_dist[row * K + lane];
+      }
+      continue;
+    }
     // known list (sorted, -1 padded): lanes < K
     int kid = -1;
     float kd = INFINITY;
@@ -1472,15 +1582,18 @@ k_knn_fallback(const XT* __restrict__ X, int d, const int32_t* __restrict__ perm
   }
 }
 
-// bf16 copy (round to nearest even), |x~|^2 (FP32) and |x~ - x| (rounded up).
+// bf16 copy (round to nearest even), |x~|^2 and |x~ - x| (rounded up). The
+// squared norm is reduced in FP32, or in FP64 (rounded once) for d >= 512,
+// where d FP32 additions would loosen the filter bound.
 template <typename XT>
 __global__ void k_prep_bf16(const XT* __restrict__ X, int64_t n, int d, uint16_t* __restrict__ Xb,
                             float* __restrict__ nb, float* __restrict__ err) {
   const int lane = threadIdx.x & 63;
   const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (row >= n) return;
+  const bool wide = d >= 512;
   float s = 0.0f;
-  double e2 = 0.0;
+  double s64 = 0.0, e2 = 0.0;
   for (int k = lane; k < d; k += 64) {
     const float x = (float)X[row * d + k];
     uint32_t bits = __float_as_uint(x);
@@ -1488,20 +1601,87 @@ __global__ void k_prep_bf16(const XT* __restrict__ X, int64_t n, int d, uint16_t
     const uint16_t b = (uint16_t)(bits >> 16);
     Xb[row * d + k] = b;
     const float xb = __uint_as_float((uint32_t)b << 16);
-    s = fmaf(xb, xb, s);
+    if (wide) s64 += (double)xb * (double)xb;
+    else s = fmaf(xb, xb, s);
     const double df = (double)xb - (double)x;
     e2 += df * df;
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     s += __shfl_xor(s, off);
+    s64 += __shfl_xor(s64, off);
     e2 += __shfl_xor(e2, off);
   }
   if (lane == 0) {
-    nb[row] = s;
+    nb[row] = wide ? (float)s64 : s;
     float ef = (float)sqrt(e2);
     if (e2 > 0.0) ef = ef * (1.0f + 1.0e-6f) + 1.0e-30f;
     err[row] = ef;
+  }
+}
+
+// FP8 quantisation, one wave per row: scale 2^e with e the smallest exponent
+// for which max |x| / 2^e <= 448 (the e4m3 maximum, so no element saturates),
+// clamped to [-120, 120]; q = e4m3(x / 2^e) rounded to nearest by the
+// hardware conversion (the format the fp8 MFMA reads), x~ = 2^e q. The row's
+// |x~|^2 and |x~ - x| (FP64, rounded up) are reduced as for bf16.
+template <typename XT>
+__global__ void k_prep_fp8(const XT* __restrict__ X, int64_t n, int d, uint8_t* __restrict__ Xq,
+                           float* __restrict__ nb, float* __restrict__ err,
+                           int8_t* __restrict__ pexp) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (row >= n) return;
+  const XT* __restrict__ xr = X + row * d;
+  float mx = 0.0f;
+  for (int k = lane; k < d; k += 64) mx = fmaxf(mx, fabsf((float)xr[k]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+  int e = 0;
+  if (mx > 0.0f) {
+    e = ilogbf(mx) - 8;  // mx / 2^e in [256, 512)
+    if (ldexpf(mx, -e) > 448.0f) ++e;
+    e = max(-120, min(120, e));
+  }
+  const bool wide = d >= 512;
+  float s = 0.0f;
+  double s64 = 0.0, e2 = 0.0;
+  for (int k0 = 4 * lane; k0 < d; k0 += 256) {
+    float x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) x[u] = k0 + u < d ? (float)xr[k0 + u] : 0.0f;
+    int w = __builtin_amdgcn_cvt_pk_fp8_f32(ldexpf(x[0], -e), ldexpf(x[1], -e), 0, false);
+    w = __builtin_amdgcn_cvt_pk_fp8_f32(ldexpf(x[2], -e), ldexpf(x[3], -e), w, true);
+    const float dq[4] = {__builtin_amdgcn_cvt_f32_fp8(w, 0), __builtin_amdgcn_cvt_f32_fp8(w, 1),
+                         __builtin_amdgcn_cvt_f32_fp8(w, 2), __builtin_amdgcn_cvt_f32_fp8(w, 3)};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (k0 + u >= d) break;
+      const float xb = ldexpf(dq[u], e);
+      if (wide) s64 += (double)xb * (double)xb;
+      else s = fmaf(xb, xb, s);
+      const double df = (double)xb - (double)x[u];
+      e2 += df * df;
+    }
+    uint8_t* dst = Xq + row * d + k0;
+    if ((d & 3) == 0) {
+      *reinterpret_cast<uint32_t*>(dst) = (uint32_t)w;
+    } else {
+      for (int u = 0; u < 4 && k0 + u < d; ++u) dst[u] = (uint8_t)((uint32_t)w >> (8 * u));
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s += __shfl_xor(s, off);
+    s64 += __shfl_xor(s64, off);
+    e2 += __shfl_xor(e2, off);
+  }
+  if (lane == 0) {
+    nb[row] = wide ? (float)s64 : s;
+    float ef = (float)sqrt(e2);
+    if (e2 > 0.0) ef = ef * (1.0f + 1.0e-6f) + 1.0e-30f;
+    err[row] = ef;
+    pexp[row] = (int8_t)e;
   }
 }
 
@@ -1514,38 +1694,46 @@ int persistent_grid(drpt_ctx* ctx, T kernel, int threads, unsigned* grid) {
   return DRPT_OK;
 }
 
-template <typename XT, int T, int MODE, bool RING>
+template <typename XT, typename EL, int T, int MODE, bool RING>
 int launch_lowp_r(drpt_ctx* ctx, const KnnArgs& a) {
   unsigned grid;
-  DRPT_CHECK(persistent_grid(ctx, k_knn_lowp<XT, T, MODE, RING>, kThreads, &grid));
-  hipLaunchKernelGGL((k_knn_lowp<XT, T, MODE, RING>), dim3(grid), dim3(kThreads), 0, ctx->stream, a);
+  DRPT_CHECK(persistent_grid(ctx, k_knn_lowp<XT, EL, T, MODE, RING>, kThreads, &grid));
+  hipLaunchKernelGGL((k_knn_lowp<XT, EL, T, MODE, RING>), dim3(grid), dim3(kThreads), 0,
+                     ctx->stream, a);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }
 
-template <typename XT, int T, int MODE>
+template <typename XT, typename EL, int T, int MODE>
 int launch_lowp_t(drpt_ctx* ctx, const KnnArgs& a) {
   if constexpr (MODE != kModeTree) {
-    if ((a.d & 7) == 0 && a.d <= kGramChunk) return launch_lowp_r<XT, T, MODE, true>(ctx, a);
+    const int rb = a.d * EL::kBytes;
+    if ((rb & 15) == 0 && rb <= kRingRow) return launch_lowp_r<XT, EL, T, MODE, true>(ctx, a);
   }
-  return launch_lowp_r<XT, T, MODE, false>(ctx, a);
+  return launch_lowp_r<XT, EL, T, MODE, false>(ctx, a);
 }
 
-template <typename XT, int T>
+template <typename XT, typename EL, int T>
 int launch_lowp_m(drpt_ctx* ctx, int mode, const KnnArgs& a) {
-  if (mode == kModeTree) return launch_lowp_t<XT, T, kModeTree>(ctx, a);
-  if (mode == kModeCand) return launch_lowp_t<XT, T, kModeCand>(ctx, a);
-  return launch_lowp_t<XT, T, kModePool>(ctx, a);
+  if (mode == kModeTree) return launch_lowp_t<XT, EL, T, kModeTree>(ctx, a);
+  if (mode == kModeCand) return launch_lowp_t<XT, EL, T, kModeCand>(ctx, a);
+  return launch_lowp_t<XT, EL, T, kModePool>(ctx, a);
+}
+
+template <typename XT, typename EL>
+int launch_lowp_x(drpt_ctx* ctx, int t, int mode, const KnnArgs& a) {
+  if (t <= 3) return launch_lowp_m<XT, EL, 3>(ctx, mode, a);
+  if (t <= 4) return launch_lowp_m<XT, EL, 4>(ctx, mode, a);
+  if (t <= 6) return launch_lowp_m<XT, EL, 6>(ctx, mode, a);
+  if (t <= 8) return launch_lowp_m<XT, EL, 8>(ctx, mode, a);
+  if (t <= 12) return launch_lowp_m<XT, EL, 12>(ctx, mode, a);
+  return launch_lowp_m<XT, EL, 16>(ctx, mode, a);
 }
 
 template <typename XT>
-int launch_lowp_x(drpt_ctx* ctx, int t, int mode, const KnnArgs& a) {
-  if (t <= 3) return launch_lowp_m<XT, 3>(ctx, mode, a);
-  if (t <= 4) return launch_lowp_m<XT, 4>(ctx, mode, a);
-  if (t <= 6) return launch_lowp_m<XT, 6>(ctx, mode, a);
-  if (t <= 8) return launch_lowp_m<XT, 8>(ctx, mode, a);
-  if (t <= 12) return launch_lowp_m<XT, 12>(ctx, mode, a);
-  return launch_lowp_m<XT, 16>(ctx, mode, a);
+int launch_lowp_e(drpt_ctx* ctx, int precision, int t, int mode, const KnnArgs& a) {
+  if (precision == DRPT_FP8) return launch_lowp_x<XT, ElFp8>(ctx, t, mode, a);
+  return launch_lowp_x<XT, ElBf16>(ctx, t, mode, a);
 }
 
 // per-thread slots of the bound pass (4 threads per row)
@@ -1569,14 +1757,21 @@ int dispatch_exact(drpt_ctx* ctx, const drpt_params* p, const XT* X, const int32
 }  // namespace
 
 int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, int precision,
-                     void* Xq, float* nrm2, float* err, float* /*scale*/) {
-  if (precision != DRPT_BF16) {
+                     void* Xq, float* nrm2, float* err, int8_t* pexp) {
+  if (precision != DRPT_BF16 && precision != DRPT_FP8) {
     set_error("prep_lowp: precision not supported");
     return DRPT_ERR_UNSUPPORTED;
   }
   const int64_t threads = n * 64;
   const unsigned blocks = (unsigned)((threads + 255) / 256);
-  if (dtype == DRPT_U8)
+  if (precision == DRPT_FP8) {
+    if (dtype == DRPT_U8)
+      hipLaunchKernelGGL(k_prep_fp8<uint8_t>, dim3(blocks), dim3(256), 0, ctx->stream,
+                         (const uint8_t*)X, n, d, (uint8_t*)Xq, nrm2, err, pexp);
+    else
+      hipLaunchKernelGGL(k_prep_fp8<float>, dim3(blocks), dim3(256), 0, ctx->stream,
+                         (const float*)X, n, d, (uint8_t*)Xq, nrm2, err, pexp);
+  } else if (dtype == DRPT_U8)
     hipLaunchKernelGGL(k_prep_bf16<uint8_t>, dim3(blocks), dim3(256), 0, ctx->stream,
                        (const uint8_t*)X, n, d, (uint16_t*)Xq, nrm2, err);
   else
@@ -1587,10 +1782,11 @@ int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, 
 }
 
 int launch_leaf_gather(drpt_ctx* ctx, const int32_t* perm, int64_t total, const float* nrm2,
-                       const float* err, float* lnbT, float* lerrT) {
+                       const float* err, const int8_t* pexp, float* lnbT, float* lerrT,
+                       int8_t* lexpT) {
   const int64_t blocks = std::min<int64_t>((total + 255) / 256, (int64_t)ctx->num_cus * 16);
   hipLaunchKernelGGL(k_leaf_gather, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0,
-                     ctx->stream, perm, total, nrm2, err, lnbT, lerrT);
+                     ctx->stream, perm, total, nrm2, err, pexp, lnbT, lerrT, lexpT);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }
@@ -1619,19 +1815,23 @@ int launch_lowp_knn(drpt_ctx* ctx, const drpt_params* p, int mode, KnnArgs a) {
     set_error("leaf_knn: nn must be in [1, 64]");
     return DRPT_ERR_UNSUPPORTED;
   }
-  if (p->precision != DRPT_BF16) {
+  if (p->precision != DRPT_BF16 && p->precision != DRPT_FP8) {
     set_error("leaf_knn: precision not supported");
     return DRPT_ERR_UNSUPPORTED;
   }
   if (kp < K || kp > 64) {
-    set_error("leaf_knn: kprime must be in [nn, 64] for BF16");
+    set_error("leaf_knn: kprime must be in [nn, 64] for BF16 / FP8");
     return DRPT_ERR_UNSUPPORTED;
+  }
+  if (a.precision != p->precision || (p->precision == DRPT_FP8 && !a.lexpT)) {
+    set_error("leaf_knn: quantised inputs do not match the precision");
+    return DRPT_ERR_ARG;
   }
   DRPT_CHECK(ws(ctx, "knn.queue", kQueues, &a.queue));
   DRPT_CHECK(fill_u32(ctx, a.queue, 0u, kQueues));
   const int t = slots_of(kp);
-  if (p->dtype == DRPT_U8) return launch_lowp_x<uint8_t>(ctx, t, mode, a);
-  return launch_lowp_x<float>(ctx, t, mode, a);
+  if (p->dtype == DRPT_U8) return launch_lowp_e<uint8_t>(ctx, p->precision, t, mode, a);
+  return launch_lowp_e<float>(ctx, p->precision, t, mode, a);
 }
 
 int launch_leaf_gather_val(drpt_ctx* ctx, const int32_t* perm, int64_t total, const float* src,
@@ -1645,8 +1845,10 @@ int launch_leaf_gather_val(drpt_ctx* ctx, const int32_t* perm, int64_t total, co
 
 int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a) {
   const int slots = a.S * a.ntsrc;
-  if (a.K < 1 || a.K > 64 || slots > 256 || a.rerank > 64 * 4 || a.ntsrc > 64) {
-    set_error("row_merge: unsupported sizes (nn in [1, 64], at most 256 candidate slots)");
+  if (a.K < 1 || a.K > 64 || slots > 512 || a.rerank > 64 * 4 || a.ntsrc > 64 ||
+      (a.src_key != nullptr && slots > 256)) {
+    set_error("row_merge: unsupported sizes (nn in [1, 64], at most 512 candidate slots, "
+              "256 with approximate keys)");
     return DRPT_ERR_UNSUPPORTED;
   }
   // a multiple of 8 blocks: every XCD chunk of rows has blocks (k_row_merge)
@@ -1654,8 +1856,11 @@ int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a) {
                                          (int64_t)ctx->num_cus * 16);
   const unsigned grid = (unsigned)(((std::max<int64_t>(want, 8) + 7) / 8) * 8);
   const bool u8 = dtype == DRPT_U8;
-  const int E = slots <= 64 ? 1 : slots <= 128 ? 2 : 4;
-  if (E == 1) {
+  const int E = slots <= 64 ? 1 : slots <= 128 ? 2 : slots <= 256 ? 4 : 8;
+  if (E == 8) {
+    if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 8>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+    else hipLaunchKernelGGL((k_row_merge<float, 8>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+  } else if (E == 1) {
     if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 1>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
     else hipLaunchKernelGGL((k_row_merge<float, 1>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
   } else if (E == 2) {

@@ -23,34 +23,44 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// bytes of the LDS header before the X tile (16-byte aligned)
-__host__ __device__ inline size_t r_bytes(int L, int nnz) {
-  return ((size_t)L * 4 + (size_t)(L + 1) * 4 + (size_t)nnz * 6 + 15) & ~(size_t)15;
+// bytes of the LDS header before the X tile (16-byte aligned): the column
+// maxima, plus R's compressed columns when they are staged (RL)
+__host__ __device__ inline size_t r_bytes(int L, int nnz, bool RL) {
+  return ((size_t)L * 4 + (RL ? (size_t)(L + 1) * 4 + (size_t)nnz * 6 : 0) + 15) & ~(size_t)15;
 }
 
 // TR rows per tile; lane -> (row = lane % TR, column slot = lane / TR).
 // Grid-stride over row tiles; the per-column max |P| is kept in LDS and
-// flushed once per block.
-template <typename XT, int TR>
+// flushed once per block. RL: R staged in LDS (else read through the caches:
+// every block reads the same few KB).
+template <typename XT, int TR, bool RL>
 __global__ void __launch_bounds__(kThreads)
 k_project(const XT* __restrict__ X, int64_t n, int d, int L, const int32_t* __restrict__ colptr,
           const uint16_t* __restrict__ rowidx, const float* __restrict__ w, float* __restrict__ P,
           uint32_t* __restrict__ colmax, int pitch) {
   extern __shared__ __align__(16) unsigned char smem[];
   // LDS: column maxima [L], R's compressed columns (colptr [L+1], weights, row
-  // indices), then the X tile
+  // indices) when RL, then the X tile
   const int nnz = colptr[L];
   uint32_t* lmax = reinterpret_cast<uint32_t*>(smem);
-  int32_t* lcp = reinterpret_cast<int32_t*>(lmax + L);
-  float* lw = reinterpret_cast<float*>(lcp + L + 1);
-  uint16_t* lri = reinterpret_cast<uint16_t*>(lw + nnz);
-  XT* tile = reinterpret_cast<XT*>(smem + r_bytes(L, nnz));
-  for (int c = threadIdx.x; c < L; c += kThreads) lmax[c] = 0u;
-  for (int c = threadIdx.x; c <= L; c += kThreads) lcp[c] = colptr[c];
-  for (int k = threadIdx.x; k < nnz; k += kThreads) {
-    lw[k] = w[k];
-    lri[k] = rowidx[k];
+  const int32_t* lcp = colptr;
+  const float* lw = w;
+  const uint16_t* lri = rowidx;
+  if constexpr (RL) {
+    int32_t* scp = reinterpret_cast<int32_t*>(lmax + L);
+    float* sw = reinterpret_cast<float*>(scp + L + 1);
+    uint16_t* sri = reinterpret_cast<uint16_t*>(sw + nnz);
+    for (int c = threadIdx.x; c <= L; c += kThreads) scp[c] = colptr[c];
+    for (int k = threadIdx.x; k < nnz; k += kThreads) {
+      sw[k] = w[k];
+      sri[k] = rowidx[k];
+    }
+    lcp = scp;
+    lw = sw;
+    lri = sri;
   }
+  XT* tile = reinterpret_cast<XT*>(smem + r_bytes(L, nnz, RL));
+  for (int c = threadIdx.x; c < L; c += kThreads) lmax[c] = 0u;
   constexpr int kSlots = 64 / TR;  // columns a wave works on at once
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -113,27 +123,57 @@ k_project(const XT* __restrict__ X, int64_t n, int d, int L, const int32_t* __re
     if (lmax[c]) atomicMax(&colmax[c], lmax[c]);
 }
 
-template <typename XT, int TR>
-int run(drpt_ctx* ctx, const XT* X, int64_t n, int d, const int32_t* colptr, const uint16_t* rowidx,
-        const float* w, int L, int nnz, float* P, uint32_t* colmax) {
-  int pitch;
+constexpr size_t kLdsBudget = 64 * 1024;
+
+// X tile row pitch in elements (odd word pitch: rows land on distinct banks)
+template <typename XT>
+int pitch_of(int d) {
   if (sizeof(XT) == 1) {
     int words = (d + 3) / 4;
-    if ((words & 1) == 0) words += 1;  // odd word pitch: rows land on distinct banks
-    pitch = words * 4;
-  } else {
-    pitch = d | 1;
+    if ((words & 1) == 0) words += 1;
+    return words * 4;
   }
-  size_t lds = r_bytes(L, nnz) + (size_t)TR * pitch * sizeof(XT);
-  if (lds > 64 * 1024) {
+  return d | 1;
+}
+
+template <typename XT, int TR, bool RL>
+int run(drpt_ctx* ctx, const XT* X, int64_t n, int d, const int32_t* colptr, const uint16_t* rowidx,
+        const float* w, int L, int nnz, float* P, uint32_t* colmax) {
+  const int pitch = pitch_of<XT>(d);
+  const size_t lds = r_bytes(L, nnz, RL) + (size_t)TR * pitch * sizeof(XT);
+  if (lds > kLdsBudget) {
     set_error("project: LDS tile too large for this dimension / column count");
     return DRPT_ERR_UNSUPPORTED;
   }
   int64_t blocks = std::min<int64_t>((n + TR - 1) / TR, (int64_t)ctx->num_cus * 8);
-  hipLaunchKernelGGL((k_project<XT, TR>), dim3((unsigned)blocks), dim3(kThreads), lds, ctx->stream,
-                     X, n, d, L, colptr, rowidx, w, P, colmax, pitch);
+  hipLaunchKernelGGL((k_project<XT, TR, RL>), dim3((unsigned)blocks), dim3(kThreads), lds,
+                     ctx->stream, X, n, d, L, colptr, rowidx, w, P, colmax, pitch);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
+}
+
+// the largest row tile that fits beside the header: R staged in LDS when
+// some tile of at least 8 rows fits beside it, else R read from the caches
+template <typename XT>
+int dispatch(drpt_ctx* ctx, const XT* X, int64_t n, int d, const int32_t* colptr,
+             const uint16_t* rowidx, const float* w, int L, int nnz, float* P, uint32_t* colmax) {
+  const size_t row_bytes = (size_t)pitch_of<XT>(d) * sizeof(XT);
+  for (int rl = 1; rl >= 0; --rl) {
+    const size_t hdr = r_bytes(L, nnz, rl != 0);
+    if (hdr >= kLdsBudget) continue;
+    const size_t room = kLdsBudget - hdr;
+#define DRPT_PROJ_TRY(TR)                                                                    \
+  if ((TR) * row_bytes <= room)                                                              \
+    return rl ? run<XT, TR, true>(ctx, X, n, d, colptr, rowidx, w, L, nnz, P, colmax)         \
+              : run<XT, TR, false>(ctx, X, n, d, colptr, rowidx, w, L, nnz, P, colmax);
+    DRPT_PROJ_TRY(64)
+    DRPT_PROJ_TRY(32)
+    DRPT_PROJ_TRY(16)
+    DRPT_PROJ_TRY(8)
+#undef DRPT_PROJ_TRY
+  }
+  set_error("project: dimension too large for the LDS tile");
+  return DRPT_ERR_UNSUPPORTED;
 }
 
 }  // namespace
@@ -143,22 +183,9 @@ int launch_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, co
                    uint32_t* colmax) {
   DRPT_CHECK(fill_u32(ctx, colmax, 0u, (size_t)L));
   if (n <= 0) return DRPT_OK;
-  const size_t kLdsBudget = 64 * 1024 - r_bytes(L, nnz);
-  if (r_bytes(L, nnz) >= 64 * 1024) {
-    set_error("project: projection matrix too large for LDS");
-    return DRPT_ERR_UNSUPPORTED;
-  }
-  if (dtype == DRPT_U8) {
-    return run<uint8_t, 64>(ctx, (const uint8_t*)X, n, d, colptr, rowidx, w, L, nnz, P, colmax);
-  }
-  const float* Xf = (const float*)X;
-  size_t row_bytes = (size_t)((d | 1)) * 4;
-  if (64 * row_bytes <= kLdsBudget) return run<float, 64>(ctx, Xf, n, d, colptr, rowidx, w, L, nnz, P, colmax);
-  if (32 * row_bytes <= kLdsBudget) return run<float, 32>(ctx, Xf, n, d, colptr, rowidx, w, L, nnz, P, colmax);
-  if (16 * row_bytes <= kLdsBudget) return run<float, 16>(ctx, Xf, n, d, colptr, rowidx, w, L, nnz, P, colmax);
-  if (8 * row_bytes <= kLdsBudget) return run<float, 8>(ctx, Xf, n, d, colptr, rowidx, w, L, nnz, P, colmax);
-  set_error("project: dimension too large for the LDS tile");
-  return DRPT_ERR_UNSUPPORTED;
+  if (dtype == DRPT_U8)
+    return dispatch<uint8_t>(ctx, (const uint8_t*)X, n, d, colptr, rowidx, w, L, nnz, P, colmax);
+  return dispatch<float>(ctx, (const float*)X, n, d, colptr, rowidx, w, L, nnz, P, colmax);
 }
 
 }  // namespace drpt

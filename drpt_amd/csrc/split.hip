@@ -393,6 +393,11 @@ k_part(const float* __restrict__ P, uint32_t* __restrict__ node, int64_t n, int 
   }
 }
 
+int split_levels(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t n, int T, int D,
+                 int mode, int leaf_max, int bitlen, uint32_t* node, uint32_t* cnt,
+                 uint32_t* split, uint32_t* H, unsigned long long* S1, unsigned long long* S2,
+                 float* mu, float* E, float* med);
+
 int bitlen_of(int64_t n) {
   int b = 0;
   while (b < 63 && (n >> b) != 0) ++b;
@@ -407,8 +412,7 @@ int grow_trees_impl(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64
     set_error("grow_trees: depth must be in [1, 24], ntrees >= 1, n >= 1");
     return DRPT_ERR_ARG;
   }
-  DRPT_CHECK(fill_u32(ctx, node, 1u, (size_t)T * n));
-  if (D == 1) return DRPT_OK;
+  if (D == 1) return fill_u32(ctx, node, 1u, (size_t)T * n);
   const int64_t NMAX = (int64_t)1 << (D - 2);
   uint32_t *cnt, *split, *H;
   unsigned long long *S1, *S2;
@@ -422,7 +426,27 @@ int grow_trees_impl(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64
   DRPT_CHECK(ws(ctx, "split.E", T * NMAX * 16, &E));
   DRPT_CHECK(ws(ctx, "split.H", T * NMAX * 16, &H));
   const int bitlen = bitlen_of(n);
+  // the per-level launches (4 fills + 7 kernels per level) as one hipGraph,
+  // replayed while sizes, parameters and buffers stay the same
+  char key[512];
+  snprintf(key, sizeof key, "%p %p %lld %d %d %d %d %p %p %p %p %p %p %p %p %p %p", (const void*)P,
+           (const void*)colmax, (long long)n, T, D, mode, leaf_max, (void*)node, (void*)cnt,
+           (void*)split, (void*)S1, (void*)S2, (void*)mu, (void*)med, (void*)E, (void*)H,
+           (void*)ctx->stream);
+  return run_graph(ctx, "split", key, [&]() -> int {
+    return split_levels(ctx, P, colmax, n, T, D, mode, leaf_max, bitlen, node, cnt, split, H, S1, S2,
+                        mu, E, med);
+  });
+}
+
+namespace {
+
+int split_levels(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t n, int T, int D,
+                 int mode, int leaf_max, int bitlen, uint32_t* node, uint32_t* cnt,
+                 uint32_t* split, uint32_t* H, unsigned long long* S1, unsigned long long* S2,
+                 float* mu, float* E, float* med) {
   hipStream_t s = ctx->stream;
+  DRPT_CHECK(fill_u32(ctx, node, 1u, (size_t)T * n));
   for (int l = 0; l <= D - 2; ++l) {
     const int64_t NN = (int64_t)1 << l;
     const int64_t TN = (int64_t)T * NN;
@@ -478,4 +502,5 @@ int grow_trees_impl(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64
   return DRPT_OK;
 }
 
+}  // namespace
 }  // namespace drpt

@@ -22,4 +22,34 @@ int fill_u32(drpt_ctx* ctx, uint32_t* p, uint32_t v, size_t count) {
   return DRPT_OK;
 }
 
+int run_graph(drpt_ctx* ctx, const char* name, const std::string& key,
+              const std::function<int()>& record) {
+  if (!ctx->use_graphs) return record();
+  drpt_ctx::Graph& g = ctx->graphs[name];
+  if (g.exec == nullptr || g.key != key) {
+    if (g.exec) {
+      DRPT_HIP(hipStreamSynchronize(ctx->stream));  // a replay may still run
+      DRPT_HIP(hipGraphExecDestroy(g.exec));
+      g.exec = nullptr;
+    }
+    g.key.clear();
+    DRPT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = record();
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc != DRPT_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    DRPT_HIP(e);
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) g.exec = nullptr;
+    DRPT_HIP(ei);
+    g.key = key;
+  }
+  DRPT_HIP(hipGraphLaunch(g.exec, ctx->stream));
+  return DRPT_OK;
+}
+
 }  // namespace drpt

@@ -55,7 +55,9 @@ typedef struct {
   int32_t kprime;       /* widened candidate count for BF16/FP8 (>= 2*nn; 0 = default) */
   uint32_t seed;        /* projection seed (reference draws it from random_device) */
   float density;        /* projection density (main.cpp:129-132 default 1/sqrt(d)) */
-  int32_t reserved[5];
+  int32_t tree_group;   /* BF16/FP8 builds: trees per pooled filter pass, each followed by a
+                           merge that tightens the bound (0 = automatic) */
+  int32_t reserved[4];
 } drpt_params;
 
 typedef struct {

@@ -32,15 +32,29 @@ CASES = {
     "int128_f32_4000": lambda: datagen.int_mixture(4000),
     "gauss_d37_1500": lambda: np.random.default_rng(5).standard_normal((1500, 37)).astype(np.float32),
     "gauss960_800": lambda: datagen.gauss_mixture(800),
+    # rows scaled by 10^U(-3, 3), signs mixed, one zero row: per-row fp8 scales
+    "scaled_d64_2000": lambda: _scaled_rows(2000, 64),
 }
 
 
-@pytest.mark.parametrize("case", list(CASES))
-def test_project_bitexact(ctx, orc, case):
-    _log(f"[project] {case}")
+def _scaled_rows(n, d):
+    rng = np.random.default_rng(11)
+    base = rng.standard_normal((n, d)).astype(np.float32)
+    base *= (10.0 ** rng.uniform(-3, 3, size=(n, 1))).astype(np.float32)
+    base[7] = 0.0
+    return base
+
+
+@pytest.mark.parametrize("case,L", [(c, 24) for c in CASES] +
+                         [("mnist_u8_3000", 160), ("gauss960_800", 100), ("gauss960_800", 240),
+                          ("gauss960_800", 600)])
+def test_project_bitexact(ctx, orc, case, L):
+    """X.R bit for bit; L = 160 / 100 shrink the row tile beside the staged R,
+    L = 240 / 600 leave R in global memory."""
+    _log(f"[project] {case} L={L}")
     X = CASES[case]()
     n, d = X.shape
-    R = orc.build_R(d, 24, 1.0 / np.sqrt(d), 17)
+    R = orc.build_R(d, L, 1.0 / np.sqrt(d), 17)
     P0, cm0 = orc.project(X, R)
     P, cm = ctx.project(_dev(X), R)
     assert np.array_equal(_bits(P.cpu().numpy()), _bits(P0))
@@ -63,6 +77,42 @@ def test_grow_trees_bitexact(ctx, orc, case, mode, leaf_max):
     for t in range(T):
         ref = orc.grow_tree(P0, cm0, D, t, mode, leaf_max, 0)
         assert np.array_equal(heap[t], ref), f"tree {t}: {(heap[t] != ref).sum()} points differ"
+
+
+def test_split_graph_replay(ctx, orc):
+    """The split levels run as a captured hipGraph: replays into the same buffers
+    (with eager work and a clobbered output between them), a recapture for a
+    new size and a recapture back all equal the oracle, and equal a context
+    that launches eagerly (DRPT_GRAPHS=0)."""
+    import os
+    import torch
+    _log("[graph] split replay")
+    X = CASES["int128_f32_4000"]()
+    n, d = X.shape
+    T, D = 3, 9
+    R = orc.build_R(d, T * D, 1.0 / np.sqrt(d), 29)
+    P0, cm0 = orc.project(X, R)
+    refs = {m: np.stack([orc.grow_tree(P0[:, :m].copy(), cm0, D, t, 1, 64, 0) for t in range(T)])
+            for m in (n, 3000)}
+    Pd, cmd = _dev(P0), _dev(cm0.view(np.int32))
+    Pd3 = _dev(P0[:, :3000])
+    heap = torch.empty((T, n), dtype=torch.int32, device="cuda:0")
+    heap3 = torch.empty((T, 3000), dtype=torch.int32, device="cuda:0")
+    for it, (P, out, m) in enumerate([(Pd, heap, n), (Pd, heap, n), (Pd, heap, n), (Pd3, heap3, 3000),
+                                      (Pd, heap, n), (Pd, heap, n)]):
+        out.fill_(-7)
+        ctx.project(_dev(X[:64]), R)  # eager work between replays
+        torch.cuda.synchronize()
+        got = ctx.grow_trees(P, cmd, T, D, 1, 64, out=out).cpu().numpy().view(np.uint32)
+        assert np.array_equal(got, refs[m]), f"call {it} (n={m}) differs from the oracle"
+    os.environ["DRPT_GRAPHS"] = "0"
+    try:
+        eager = drpt_amd.Context(0)
+    finally:
+        del os.environ["DRPT_GRAPHS"]
+    with eager:
+        got = eager.grow_trees(Pd, cmd, T, D, 1, 64).cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, refs[n])
 
 
 def _groups(heap_t):
@@ -92,18 +142,22 @@ def test_tree_knn_fp32_bitexact(ctx, orc, case, K):
         assert np.array_equal(_bits(cd[t]), _bits(rd))
 
 
-@pytest.mark.parametrize("case,K,leaf_max,kprime", [("int128_f32_4000", 10, 150, 0),
-                                                    ("int128_f32_4000", 10, 700, 12),
-                                                    ("gauss_d37_1500", 7, 150, 0),
-                                                    ("mnist_u8_3000", 16, 300, 0),
-                                                    ("gauss960_800", 32, 150, 0)])
-def test_tree_knn_bf16_bitexact(ctx, orc, case, K, leaf_max, kprime):
-    """Per-tree BF16 lists (bound pass + filter pass, exact re-rank) equal the
-    oracle's exact per-leaf lists bit for bit."""
-    _log(f"[tree_knn bf16] {case} K={K} leaf_max={leaf_max} kprime={kprime}")
+LOWP_TREE_CASES = [("int128_f32_4000", 10, 150, 0), ("int128_f32_4000", 10, 700, 12),
+                   ("gauss_d37_1500", 7, 150, 0), ("mnist_u8_3000", 16, 300, 0),
+                   ("gauss960_800", 32, 150, 0)]
+
+
+@pytest.mark.parametrize("prec,case,K,leaf_max,kprime",
+                         [("bf16",) + c for c in LOWP_TREE_CASES] +
+                         [("fp8",) + c for c in LOWP_TREE_CASES] +
+                         [("fp8", "scaled_d64_2000", 10, 200, 0), ("fp8", "mnist_u8_3000", 10, 300, 64)])
+def test_tree_knn_lowp_bitexact(ctx, orc, prec, case, K, leaf_max, kprime):
+    """Per-tree BF16 / FP8 lists (bound pass + filter pass, exact re-rank)
+    equal the oracle's exact per-leaf lists bit for bit."""
+    _log(f"[tree_knn {prec}] {case} K={K} leaf_max={leaf_max} kprime={kprime}")
     X = CASES[case]()
     n, d = X.shape
-    p = drpt_amd.leafmax_params(n, d, 2, K, leaf_max=leaf_max, precision="bf16", kprime=kprime,
+    p = drpt_amd.leafmax_params(n, d, 2, K, leaf_max=leaf_max, precision=prec, kprime=kprime,
                                 seed=3)
     p.dtype = drpt_amd.DRPT_U8 if X.dtype == np.uint8 else drpt_amd.DRPT_F32
     R = orc.build_R(d, p.ntrees * p.depth, p.density, p.seed)
@@ -119,41 +173,47 @@ def test_tree_knn_bf16_bitexact(ctx, orc, case, K, leaf_max, kprime):
         assert np.array_equal(_bits(cd[t]), _bits(rd))
 
 
-@pytest.mark.parametrize("case,T", [("int128_f32_4000", 4), ("int128_f32_4000", 7),
-                                    ("gauss_d37_1500", 6), ("mnist_u8_3000", 6),
-                                    ("gauss960_800", 5)])
-def test_bf16_lists_equal_fp32(ctx, case, T):
-    """Whole build: BF16 (trees beyond the first four pruned against the merged
-    bound) returns the FP32 build's lists."""
-    _log(f"[bf16==fp32] {case} T={T}")
+LOWP_BUILD_CASES = [("int128_f32_4000", 4), ("int128_f32_4000", 7), ("gauss_d37_1500", 6),
+                    ("mnist_u8_3000", 6), ("gauss960_800", 5)]
+
+
+@pytest.mark.parametrize("prec,case,T", [("bf16",) + c for c in LOWP_BUILD_CASES] +
+                         [("fp8",) + c for c in LOWP_BUILD_CASES] +
+                         [("fp8", "scaled_d64_2000", 6)])
+def test_lowp_lists_equal_fp32(ctx, prec, case, T):
+    """Whole build: BF16 / FP8 (trees beyond the first four pruned against the
+    merged bound) returns the FP32 build's lists."""
+    _log(f"[{prec}==fp32] {case} T={T}")
     X = CASES[case]()
     n, d = X.shape
     out = {}
-    for prec in ("fp32", "bf16"):
-        p = drpt_amd.leafmax_params(n, d, T, 10, leaf_max=200, precision=prec, seed=5)
+    for pr in ("fp32", prec):
+        p = drpt_amd.leafmax_params(n, d, T, 10, leaf_max=200, precision=pr, seed=5)
         p.dtype = drpt_amd.DRPT_U8 if X.dtype == np.uint8 else drpt_amd.DRPT_F32
         oi, od, tm = ctx.build_graph(p, _dev(X))
-        out[prec] = (oi.cpu().numpy(), od.cpu().numpy(), tm)
-    assert np.array_equal(out["fp32"][0], out["bf16"][0])
-    assert np.array_equal(_bits(out["fp32"][1]), _bits(out["bf16"][1]))
-    tm = out["bf16"][2]
+        out[pr] = (oi.cpu().numpy(), od.cpu().numpy(), tm)
+    assert np.array_equal(out["fp32"][0], out[prec][0])
+    assert np.array_equal(_bits(out["fp32"][1]), _bits(out[prec][1]))
+    tm = out[prec][2]
     _log(f"  fallback rows {tm.n_fallback_rows} of {n * T}")
-    assert tm.n_fallback_rows <= n * T // 100
+    if prec == "bf16":  # fp8's wider error bound may send many rows to the exact path
+        assert tm.n_fallback_rows <= n * T // 100
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
 @pytest.mark.parametrize("copies,nbase,T,overflow", [(20, 20, 8, False), (400, 5, 3, True),
                                                      (400, 5, 6, True)])
-def test_bf16_duplicates_overflow(ctx, orc, copies, nbase, T, overflow):
+def test_lowp_duplicates_overflow(ctx, orc, prec, copies, nbase, T, overflow):
     """Each vector repeated: ties at distance 0 fill the candidate pools (20
     copies over 8 trees: the same duplicates in every tree's pool segment,
     merged once; 400 copies: segments overflow and those rows are redone
     exactly over every tree); the lists still equal the oracle's."""
-    _log(f"[edge] bf16 duplicates copies={copies} T={T}")
+    _log(f"[edge] {prec} duplicates copies={copies} T={T}")
     rng = np.random.default_rng(4)
     base = rng.integers(0, 50, size=(nbase, 16)).astype(np.float32)
     X = np.repeat(base, copies, axis=0)
     n, d = X.shape
-    p = drpt_amd.leafmax_params(n, d, T, 10, leaf_max=300, precision="bf16", seed=8)
+    p = drpt_amd.leafmax_params(n, d, T, 10, leaf_max=300, precision=prec, seed=8)
     oi, od, tm = ctx.build_graph(p, _dev(X))
     ri, rd, _, _ = orc.build_graph(X, T, p.depth, 10, local_mode=1, leaf_max=300, seed=8,
                                    density=p.density)
