@@ -995,9 +995,9 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
       // kModePool flush: thread (row r = tid / 4, quarter tid % 4) drops the
       // row's known ids from the list of the thread that owned (r, quarter) in
       // the tiles, the 4 lanes of a row reserve pool slots with one atomic.
-      // Runs at the end of the pass, and between tiles once some list could
-      // overflow during the next tile (flag set in the epilogue, read after
-      // a barrier, so every thread takes the same branch).
+      // Runs at the end of the pass, and between tiles once some list is half
+      // full (flag set in the epilogue, read after a barrier, so every thread
+      // takes the same branch).
       auto flush = [&]() {
         if constexpr (MODE == kModePool) {
           S.ccnt[q * 4 + sub] = nsub;  // the owner's count, visible after the barrier
@@ -1086,8 +1086,10 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
           }
         }
         if constexpr (MODE == kModePool) {
-          // a tile adds at most 16 entries per thread
-          if (nsub > kSubCap - 16) S.flush_flag = 1;
+          // flushed at half capacity: a tile adds at most 16 entries per
+          // thread but rarely more than a few; a list that still overflows
+          // sends its row to the exact row path
+          if (nsub > kSubCap / 2) S.flush_flag = 1;
         }
       }, hook);
       if constexpr (MODE == kModePool) {
