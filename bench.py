@@ -10,7 +10,7 @@ FP64 brute force on sampled queries, the roofline of the leaf k-NN kernel and
 the oracle (CPU restatement, OpenMP) on a bounded sample of the same workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c3|c4]
-                    [--precision bf16|fp8|fp32] [--no-cpu] [--no-fp32]
+                    [--precision bf16|fp8|fp32] [--no-cpu] [--no-fp32] [--no-autotune]
 """
 import argparse
 import json
@@ -146,6 +146,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip drpt_autotune (work-item layout per leaf-size bucket) before warm-up")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scale for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -193,6 +195,11 @@ def main():
         log(f"[bench] {label}: {steps} steps in {el:.3f} s")
         return el / steps * 1e3, tms
 
+    plan = None
+    if prec != "fp32" and not a.no_autotune:
+        t = time.perf_counter()
+        plan = ctx.autotune(p, Xd)  # one-time set-up, kept by the context
+        log(f"[bench] autotune {plan} in {time.perf_counter() - t:.2f} s")
     ms, tms = run(p, a.steps, a.warmup, prec)
     idx_main = out[0].clone()
     dist_main = out[1].clone()
@@ -209,7 +216,7 @@ def main():
                       "kprime": p.kprime if prec != "fp32" else 0,
                       "local_mode": w["mode"]},
            "phases_ms": phases, "n_leaves": tm_last["n_leaves"], "max_leaf": tm_last["max_leaf"],
-           "fallback_rows": tm_last["n_fallback_rows"]}
+           "fallback_rows": tm_last["n_fallback_rows"], "item_plan": plan}
 
     log("[bench] recall")
     r, rref, derr = recall_fp64(Xd, idx_main, p.nn, dist=dist_main)

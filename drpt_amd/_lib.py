@@ -26,7 +26,8 @@ DRPT_TREE_FIXED, DRPT_TREE_LEAFMAX = 0, 1
 EXPORTS = [
     "drpt_last_error", "drpt_version", "drpt_create", "drpt_destroy", "drpt_sync",
     "drpt_get_stream", "drpt_derive_params", "drpt_build_projection", "drpt_project",
-    "drpt_grow_trees", "drpt_tree_knn", "drpt_merge", "drpt_build_graph",
+    "drpt_grow_trees", "drpt_tree_knn", "drpt_merge", "drpt_build_graph", "drpt_autotune",
+    "drpt_set_item_plan",
 ]
 
 
@@ -92,6 +93,8 @@ def load():
     L.drpt_tree_knn.argtypes = [vp, ctypes.POINTER(Params), vp, vp, vp, vp, ctypes.POINTER(Timings)]
     L.drpt_merge.argtypes = [vp, i64, i32, i32, vp, vp, vp, vp]
     L.drpt_build_graph.argtypes = [vp, ctypes.POINTER(Params), vp, vp, vp, ctypes.POINTER(Timings)]
+    L.drpt_autotune.argtypes = [vp, ctypes.POINTER(Params), vp, vp]
+    L.drpt_set_item_plan.argtypes = [vp, ctypes.POINTER(Params), vp]
     for name in EXPORTS:
         fn = getattr(L, name)
         if fn.restype is ctypes.c_int or name in ("drpt_create", "drpt_destroy", "drpt_sync"):

@@ -5,6 +5,7 @@
 // (cpp/src/drpt/algo/mdrpt.cpp:46-121) and MDRPT::gather_nns (:273-291) on
 // every MPI rank; here one context drives one GPU and one stream.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -266,7 +267,7 @@ int drpt_grow_trees(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64
 }
 
 // Final leaves as work items: the points of every leaf contiguous in G->perm
-// (tree-major) and items of rows_per_item rows; items of trees < first_trees
+// (tree-major) and items of plan.rpi[bucket] rows; items of trees < first_trees
 // come first. Leaf-max builds do this on the device; the reference local stage
 // derives its position tables on the host.
 struct Grouping {
@@ -280,7 +281,8 @@ struct Grouping {
 };
 
 static int group_impl(drpt_ctx* ctx, const drpt_params* p, const uint32_t* leaf_heap,
-                      int rows_per_item, int first_trees, Grouping* G) {
+                      const ItemPlan& plan, int first_trees, Grouping* G,
+                      bool items_only = false) {
   const int T = p->ntrees, D = p->depth, K = p->nn;
   const int64_t n = p->n;
   const int64_t HS = (int64_t)1 << D;
@@ -302,16 +304,20 @@ static int group_impl(drpt_ctx* ctx, const drpt_params* p, const uint32_t* leaf_
   DRPT_CHECK(ws(ctx, "group.perm", T * n, &perm));
   DRPT_CHECK(ws(ctx, "knn.items", item_cap, &items));
   DRPT_CHECK(fill_u32(ctx, (uint32_t*)stats, 0u, 8));
-  DRPT_CHECK(launch_count_leaves(ctx, leaf_heap, n, T, D, hcount));
+  // items_only (leaf-max builds): new work items over the leaves and point
+  // order of the previous call (the scatter's order within a leaf is not
+  // reproducible, and leaf-ordered copies made since depend on it)
+  if (!items_only || p->local_mode != DRPT_LOCAL_LEAFMAX)
+    DRPT_CHECK(launch_count_leaves(ctx, leaf_heap, n, T, D, hcount));
   const int32_t* perm_final = perm;
   double host_gram = -1.0;
   int64_t host_leaves = 0, host_max = 0;
 
   if (p->local_mode == DRPT_LOCAL_LEAFMAX) {
     // everything on the device: offsets, work items, scatter
-    DRPT_CHECK(launch_device_items(ctx, hcount, T, D, rows_per_item, first_trees, hoff, tree_items,
+    DRPT_CHECK(launch_device_items(ctx, hcount, T, D, plan, first_trees, hoff, tree_items,
                                    base, range, items, stats));
-    DRPT_CHECK(launch_scatter_leaves(ctx, leaf_heap, n, T, D, hoff, cursor, perm));
+    if (!items_only) DRPT_CHECK(launch_scatter_leaves(ctx, leaf_heap, n, T, D, hoff, cursor, perm));
   } else {
     // reference local stage: the host derives the position tables, so the
     // global leaf sizes come back once
@@ -404,8 +410,10 @@ static int group_impl(drpt_ctx* ctx, const drpt_params* p, const uint32_t* leaf_
     for (auto& l : final_leaves) {
       host_gram += 2.0 * p->d * (double)l.size * (double)l.size;
       host_max = std::max<int64_t>(host_max, l.size);
-      for (int64_t r0 = 0; r0 < l.size; r0 += rows_per_item) {
-        hitems.push_back({l.tree, l.size, l.start, (int32_t)r0, 0});
+      const int rpi = plan.rpi[size_bucket(l.size)];
+      for (int64_t r0 = 0; r0 < l.size; r0 += rpi) {
+        hitems.push_back({l.tree, l.size, l.start, (int32_t)r0,
+                          (int32_t)std::min<int64_t>(rpi, l.size - r0)});
         if (l.tree < first_trees) ++nfirst;
       }
     }
@@ -511,7 +519,7 @@ static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
   hipStream_t s = ctx->stream;
   DRPT_HIP(hipEventRecord(ctx->ev[6], s));
   Grouping G;
-  DRPT_CHECK(group_impl(ctx, p, leaf_heap, 64, T, &G));
+  DRPT_CHECK(group_impl(ctx, p, leaf_heap, uniform_plan(64), T, &G));
   drpt_params pp = *p;
   pp.kprime = kprime_of(p);
   int32_t *fb = nullptr, *nfb = nullptr;
@@ -533,7 +541,9 @@ static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
     DRPT_CHECK(fill_u32(ctx, (uint32_t*)nfb, 0u, 1));
     a.items = G.items;
     a.range = G.range + 4;
-    a.rows_per_item = 64;
+    a.size_lo = 0;
+    a.size_hi = INT_MAX;
+    a.ring = 0;
     a.out_idx = cand_idx;
     a.out_dist = cand_dist;
     a.fallback = fb;
@@ -588,65 +598,142 @@ static int tree_group_auto(int T, int K) {
   return std::max(1, std::min(T, std::max(4, 160 / std::max(K, 1))));
 }
 
-static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
-                          const uint32_t* leaf_heap, int32_t* out_idx, float* out_dist,
-                          drpt_timings* timings) {
-  const int T = p->ntrees, K = p->nn;
-  const int64_t n = p->n;
-  hipStream_t s = ctx->stream;
-  drpt_params pp = *p;
-  pp.kprime = kprime_of(p);
-  if (pp.kprime < K) {
+// The low-precision pipeline's state: grouping, quantised inputs, buffers.
+struct LowpRun {
+  drpt_params pp;
+  Grouping G;
+  KnnArgs a{};
+  int T = 0, K = 0, KP = 0, first = 0, pool_cap = 0;
+  int64_t n = 0;
+  uint64_t* s1 = nullptr;
+  int32_t *ki = nullptr, *pool = nullptr, *pcnt = nullptr, *oflag = nullptr, *olist = nullptr,
+          *novf = nullptr;
+  float *kd = nullptr, *boundT = nullptr;
+};
+
+static std::string plan_key(const drpt_params* p) {
+  char k[256];
+  snprintf(k, sizeof k, "%lld %d %d %d %d %d %d %d %d %d %d", (long long)p->n, p->d, p->dtype,
+           p->precision, p->nn, p->ntrees, p->depth, p->leaf_max, p->local_mode, kprime_of(p),
+           p->tree_group);
+  return k;
+}
+
+static ItemPlan lowp_default_plan() { return uniform_plan(1 << 30); }  // whole leaves, ring
+
+static ItemPlan lowp_plan_of(drpt_ctx* ctx, const drpt_params* p) {
+  auto it = ctx->plans.find(plan_key(p));
+  if (it == ctx->plans.end()) return lowp_default_plan();
+  ItemPlan plan;
+  for (int b = 0; b < kSizeBuckets; ++b) {
+    plan.rpi[b] = it->second[b];
+    plan.ring[b] = it->second[kSizeBuckets + b];
+  }
+  return plan;
+}
+
+// grouping with `plan`, quantised copies, buffers; the pool and overflow
+// state cleared
+static int lowp_setup(drpt_ctx* ctx, const drpt_params* p, const void* X,
+                      const uint32_t* leaf_heap, const ItemPlan& plan, LowpRun* r) {
+  r->pp = *p;
+  r->pp.kprime = kprime_of(p);
+  r->T = p->ntrees;
+  r->K = p->nn;
+  r->n = p->n;
+  if (r->pp.kprime < r->K) {
     set_error("kprime must be >= nn");
     return DRPT_ERR_ARG;
   }
-  const int KP = lowp_slots(&pp);
-  const int first = std::max(1, std::min(std::min(T, kFirstTrees), 256 / KP));
-  const int rows_per_item = 1 << 30;  // one item per leaf
-  DRPT_HIP(hipEventRecord(ctx->ev[6], s));
-  Grouping G;
-  DRPT_CHECK(group_impl(ctx, p, leaf_heap, rows_per_item, first, &G));
-  KnnArgs a{};
-  DRPT_CHECK(prep_lowp(ctx, &pp, X, G.perm, &a));
-  uint64_t* s1;
-  int32_t *ki, *pool, *pcnt, *oflag, *olist, *novf;
-  float *kd, *boundT;
-  DRPT_CHECK(ws(ctx, "knn.s1", (size_t)first * n * KP, &s1));
-  DRPT_CHECK(ws(ctx, "knn.known_idx", (size_t)n * K, &ki));
-  DRPT_CHECK(ws(ctx, "knn.known_dist", (size_t)n * K, &kd));
-  DRPT_CHECK(ws(ctx, "knn.boundT", (size_t)T * n, &boundT));
-  const int pool_cap = pool_cap_of(K);
-  DRPT_CHECK(ws(ctx, "knn.pool", (size_t)n * pool_cap, &pool));
-  DRPT_CHECK(ws(ctx, "knn.pool_cnt", n, &pcnt));
-  DRPT_CHECK(fill_u32(ctx, (uint32_t*)pcnt, 0u, n));
-  DRPT_CHECK(ws(ctx, "knn.ovf_flag", n, &oflag));
-  DRPT_CHECK(ws(ctx, "knn.ovf_list", n, &olist));
-  DRPT_CHECK(ws(ctx, "knn.novf", 1, &novf));
-  DRPT_CHECK(fill_u32(ctx, (uint32_t*)oflag, 0u, n));
-  DRPT_CHECK(fill_u32(ctx, (uint32_t*)novf, 0u, 1));
-  a.items = G.items;
-  a.rows_per_item = rows_per_item;
-  DRPT_HIP(hipEventRecord(ctx->ev[7], s));
-  // 1. widened candidates of the first trees
-  a.range = G.range + 0;
-  a.out_key = s1;
-  a.tree0 = 0;
-  DRPT_CHECK(launch_lowp_knn(ctx, &pp, kModeCand, a));
-  // 2. known lists and bounds
+  r->KP = lowp_slots(&r->pp);
+  r->first = std::max(1, std::min(std::min(r->T, kFirstTrees), 256 / r->KP));
+  r->pool_cap = pool_cap_of(r->K);
+  const int64_t n = r->n;
+  DRPT_CHECK(group_impl(ctx, p, leaf_heap, plan, r->first, &r->G));
+  DRPT_CHECK(prep_lowp(ctx, &r->pp, X, r->G.perm, &r->a));
+  DRPT_CHECK(ws(ctx, "knn.s1", (size_t)r->first * n * r->KP, &r->s1));
+  DRPT_CHECK(ws(ctx, "knn.known_idx", (size_t)n * r->K, &r->ki));
+  DRPT_CHECK(ws(ctx, "knn.known_dist", (size_t)n * r->K, &r->kd));
+  DRPT_CHECK(ws(ctx, "knn.boundT", (size_t)r->T * n, &r->boundT));
+  DRPT_CHECK(ws(ctx, "knn.pool", (size_t)n * r->pool_cap, &r->pool));
+  DRPT_CHECK(ws(ctx, "knn.pool_cnt", n, &r->pcnt));
+  DRPT_CHECK(ws(ctx, "knn.ovf_flag", n, &r->oflag));
+  DRPT_CHECK(ws(ctx, "knn.ovf_list", n, &r->olist));
+  DRPT_CHECK(ws(ctx, "knn.novf", 1, &r->novf));
+  DRPT_CHECK(fill_u32(ctx, (uint32_t*)r->pcnt, 0u, n));
+  DRPT_CHECK(fill_u32(ctx, (uint32_t*)r->oflag, 0u, n));
+  DRPT_CHECK(fill_u32(ctx, (uint32_t*)r->novf, 0u, 1));
+  KnnArgs& a = r->a;
+  a.items = r->G.items;
+  a.out_idx = r->pool;
+  a.out_dist = nullptr;
+  a.boundT = r->boundT;
+  a.pool_cnt = r->pcnt;
+  a.pool_cap = r->pool_cap;
+  a.ovf_flag = r->oflag;
+  a.ovf_list = r->olist;
+  a.novf = r->novf;
+  return DRPT_OK;
+}
+
+// one low-precision pass over items [range[0], range[1]): a single launch when
+// every size bucket uses the same Gram pipeline, else one launch per bucket
+// (items of other buckets skipped); bucket >= 0 restricts it to that bucket
+static int lowp_pass(drpt_ctx* ctx, LowpRun* r, int mode, const ItemPlan& plan,
+                     const int64_t* range, int bucket = -1) {
+  KnnArgs a = r->a;
+  a.range = range;
+  bool uniform = true;
+  for (int b = 1; b < kSizeBuckets; ++b) uniform = uniform && plan.ring[b] == plan.ring[0];
+  if (bucket < 0 && uniform) {
+    a.size_lo = 0;
+    a.size_hi = INT_MAX;
+    a.ring = plan.ring[0];
+    return launch_lowp_knn(ctx, &r->pp, mode, a);
+  }
+  for (int b = 0; b < kSizeBuckets; ++b) {
+    if (bucket >= 0 && b != bucket) continue;
+    bucket_sizes(b, &a.size_lo, &a.size_hi);
+    a.ring = plan.ring[b];
+    DRPT_CHECK(launch_lowp_knn(ctx, &r->pp, mode, a));
+  }
+  return DRPT_OK;
+}
+
+// 1. widened candidates of the first trees; 2. known lists (ki, kd) and bounds
+static int lowp_known(drpt_ctx* ctx, LowpRun* r, const void* X, const ItemPlan& plan) {
+  r->a.out_key = r->s1;
+  r->a.tree0 = 0;
+  DRPT_CHECK(lowp_pass(ctx, r, kModeCand, plan, r->G.range + 0));
   RowMergeArgs m{};
   m.X = X;
-  m.n = n;
-  m.d = p->d;
-  m.K = K;
-  m.src_key = s1;
-  m.src_tree_stride = n * KP;
-  m.S = KP;
-  m.ntsrc = first;
-  m.rerank = std::min(2 * K, first * KP);
-  m.out_idx = ki;
-  m.out_dist = kd;
-  m.order = G.perm;  // tree 0's leaf order
-  DRPT_CHECK(launch_row_merge(ctx, p->dtype, m));
+  m.n = r->n;
+  m.d = r->pp.d;
+  m.K = r->K;
+  m.src_key = r->s1;
+  m.src_tree_stride = r->n * r->KP;
+  m.S = r->KP;
+  m.ntsrc = r->first;
+  m.rerank = std::min(2 * r->K, r->first * r->KP);
+  m.out_idx = r->ki;
+  m.out_dist = r->kd;
+  m.order = r->G.perm;  // tree 0's leaf order
+  return launch_row_merge(ctx, r->pp.dtype, m);
+}
+
+static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
+                          const uint32_t* leaf_heap, int32_t* out_idx, float* out_dist,
+                          drpt_timings* timings) {
+  hipStream_t s = ctx->stream;
+  const ItemPlan plan = lowp_plan_of(ctx, p);
+  DRPT_HIP(hipEventRecord(ctx->ev[6], s));
+  LowpRun r;
+  DRPT_CHECK(lowp_setup(ctx, p, X, leaf_heap, plan, &r));
+  const int T = r.T, K = r.K;
+  const int64_t n = r.n;
+  const Grouping& G = r.G;
+  DRPT_HIP(hipEventRecord(ctx->ev[7], s));
+  DRPT_CHECK(lowp_known(ctx, &r, X, plan));
   // 3. pools over groups of trees, each merged into the known lists before
   // the next group (duplicates across the group's trees removed, the bound
   // tightened); the last merge writes the output
@@ -667,53 +754,44 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   }
   for (int g = 0; g < ngroups; ++g)
     DRPT_CHECK(launch_tree_range(ctx, G.base, g * gsz, std::min(T, (g + 1) * gsz), granges + 2 * g));
-  a.out_idx = pool;
-  a.out_dist = nullptr;
-  a.boundT = boundT;
-  a.pool_cnt = pcnt;
-  a.pool_cap = pool_cap;
-  a.ovf_flag = oflag;
-  a.ovf_list = olist;
-  a.novf = novf;
-  int32_t* kin = ki;  // known lists in, merged lists out (ping-pong)
-  float* kdn = kd;
+  int32_t* kin = r.ki;  // known lists in, merged lists out (ping-pong)
+  float* kdn = r.kd;
   for (int g = 0; g < ngroups; ++g) {
     const int t0 = g * gsz, t1 = std::min(T, t0 + gsz);
     const bool last = g + 1 == ngroups;
     DRPT_CHECK(launch_leaf_gather_val(ctx, G.perm + (int64_t)t0 * n, (int64_t)(t1 - t0) * n, kdn, K,
-                                      K - 1, boundT + (int64_t)t0 * n));
-    if (g > 0) DRPT_CHECK(fill_u32(ctx, (uint32_t*)pcnt, 0u, n));
+                                      K - 1, r.boundT + (int64_t)t0 * n));
+    if (g > 0) DRPT_CHECK(fill_u32(ctx, (uint32_t*)r.pcnt, 0u, n));
     DRPT_HIP(hipEventRecord(ctx->ev_pass[2 * g], s));
-    a.range = granges + 2 * g;
-    a.known = kin;
-    DRPT_CHECK(launch_lowp_knn(ctx, &pp, kModePool, a));
+    r.a.known = kin;
+    DRPT_CHECK(lowp_pass(ctx, &r, kModePool, plan, granges + 2 * g));
     DRPT_HIP(hipEventRecord(ctx->ev_pass[2 * g + 1], s));
     RowMergeArgs f{};
     f.X = X;
     f.n = n;
     f.d = p->d;
     f.K = K;
-    f.src = pool;
+    f.src = r.pool;
     f.src_tree_stride = 0;
-    f.S = pool_cap;
+    f.S = r.pool_cap;
     f.ntsrc = 1;
-    f.cnt_tree = pcnt;
+    f.cnt_tree = r.pcnt;
     f.known_idx = kin;
     f.known_dist = kdn;
-    f.skip_flag = oflag;
-    f.out_idx = last ? out_idx : (kin == ki ? ki2 : ki);
-    f.out_dist = last ? out_dist : (kdn == kd ? kd2 : kd);
+    f.skip_flag = r.oflag;
+    f.out_idx = last ? out_idx : (kin == r.ki ? ki2 : r.ki);
+    f.out_dist = last ? out_dist : (kdn == r.kd ? kd2 : r.kd);
     f.order = G.perm;
     DRPT_CHECK(launch_row_merge(ctx, p->dtype, f));
     kin = f.out_idx;
     kdn = f.out_dist;
   }
   int32_t nov = 0;
-  DRPT_HIP(hipMemcpyAsync(&nov, novf, 4, hipMemcpyDeviceToHost, s));
+  DRPT_HIP(hipMemcpyAsync(&nov, r.novf, 4, hipMemcpyDeviceToHost, s));
   DRPT_HIP(hipStreamSynchronize(s));
   if (nov > 0)
-    DRPT_CHECK(launch_row_exact(ctx, &pp, X, G.perm, G.items, G.range + 4, rows_per_item, olist,
-                                novf, nov, out_idx, out_dist));
+    DRPT_CHECK(launch_row_exact(ctx, &r.pp, X, G.perm, G.items, G.range + 4, r.olist, r.novf, nov,
+                                out_idx, out_dist));
   DRPT_HIP(hipEventRecord(ctx->ev[5], s));
   DRPT_CHECK(leaf_stats(ctx, p, G, timings));
   if (timings) {
@@ -737,15 +815,10 @@ int drpt_merge(drpt_ctx* ctx, int64_t n, int T, int K, const int32_t* cand_idx,
   return launch_merge(ctx, n, T, K, cand_idx, cand_dist, out_idx, out_dist);
 }
 
-int drpt_build_graph(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* out_idx,
-                     float* out_dist, drpt_timings* timings) {
-  if (!ctx || !X || !out_idx || !out_dist) {
-    set_error("build_graph: null pointer");
-    return DRPT_ERR_ARG;
-  }
-  DRPT_CHECK(validate(p));
-  DRPT_HIP(hipSetDevice(ctx->device));
-  const int T = p->ntrees, D = p->depth, K = p->nn, L = T * D;
+// projection and trees of a build: leaf heap index of every point, [T][n]
+// (events 0-2 bracket the two stages)
+static int build_front(drpt_ctx* ctx, const drpt_params* p, const void* X, uint32_t** heap_out) {
+  const int T = p->ntrees, D = p->depth, L = T * D;
   const int64_t n = p->n;
   hipStream_t s = ctx->stream;
   std::vector<float> R((size_t)p->d * L);
@@ -770,6 +843,23 @@ int drpt_build_graph(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t
   const int mode = p->local_mode == DRPT_LOCAL_REFERENCE ? DRPT_TREE_FIXED : DRPT_TREE_LEAFMAX;
   DRPT_CHECK(grow_trees_impl(ctx, P, colmax, n, T, D, mode, p->leaf_max, heap));
   DRPT_HIP(hipEventRecord(ctx->ev[2], s));
+  *heap_out = heap;
+  return DRPT_OK;
+}
+
+int drpt_build_graph(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* out_idx,
+                     float* out_dist, drpt_timings* timings) {
+  if (!ctx || !X || !out_idx || !out_dist) {
+    set_error("build_graph: null pointer");
+    return DRPT_ERR_ARG;
+  }
+  DRPT_CHECK(validate(p));
+  DRPT_HIP(hipSetDevice(ctx->device));
+  const int T = p->ntrees, K = p->nn;
+  const int64_t n = p->n;
+  hipStream_t s = ctx->stream;
+  uint32_t* heap;
+  DRPT_CHECK(build_front(ctx, p, X, &heap));
   drpt_timings tk{};
   if (p->precision == DRPT_FP32) {
     // per-tree lists (MDRPT::calculate_nns), then the merge (:180-198)
@@ -794,6 +884,93 @@ int drpt_build_graph(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t
     timings->merge_ms = ms_between(ctx->ev[3], ctx->ev[4]);
     timings->total_ms = ms_between(ctx->ev[0], ctx->ev[4]);
   }
+  return DRPT_OK;
+}
+
+int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* plan_out) {
+  if (!ctx || !X || !plan_out) {
+    set_error("autotune: null pointer");
+    return DRPT_ERR_ARG;
+  }
+  DRPT_CHECK(validate(p));
+  if (p->precision == DRPT_FP32) {
+    set_error("autotune: the FP32 path has a fixed work-item layout (64 rows)");
+    return DRPT_ERR_UNSUPPORTED;
+  }
+  DRPT_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  uint32_t* heap;
+  DRPT_CHECK(build_front(ctx, p, X, &heap));
+  // the inputs of the pooled filter pass of a build: known lists from the
+  // first trees, their K-th distances as every tree's bound
+  ItemPlan best = lowp_default_plan();
+  LowpRun r;
+  DRPT_CHECK(lowp_setup(ctx, p, X, heap, best, &r));
+  DRPT_CHECK(lowp_known(ctx, &r, X, best));
+  DRPT_CHECK(launch_leaf_gather_val(ctx, r.G.perm, (int64_t)r.T * r.n, r.kd, r.K, r.K - 1,
+                                    r.boundT));
+  r.a.known = r.ki;
+  const int rb = p->d * (p->precision == DRPT_FP8 ? 1 : 2);
+  const bool ring_ok = rb % 16 == 0 && rb <= 256;
+  hipEvent_t e0 = ctx->ev[10], e1 = ctx->ev[11];
+  // candidates per bucket: rows per item x Gram pipeline; each timed twice
+  // after a warm-up, the incumbent kept unless another is 3% faster
+  const int rpis[4] = {1 << 30, 256, 128, 64};
+  for (int b = 0; b < kSizeBuckets; ++b) {
+    double best_ms = -1.0;
+    for (int ir = 0; ir < 4; ++ir) {
+      for (int ring = 1; ring >= (ring_ok ? 0 : 1); --ring) {
+        ItemPlan cand = best;
+        cand.rpi[b] = rpis[ir];
+        cand.ring[b] = ring;
+        DRPT_CHECK(group_impl(ctx, p, heap, cand, r.first, &r.G, true));
+        r.a.items = r.G.items;
+        double ms = 0.0;
+        for (int rep = 0; rep < 3; ++rep) {
+          DRPT_CHECK(fill_u32(ctx, (uint32_t*)r.pcnt, 0u, r.n));
+          DRPT_CHECK(fill_u32(ctx, (uint32_t*)r.oflag, 0u, r.n));
+          DRPT_CHECK(fill_u32(ctx, (uint32_t*)r.novf, 0u, 1));
+          DRPT_HIP(hipEventRecord(e0, s));
+          DRPT_CHECK(lowp_pass(ctx, &r, kModePool, cand, r.G.range + 4, b));
+          DRPT_HIP(hipEventRecord(e1, s));
+          DRPT_HIP(hipEventSynchronize(e1));
+          if (rep > 0) ms += ms_between(e0, e1);
+        }
+        if (best_ms < 0.0 || ms < 0.97 * best_ms) {
+          best_ms = ms;
+          best = cand;
+        }
+      }
+    }
+  }
+  std::vector<int> v(2 * kSizeBuckets);
+  for (int b = 0; b < kSizeBuckets; ++b) {
+    v[b] = best.rpi[b];
+    v[kSizeBuckets + b] = best.ring[b];
+    plan_out[b] = best.rpi[b];
+    plan_out[kSizeBuckets + b] = best.ring[b];
+  }
+  ctx->plans[plan_key(p)] = v;
+  DRPT_HIP(hipStreamSynchronize(s));
+  return DRPT_OK;
+}
+
+int drpt_set_item_plan(drpt_ctx* ctx, const drpt_params* p, const int32_t* plan) {
+  if (!ctx || !p || !plan) {
+    set_error("set_item_plan: null pointer");
+    return DRPT_ERR_ARG;
+  }
+  DRPT_CHECK(validate(p));
+  std::vector<int> v(2 * kSizeBuckets);
+  for (int b = 0; b < kSizeBuckets; ++b) {
+    if (plan[b] < 1 || (plan[kSizeBuckets + b] != 0 && plan[kSizeBuckets + b] != 1)) {
+      set_error("set_item_plan: rows per item must be >= 1 and ring flags 0 or 1");
+      return DRPT_ERR_ARG;
+    }
+    v[b] = plan[b];
+    v[kSizeBuckets + b] = plan[kSizeBuckets + b];
+  }
+  ctx->plans[plan_key(p)] = v;
   return DRPT_OK;
 }
 

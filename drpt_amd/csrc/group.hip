@@ -134,7 +134,7 @@ __device__ __forceinline__ void block_scan2(long long& a, long long& b, long lon
 // perm[t] (slots in ascending order); tree totals of work items (blocks of
 // rpi rows) and leaves; sum of m^2 and max m over leaves.
 __global__ void __launch_bounds__(1024)
-k_leaf_scan(const uint32_t* __restrict__ hcount, int HS, int rpi, int64_t* __restrict__ hoff,
+k_leaf_scan(const uint32_t* __restrict__ hcount, int HS, ItemPlan plan, int64_t* __restrict__ hoff,
             int64_t* __restrict__ tree_items, unsigned long long* __restrict__ stats) {
   __shared__ long long sa[17], sb[17];
   const int t = blockIdx.x;
@@ -145,6 +145,7 @@ k_leaf_scan(const uint32_t* __restrict__ hcount, int HS, int rpi, int64_t* __res
   for (int base = 0; base < HS; base += blockDim.x) {
     const int h = base + threadIdx.x;
     const long long c = h < HS ? (long long)hc[h] : 0;
+    const int rpi = plan.rpi[size_bucket((int)min(c, (long long)INT_MAX))];
     long long a = c, b = (c + rpi - 1) / rpi, ta, tb;
     block_scan2(a, b, ta, tb, sa, sb);
     if (h < HS) ho[h] = cp + a;
@@ -189,10 +190,11 @@ __global__ void k_tree_range(const int64_t* __restrict__ base, int t0, int t1,
   }
 }
 
-// Per tree (one block): the work items (blocks of rpi rows) of every non-empty slot.
+// Per tree (one block): the work items (blocks of plan.rpi[bucket] rows) of
+// every non-empty slot.
 __global__ void __launch_bounds__(1024)
-k_emit_items(const uint32_t* __restrict__ hcount, const int64_t* __restrict__ hoff, int HS, int rpi,
-             const int64_t* __restrict__ base, LeafItem* __restrict__ items) {
+k_emit_items(const uint32_t* __restrict__ hcount, const int64_t* __restrict__ hoff, int HS,
+             ItemPlan plan, const int64_t* __restrict__ base, LeafItem* __restrict__ items) {
   __shared__ long long sa[17], sb[17];
   const int t = blockIdx.x;
   const uint32_t* hc = hcount + (int64_t)t * HS;
@@ -201,6 +203,7 @@ k_emit_items(const uint32_t* __restrict__ hcount, const int64_t* __restrict__ ho
   for (int b0 = 0; b0 < HS; b0 += blockDim.x) {
     const int h = b0 + threadIdx.x;
     const long long c = h < HS ? (long long)hc[h] : 0;
+    const int rpi = plan.rpi[size_bucket((int)min(c, (long long)INT_MAX))];
     long long a = (c + rpi - 1) / rpi, z = 0, ta, tz;
     block_scan2(a, z, ta, tz, sa, sb);
     if (c > 0) {
@@ -208,9 +211,9 @@ k_emit_items(const uint32_t* __restrict__ hcount, const int64_t* __restrict__ ho
       it.tree = t;
       it.size = (int32_t)c;
       it.start = ho[h];
-      it.pad = 0;
       for (long long r0 = 0, k = 0; r0 < c; r0 += rpi, ++k) {
         it.row0 = (int32_t)r0;
+        it.rows = (int32_t)min((long long)rpi, c - r0);
         items[ci + a + k] = it;
       }
     }
@@ -274,22 +277,24 @@ int launch_apply_local(drpt_ctx* ctx, const int32_t* perm_in, int32_t* perm_out,
 
 namespace drpt {
 
-int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, int rows_per_item,
+int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, const ItemPlan& plan,
                         int first_trees, int64_t* hoff, int64_t* tree_items, int64_t* base,
                         int64_t* range, LeafItem* items, unsigned long long* stats) {
   const int HS = 1 << D;
-  if (first_trees < 0 || first_trees > T || rows_per_item < 1) {
+  bool ok = first_trees >= 0 && first_trees <= T;
+  for (int b = 0; b < kSizeBuckets; ++b) ok = ok && plan.rpi[b] >= 1;
+  if (!ok) {
     set_error("internal: invalid work item layout");
     return DRPT_ERR_ARG;
   }
-  hipLaunchKernelGGL(k_leaf_scan, dim3(T), dim3(1024), 0, ctx->stream, hcount, HS, rows_per_item,
-                     hoff, tree_items, stats);
+  hipLaunchKernelGGL(k_leaf_scan, dim3(T), dim3(1024), 0, ctx->stream, hcount, HS, plan, hoff,
+                     tree_items, stats);
   DRPT_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_tree_bases, dim3(1), dim3(64), 0, ctx->stream, tree_items, T, first_trees,
                      base, range);
   DRPT_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_emit_items, dim3(T), dim3(1024), 0, ctx->stream, hcount, hoff, HS,
-                     rows_per_item, base, items);
+  hipLaunchKernelGGL(k_emit_items, dim3(T), dim3(1024), 0, ctx->stream, hcount, hoff, HS, plan,
+                     base, items);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }

@@ -49,6 +49,8 @@ struct drpt_ctx {
   };
   std::map<std::string, Graph> graphs;
   bool use_graphs = true;  // environment DRPT_GRAPHS=0: launch eagerly
+  // work-item plans found by drpt_autotune, by workload shape (ItemPlan)
+  std::map<std::string, std::vector<int>> plans;
 };
 
 namespace drpt {
@@ -94,17 +96,45 @@ struct LeafItem {   // one work item: a block of rows of one leaf (or the whole 
   int32_t size;     // leaf size m
   int64_t start;    // offset of the leaf in perm[tree]
   int32_t row0;     // first row of this block within the leaf
-  int32_t pad;
+  int32_t rows;     // rows of the block (row0 + rows <= m)
 };
+
+// Work-item layout by leaf size: leaves of bucket b (sizes <= 64, <= 256,
+// <= 1024, larger) are cut into items of rpi[b] rows, and the low-precision
+// Gram passes over them use the LDS tile ring (ring[b] = 1, where the row
+// length allows it) or the register double buffer (0). Chosen per bucket by
+// drpt_autotune; the default is whole leaves and the ring.
+constexpr int kSizeBuckets = 4;
+struct ItemPlan {
+  int rpi[kSizeBuckets];
+  int ring[kSizeBuckets];
+};
+__host__ __device__ inline int size_bucket(int m) {
+  return m <= 64 ? 0 : m <= 256 ? 1 : m <= 1024 ? 2 : 3;
+}
+// [lo, hi] leaf sizes of bucket b
+inline void bucket_sizes(int b, int* lo, int* hi) {
+  static const int lim[kSizeBuckets + 1] = {0, 64, 256, 1024, 1 << 30};
+  *lo = lim[b] + 1;
+  *hi = lim[b + 1];
+}
+inline ItemPlan uniform_plan(int rpi) {
+  ItemPlan p;
+  for (int b = 0; b < kSizeBuckets; ++b) {
+    p.rpi[b] = rpi;
+    p.ring[b] = 1;
+  }
+  return p;
+}
 
 int launch_count_leaves(drpt_ctx* ctx, const uint32_t* heap, int64_t n, int T, int D, uint32_t* hcount);
 int launch_scatter_leaves(drpt_ctx* ctx, const uint32_t* heap, int64_t n, int T, int D,
                           const int64_t* hoff, uint32_t* cursor, int32_t* perm);
 // device-side grouping for leaf-max builds: heap offsets, per-tree item
 // counts, item ranges (trees < first_trees / the others / all) and the work
-// items (blocks of rows_per_item rows); stats[0] += sum m^2, stats[1] = max m,
-// stats[2] += leaves
-int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, int rows_per_item,
+// items (blocks of plan.rpi[bucket] rows); stats[0] += sum m^2, stats[1] = max
+// m, stats[2] += leaves
+int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, const ItemPlan& plan,
                         int first_trees, int64_t* hoff, int64_t* tree_items, int64_t* base,
                         int64_t* range, LeafItem* items, unsigned long long* stats);
 // range[0] = base[t0], range[1] = base[t1]: the work items of trees [t0, t1)
@@ -144,8 +174,9 @@ struct KnnArgs {
   const int64_t* range;  // items [range[0], range[1])
   int64_t n;
   int d, K;
-  int rows_per_item;  // rows of a leaf one item covers (from row0)
-  unsigned* queue;    // ticket counters (set by launch_lowp_knn)
+  int size_lo, size_hi;  // items of leaves with sizes outside [lo, hi] are skipped
+  int ring;              // 1: LDS tile ring where the row length allows it; 0: registers
+  unsigned* queue;       // ticket counters (set by launch_lowp_knn)
   // kModeTree: per-tree lists [T][n][K]; kModePool: pool ids [n][pool_cap]
   int32_t* out_idx;
   // kModeCand: keys approx_bits << 32 | id, [tree - tree0][n][k']
@@ -195,9 +226,8 @@ struct RowMergeArgs {
 int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a);
 // rows listed in ovf_list: exact K best over every leafmate in every tree
 int launch_row_exact(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
-                     const LeafItem* items, const int64_t* range, int rows_per_item,
-                     const int32_t* ovf_list, const int32_t* novf, int64_t nrows_max,
-                     int32_t* out_idx, float* out_dist);
+                     const LeafItem* items, const int64_t* range, const int32_t* ovf_list,
+                     const int32_t* novf, int64_t nrows_max, int32_t* out_idx, float* out_dist);
 // leaf-ordered gather of a per-point value: outT[i] = src[perm[i] * stride + offset]
 int launch_leaf_gather_val(drpt_ctx* ctx, const int32_t* perm, int64_t total, const float* src,
                            int stride, int offset, float* outT);

@@ -250,7 +250,7 @@ k_knn_exact(const XT* __restrict__ X, int d, const int32_t* __restrict__ perm, i
     const LeafItem it = items[item];
     const int32_t* pl = perm + (int64_t)it.tree * n + it.start;
     const int m = it.size;
-    const int rows = min(kRows, m - it.row0);
+    const int rows = min(kRows, it.rows);
     if (tid < kRows) rowid[tid] = tid < rows ? pl[it.row0 + tid] : -1;
 #pragma unroll
     for (int rr = 0; rr < kRowsPerWave; ++rr) {
@@ -803,7 +803,12 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
     const int64_t lofs = (int64_t)it.tree * n + it.start;  // leaf offset in the per-tree arrays
     const int32_t* pl = A.perm + lofs;
     const int m = it.size;
-    const int row_end = (int)min((int64_t)m, (int64_t)it.row0 + A.rows_per_item);
+    const int row_end = it.row0 + it.rows;
+    if (m < A.size_lo || m > A.size_hi) {  // another launch's size bucket
+      __syncthreads();
+      item = begin + S.ticket[parity];
+      continue;
+    }
     if (m > kLeafStage) {  // too large to stage
       if constexpr (MODE == kModeTree) {  // exact path for these rows
         for (int r = it.row0 + tid; r < row_end; r += kThreads) {
@@ -1531,11 +1536,11 @@ __global__ void k_invperm(const int32_t* __restrict__ perm, int64_t n, int T,
 
 // rowitem[tree * n + position] = the item covering that leaf position
 __global__ void k_rowitem(const LeafItem* __restrict__ items, const int64_t* __restrict__ range,
-                          int rows_per_item, int64_t n, int32_t* __restrict__ rowitem) {
+                          int64_t n, int32_t* __restrict__ rowitem) {
   const int64_t b = range[0], e = range[1];
   for (int64_t i = b + blockIdx.x; i < e; i += gridDim.x) {
     const LeafItem it = items[i];
-    const int r1 = (int)min((int64_t)it.size, (int64_t)it.row0 + rows_per_item);
+    const int r1 = it.row0 + it.rows;
     for (int r = it.row0 + threadIdx.x; r < r1; r += blockDim.x)
       rowitem[(int64_t)it.tree * n + it.start + r] = (int32_t)i;
   }
@@ -1710,7 +1715,8 @@ template <typename XT, typename EL, int T, int MODE>
 int launch_lowp_t(drpt_ctx* ctx, const KnnArgs& a) {
   if constexpr (MODE != kModeTree) {
     const int rb = a.d * EL::kBytes;
-    if ((rb & 15) == 0 && rb <= kRingRow) return launch_lowp_r<XT, EL, T, MODE, true>(ctx, a);
+    if (a.ring && (rb & 15) == 0 && rb <= kRingRow)
+      return launch_lowp_r<XT, EL, T, MODE, true>(ctx, a);
   }
   return launch_lowp_r<XT, EL, T, MODE, false>(ctx, a);
 }
@@ -1877,9 +1883,8 @@ int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a) {
 }
 
 int launch_row_exact(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
-                     const LeafItem* items, const int64_t* range, int rows_per_item,
-                     const int32_t* ovf_list, const int32_t* novf, int64_t nrows_max,
-                     int32_t* out_idx, float* out_dist) {
+                     const LeafItem* items, const int64_t* range, const int32_t* ovf_list,
+                     const int32_t* novf, int64_t nrows_max, int32_t* out_idx, float* out_dist) {
   const int64_t n = p->n;
   const int T = p->ntrees, K = p->nn;
   if (K > 32) {
@@ -1893,7 +1898,7 @@ int launch_row_exact(drpt_ctx* ctx, const drpt_params* p, const void* X, const i
   hipLaunchKernelGGL(k_invperm, dim3(g), dim3(256), 0, ctx->stream, perm, n, T, inv);
   DRPT_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_rowitem, dim3((unsigned)ctx->num_cus * 8), dim3(256), 0, ctx->stream, items,
-                     range, rows_per_item, n, rowitem);
+                     range, n, rowitem);
   DRPT_HIP(hipGetLastError());
   int32_t* ti;
   float* td;

@@ -278,3 +278,37 @@ def test_duplicates_and_tiny_leaves(ctx, orc):
                                    density=p.density)
     assert np.array_equal(oi.cpu().numpy(), ri)
     assert np.array_equal(_bits(od.cpu().numpy()), _bits(rd))
+
+
+@pytest.mark.parametrize("prec,case,leaf_max,T", [("bf16", "int128_f32_4000", 700, 5),
+                                                  ("fp8", "gauss_d37_1500", 100, 4),
+                                                  ("bf16", "mnist_u8_3000", 300, 4)])
+def test_autotune_and_item_plans(ctx, prec, case, leaf_max, T):
+    """drpt_autotune returns a valid per-bucket plan; builds with the tuned plan
+    and with forced plans (leaves cut into 64-row items, register pipeline,
+    mixed pipelines across buckets: one launch per bucket) equal the default
+    build bit for bit."""
+    _log(f"[autotune] {prec} {case} leaf_max={leaf_max}")
+    X = CASES[case]()
+    n, d = X.shape
+    p = drpt_amd.leafmax_params(n, d, T, 10, leaf_max=leaf_max, precision=prec, seed=6)
+    p.dtype = drpt_amd.DRPT_U8 if X.dtype == np.uint8 else drpt_amd.DRPT_F32
+    Xd = _dev(X)
+    ref = drpt_amd.Context(0)
+    with ref:
+        ri, rd, _ = ref.build_graph(p, Xd)
+        ri, rd = ri.cpu().numpy(), rd.cpu().numpy()
+    with drpt_amd.Context(0) as c:
+        plan = c.autotune(p, Xd)
+        _log(f"  plan {plan}")
+        assert all(r in (64, 128, 256, 1 << 30) for r in plan["rows_per_item"])
+        assert all(g in (0, 1) for g in plan["ring"])
+        forced = [plan,
+                  {"rows_per_item": [64] * 4, "ring": [0] * 4},
+                  {"rows_per_item": [64, 128, 256, 64], "ring": [1, 0, 1, 0]},
+                  {"rows_per_item": [1 << 30] * 4, "ring": [0, 1, 0, 1]}]
+        for pl in forced:
+            c.set_item_plan(p, pl)
+            oi, od, _ = c.build_graph(p, Xd)
+            assert np.array_equal(oi.cpu().numpy(), ri), f"plan {pl}"
+            assert np.array_equal(_bits(od.cpu().numpy()), _bits(rd)), f"plan {pl}"
