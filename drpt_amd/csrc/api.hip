@@ -469,17 +469,18 @@ static int prep_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X, const i
                      KnnArgs* a) {
   const int64_t n = p->n;
   const bool fp8 = p->precision == DRPT_FP8;
+  const std::string tag = fp8 ? "lowp8." : "lowp.";  // both may be live in one build
   uint8_t* xq;
   float *nb, *er, *lnbT, *lerrT;
   int8_t *pexp = nullptr, *lexpT = nullptr;
-  DRPT_CHECK(ws(ctx, "lowp.x", (size_t)n * p->d * (fp8 ? 1 : 2), &xq));
-  DRPT_CHECK(ws(ctx, "lowp.nrm2", n, &nb));
-  DRPT_CHECK(ws(ctx, "lowp.err", n, &er));
-  if (fp8) DRPT_CHECK(ws(ctx, "lowp.pexp", n, &pexp));
+  DRPT_CHECK(ws(ctx, (tag + "x").c_str(), (size_t)n * p->d * (fp8 ? 1 : 2), &xq));
+  DRPT_CHECK(ws(ctx, (tag + "nrm2").c_str(), n, &nb));
+  DRPT_CHECK(ws(ctx, (tag + "err").c_str(), n, &er));
+  if (fp8) DRPT_CHECK(ws(ctx, (tag + "pexp").c_str(), n, &pexp));
   DRPT_CHECK(launch_prep_lowp(ctx, X, p->dtype, n, p->d, p->precision, xq, nb, er, pexp));
-  DRPT_CHECK(ws(ctx, "lowp.lnbT", (size_t)p->ntrees * n, &lnbT));
-  DRPT_CHECK(ws(ctx, "lowp.lerrT", (size_t)p->ntrees * n, &lerrT));
-  if (fp8) DRPT_CHECK(ws(ctx, "lowp.lexpT", (size_t)p->ntrees * n, &lexpT));
+  DRPT_CHECK(ws(ctx, (tag + "lnbT").c_str(), (size_t)p->ntrees * n, &lnbT));
+  DRPT_CHECK(ws(ctx, (tag + "lerrT").c_str(), (size_t)p->ntrees * n, &lerrT));
+  if (fp8) DRPT_CHECK(ws(ctx, (tag + "lexpT").c_str(), (size_t)p->ntrees * n, &lexpT));
   DRPT_CHECK(launch_leaf_gather(ctx, perm, (int64_t)p->ntrees * n, nb, er, pexp, lnbT, lerrT,
                                 lexpT));
   a->X = X;
@@ -599,10 +600,14 @@ static int tree_group_auto(int T, int K) {
 }
 
 // The low-precision pipeline's state: grouping, quantised inputs, buffers.
+// The candidate pass runs at the build's precision (pc, ac); the pooled
+// filter passes that certify the lists run in BF16 (pp, a): an FP8 error
+// radius (about 3% of |x| per row) admits so many leafmates on clustered data
+// that most rows would leave their pools.
 struct LowpRun {
-  drpt_params pp;
+  drpt_params pp, pc;
   Grouping G;
-  KnnArgs a{};
+  KnnArgs a{}, ac{};
   int T = 0, K = 0, KP = 0, first = 0, pool_cap = 0;
   int64_t n = 0;
   uint64_t* s1 = nullptr;
@@ -638,6 +643,8 @@ static int lowp_setup(drpt_ctx* ctx, const drpt_params* p, const void* X,
                       const uint32_t* leaf_heap, const ItemPlan& plan, LowpRun* r) {
   r->pp = *p;
   r->pp.kprime = kprime_of(p);
+  r->pc = r->pp;
+  r->pp.precision = DRPT_BF16;
   r->T = p->ntrees;
   r->K = p->nn;
   r->n = p->n;
@@ -651,6 +658,12 @@ static int lowp_setup(drpt_ctx* ctx, const drpt_params* p, const void* X,
   const int64_t n = r->n;
   DRPT_CHECK(group_impl(ctx, p, leaf_heap, plan, r->first, &r->G));
   DRPT_CHECK(prep_lowp(ctx, &r->pp, X, r->G.perm, &r->a));
+  if (r->pc.precision != r->pp.precision) {
+    DRPT_CHECK(prep_lowp(ctx, &r->pc, X, r->G.perm, &r->ac));
+  } else {
+    r->ac = r->a;
+  }
+  r->ac.items = r->G.items;
   DRPT_CHECK(ws(ctx, "knn.s1", (size_t)r->first * n * r->KP, &r->s1));
   DRPT_CHECK(ws(ctx, "knn.known_idx", (size_t)n * r->K, &r->ki));
   DRPT_CHECK(ws(ctx, "knn.known_dist", (size_t)n * r->K, &r->kd));
@@ -681,7 +694,9 @@ static int lowp_setup(drpt_ctx* ctx, const drpt_params* p, const void* X,
 // (items of other buckets skipped); bucket >= 0 restricts it to that bucket
 static int lowp_pass(drpt_ctx* ctx, LowpRun* r, int mode, const ItemPlan& plan,
                      const int64_t* range, int bucket = -1) {
-  KnnArgs a = r->a;
+  KnnArgs a = mode == kModeCand ? r->ac : r->a;
+  const drpt_params* pr = mode == kModeCand ? &r->pc : &r->pp;
+  a.items = r->G.items;
   a.range = range;
   bool uniform = true;
   for (int b = 1; b < kSizeBuckets; ++b) uniform = uniform && plan.ring[b] == plan.ring[0];
@@ -689,21 +704,21 @@ static int lowp_pass(drpt_ctx* ctx, LowpRun* r, int mode, const ItemPlan& plan,
     a.size_lo = 0;
     a.size_hi = INT_MAX;
     a.ring = plan.ring[0];
-    return launch_lowp_knn(ctx, &r->pp, mode, a);
+    return launch_lowp_knn(ctx, pr, mode, a);
   }
   for (int b = 0; b < kSizeBuckets; ++b) {
     if (bucket >= 0 && b != bucket) continue;
     bucket_sizes(b, &a.size_lo, &a.size_hi);
     a.ring = plan.ring[b];
-    DRPT_CHECK(launch_lowp_knn(ctx, &r->pp, mode, a));
+    DRPT_CHECK(launch_lowp_knn(ctx, pr, mode, a));
   }
   return DRPT_OK;
 }
 
 // 1. widened candidates of the first trees; 2. known lists (ki, kd) and bounds
 static int lowp_known(drpt_ctx* ctx, LowpRun* r, const void* X, const ItemPlan& plan) {
-  r->a.out_key = r->s1;
-  r->a.tree0 = 0;
+  r->ac.out_key = r->s1;
+  r->ac.tree0 = 0;
   DRPT_CHECK(lowp_pass(ctx, r, kModeCand, plan, r->G.range + 0));
   RowMergeArgs m{};
   m.X = X;
@@ -910,7 +925,7 @@ int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* p
   DRPT_CHECK(launch_leaf_gather_val(ctx, r.G.perm, (int64_t)r.T * r.n, r.kd, r.K, r.K - 1,
                                     r.boundT));
   r.a.known = r.ki;
-  const int rb = p->d * (p->precision == DRPT_FP8 ? 1 : 2);
+  const int rb = p->d * 2;  // the pooled passes run in BF16
   const bool ring_ok = rb % 16 == 0 && rb <= 256;
   hipEvent_t e0 = ctx->ev[10], e1 = ctx->ev[11];
   // candidates per bucket: rows per item x Gram pipeline; each timed twice
@@ -924,7 +939,6 @@ int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* p
         cand.rpi[b] = rpis[ir];
         cand.ring[b] = ring;
         DRPT_CHECK(group_impl(ctx, p, heap, cand, r.first, &r.G, true));
-        r.a.items = r.G.items;
         double ms = 0.0;
         for (int rep = 0; rep < 3; ++rep) {
           DRPT_CHECK(fill_u32(ctx, (uint32_t*)r.pcnt, 0u, r.n));

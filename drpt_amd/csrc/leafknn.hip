@@ -489,7 +489,9 @@ struct LowpShared<false> {
   };
   DRPT_LOWP_COMMON
   int known[kRows * kKnownLds];   // pool mode: the block's known ids
-  int flush_flag;                 // pool mode: some list may fill up in the next tile
+  // pool mode: some list is half full after a tile of that parity (a wave may
+  // set the next tile's flag while a slower one still reads this one)
+  int flush_flag[2];
 };
 #undef DRPT_LOWP_COMMON
 
@@ -523,8 +525,8 @@ __device__ __forceinline__ void load_query(SH& S, const uint8_t* __restrict__ Xq
 }
 
 // One pass of Gram tiles of the query block [row0, row0 + 64) against every
-// leafmate tile; epi(c0, acc) sees each thread's 16 accumulators; hook() runs
-// on every thread between tiles, after a barrier. With the
+// leafmate tile; epi(c0, acc) sees each thread's 16 accumulators; hook(t) runs
+// on every thread after tile t (and a barrier) when another tile follows. With the
 // query operand in registers (rows of at most kChunkBytes) the leafmate tiles
 // are double-buffered: one barrier per tile, the next tile's loads in flight
 // during this tile's MFMAs and epilogue. rb = bytes per row, d = elements.
@@ -578,7 +580,7 @@ __device__ __forceinline__ void gram_pass(SH& S, const uint8_t* __restrict__ Xq,
         }
       }
       __syncthreads();
-      if (more) hook();
+      if (more) hook(c0 / kCols);
     }
     return;
   }
@@ -613,7 +615,7 @@ __device__ __forceinline__ void gram_pass(SH& S, const uint8_t* __restrict__ Xq,
     epi(c0, acc);
     if (c0 + kCols < m) {
       __syncthreads();
-      hook();
+      hook(c0 / kCols);
     }
   }
   __syncthreads();
@@ -696,7 +698,7 @@ __device__ __forceinline__ void gram_pass_ring(LowpShared<false>& S,
     if (t + 1 < ntiles) wait_vmcnt<4>(); else wait_vmcnt<0>();
     ring_barrier();  // tile t visible; every wave is done with tile t - 1's stage
     if (t + 2 < ntiles) issue(t + 2);
-    if (t > 0) hook();  // touches neither the ring nor the operand registers
+    if (t > 0) hook(t - 1);  // touches neither the ring nor the operand registers
     const unsigned char* st = S.ring[t % kRing];
     floatx16 acc;
 #pragma unroll
@@ -792,7 +794,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
   unsigned* myq = A.queue + qx;
   if (tid == 0) S.ticket[0] = (int)atomicAdd(myq, 1u);
   if constexpr (MODE == kModePool) {
-    if (tid == 0) S.flush_flag = 0;
+    if (tid == 0) S.flush_flag[0] = S.flush_flag[1] = 0;
   }
   __syncthreads();
   int64_t item = begin + S.ticket[0];
@@ -920,7 +922,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
               }
             }
           }
-        }, [] {});
+        }, [](int) {});
       }
       if constexpr (MODE == kModeCand) {
         if (qv) {
@@ -1001,8 +1003,10 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
       // row's known ids from the list of the thread that owned (r, quarter) in
       // the tiles, the 4 lanes of a row reserve pool slots with one atomic.
       // Runs at the end of the pass, and between tiles once some list is half
-      // full (flag set in the epilogue, read after a barrier, so every thread
-      // takes the same branch).
+      // full: the epilogue of tile t sets flag[t & 1], the hook after tile t
+      // reads and clears it. Every thread reads it after the barrier that
+      // follows tile t, and the next write to it (tile t + 2) comes after a
+      // barrier that follows the read, so every thread takes the same branch.
       auto flush = [&]() {
         if constexpr (MODE == kModePool) {
           S.ccnt[q * 4 + sub] = nsub;  // the owner's count, visible after the barrier
@@ -1050,11 +1054,12 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
           nsub = 0;
         }
       };
-      auto hook = [&]() {
+      auto hook = [&](int tprev) {
         if constexpr (MODE == kModePool) {
-          if (S.flush_flag) {
+          const int f = tprev & 1;
+          if (S.flush_flag[f]) {
             flush();
-            if (tid == 0) S.flush_flag = 0;  // no epilogue runs before the barrier
+            if (tid == 0) S.flush_flag[f] = 0;
             __syncthreads();
           }
         }
@@ -1094,11 +1099,12 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
           // flushed at half capacity: a tile adds at most 16 entries per
           // thread but rarely more than a few; a list that still overflows
           // sends its row to the exact row path
-          if (nsub > kSubCap / 2) S.flush_flag = 1;
+          if (nsub > kSubCap / 2) S.flush_flag[(c0 / kCols) & 1] = 1;
         }
       }, hook);
       if constexpr (MODE == kModePool) {
         flush();
+        if (tid == 0) S.flush_flag[0] = S.flush_flag[1] = 0;  // the last tile's, unread
         __syncthreads();  // lists and tiles reused by the next block
         continue;
       }

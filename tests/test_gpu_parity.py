@@ -312,3 +312,24 @@ def test_autotune_and_item_plans(ctx, prec, case, leaf_max, T):
             oi, od, _ = c.build_graph(p, Xd)
             assert np.array_equal(oi.cpu().numpy(), ri), f"plan {pl}"
             assert np.array_equal(_bits(od.cpu().numpy()), _bits(rd)), f"plan {pl}"
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
+def test_lowp_k32_dense_clusters_equal_fp32(ctx, prec):
+    """nn = 32 on dense clusters (about 4000 points per centre, as the 10M x 96
+    workload): many survivors per row and tile, so per-thread lists are flushed
+    between tiles (512-slot pools, known ids left to the row merge, several
+    tree groups); the lists still equal the FP32 build's."""
+    _log(f"[k32] {prec} dense clusters")
+    n = 200_000
+    X = datagen.sphere_mixture(n, centres=50)
+    Xd = _dev(X)
+    out = {}
+    for pr in ("fp32", prec):
+        p = drpt_amd.leafmax_params(n, 96, 8, 32, leaf_max=1024, precision=pr,
+                                    kprime=64 if pr != "fp32" else 0, seed=2)
+        oi, od, tm = ctx.build_graph(p, Xd)
+        out[pr] = (oi.cpu().numpy(), od.cpu().numpy(), tm)
+    _log(f"  overflow rows {out[prec][2].n_fallback_rows}")
+    assert np.array_equal(out["fp32"][0], out[prec][0])
+    assert np.array_equal(_bits(out["fp32"][1]), _bits(out[prec][1]))
