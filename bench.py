@@ -10,7 +10,7 @@ FP64 brute force on sampled queries, the roofline of the leaf k-NN kernel and
 the oracle (CPU restatement, OpenMP) on a bounded sample of the same workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c3|c4]
-                    [--precision bf16|fp8|fp32] [--no-cpu] [--no-fp32] [--no-autotune]
+                    [--precision bf16|fp8|fp32] [--no-cpu] [--no-fp32] [--no-autotune | --item-plan JSON]
 """
 import argparse
 import json
@@ -148,6 +148,10 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip drpt_autotune (work-item layout per leaf-size bucket) before warm-up")
+    ap.add_argument("--item-plan", default=None, metavar="JSON",
+                    help='use this plan instead of tuning, e.g. \'{"rows_per_item": [1073741824, '
+                         '1073741824, 64, 64], "ring": [1, 1, 1, 1], "tree_group": 6}\' (as '
+                         'autotune reports it)')
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scale for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -196,7 +200,10 @@ def main():
         return el / steps * 1e3, tms
 
     plan = None
-    if prec != "fp32" and not a.no_autotune:
+    if prec != "fp32" and a.item_plan:
+        plan = json.loads(a.item_plan)
+        ctx.set_item_plan(p, plan)
+    elif prec != "fp32" and not a.no_autotune:
         t = time.perf_counter()
         plan = ctx.autotune(p, Xd)  # one-time set-up, kept by the context
         log(f"[bench] autotune {plan} in {time.perf_counter() - t:.2f} s")

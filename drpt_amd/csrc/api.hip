@@ -634,6 +634,7 @@ static ItemPlan lowp_plan_of(drpt_ctx* ctx, const drpt_params* p) {
     plan.rpi[b] = it->second[b];
     plan.ring[b] = it->second[kSizeBuckets + b];
   }
+  plan.tree_group = it->second[2 * kSizeBuckets];
   return plan;
 }
 
@@ -752,7 +753,9 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   // 3. pools over groups of trees, each merged into the known lists before
   // the next group (duplicates across the group's trees removed, the bound
   // tightened); the last merge writes the output
-  const int gsz = p->tree_group > 0 ? std::min(p->tree_group, T) : tree_group_auto(T, K);
+  const int gsz = p->tree_group > 0   ? std::min(p->tree_group, T)
+                  : plan.tree_group > 0 ? std::min(plan.tree_group, T)
+                                        : tree_group_auto(T, K);
   const int ngroups = (T + gsz - 1) / gsz;
   int64_t* granges;
   int32_t* ki2 = nullptr;
@@ -957,14 +960,50 @@ int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* p
       }
     }
   }
-  std::vector<int> v(2 * kSizeBuckets);
-  for (int b = 0; b < kSizeBuckets; ++b) {
-    v[b] = best.rpi[b];
-    v[kSizeBuckets + b] = best.ring[b];
-    plan_out[b] = best.rpi[b];
-    plan_out[kSizeBuckets + b] = best.ring[b];
+  // trees per pooled filter pass: whole low-precision stages timed per group
+  // size (smaller groups: fewer rows leave their pools for the exact path,
+  // more merges); unless the caller fixed it
+  auto store = [&]() {
+    std::vector<int> v(kPlanInts);
+    for (int b = 0; b < kSizeBuckets; ++b) {
+      v[b] = best.rpi[b];
+      v[kSizeBuckets + b] = best.ring[b];
+    }
+    v[2 * kSizeBuckets] = best.tree_group;
+    ctx->plans[plan_key(p)] = v;
+    for (int i = 0; i < kPlanInts; ++i) plan_out[i] = v[i];
+  };
+  if (p->tree_group == 0) {
+    int32_t* oi;
+    float* od;
+    DRPT_CHECK(ws(ctx, "tune.out_idx", (size_t)p->n * p->nn, &oi));
+    DRPT_CHECK(ws(ctx, "tune.out_dist", (size_t)p->n * p->nn, &od));
+    const int T = p->ntrees;
+    const int cands[7] = {T, 8, 6, 5, 4, 3, 2};
+    double best_ms = -1.0;
+    int best_g = 0;
+    std::vector<int> tried;
+    for (int g : cands) {
+      if (g < 1 || g > T || std::find(tried.begin(), tried.end(), g) != tried.end()) continue;
+      tried.push_back(g);
+      best.tree_group = g;
+      store();
+      double ms = 0.0;
+      for (int rep = 0; rep < 2; ++rep) {  // the first run sizes the workspace
+        DRPT_HIP(hipEventRecord(e0, s));
+        DRPT_CHECK(graph_knn_lowp(ctx, p, X, heap, oi, od, nullptr));
+        DRPT_HIP(hipEventRecord(e1, s));
+        DRPT_HIP(hipEventSynchronize(e1));
+        if (rep > 0) ms = ms_between(e0, e1);
+      }
+      if (best_ms < 0.0 || ms < 0.97 * best_ms) {
+        best_ms = ms;
+        best_g = g;
+      }
+    }
+    best.tree_group = best_g;
   }
-  ctx->plans[plan_key(p)] = v;
+  store();
   DRPT_HIP(hipStreamSynchronize(s));
   return DRPT_OK;
 }
@@ -975,7 +1014,7 @@ int drpt_set_item_plan(drpt_ctx* ctx, const drpt_params* p, const int32_t* plan)
     return DRPT_ERR_ARG;
   }
   DRPT_CHECK(validate(p));
-  std::vector<int> v(2 * kSizeBuckets);
+  std::vector<int> v(kPlanInts);
   for (int b = 0; b < kSizeBuckets; ++b) {
     if (plan[b] < 1 || (plan[kSizeBuckets + b] != 0 && plan[kSizeBuckets + b] != 1)) {
       set_error("set_item_plan: rows per item must be >= 1 and ring flags 0 or 1");
@@ -984,6 +1023,11 @@ int drpt_set_item_plan(drpt_ctx* ctx, const drpt_params* p, const int32_t* plan)
     v[b] = plan[b];
     v[kSizeBuckets + b] = plan[kSizeBuckets + b];
   }
+  if (plan[2 * kSizeBuckets] < 0) {
+    set_error("set_item_plan: tree_group must be >= 0");
+    return DRPT_ERR_ARG;
+  }
+  v[2 * kSizeBuckets] = plan[2 * kSizeBuckets];
   ctx->plans[plan_key(p)] = v;
   return DRPT_OK;
 }

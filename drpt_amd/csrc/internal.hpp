@@ -102,12 +102,15 @@ struct LeafItem {   // one work item: a block of rows of one leaf (or the whole 
 // Work-item layout by leaf size: leaves of bucket b (sizes <= 64, <= 256,
 // <= 1024, larger) are cut into items of rpi[b] rows, and the low-precision
 // Gram passes over them use the LDS tile ring (ring[b] = 1, where the row
-// length allows it) or the register double buffer (0). Chosen per bucket by
-// drpt_autotune; the default is whole leaves and the ring.
+// length allows it) or the register double buffer (0); tree_group = trees per
+// pooled filter pass (0: the rule of drpt_params.tree_group = 0). Chosen by
+// drpt_autotune; the default is whole leaves, the ring and the rule.
 constexpr int kSizeBuckets = 4;
+constexpr int kPlanInts = 2 * kSizeBuckets + 1;  // rpi, ring, tree_group
 struct ItemPlan {
   int rpi[kSizeBuckets];
   int ring[kSizeBuckets];
+  int tree_group;
 };
 __host__ __device__ inline int size_bucket(int m) {
   return m <= 64 ? 0 : m <= 256 ? 1 : m <= 1024 ? 2 : 3;
@@ -124,6 +127,7 @@ inline ItemPlan uniform_plan(int rpi) {
     p.rpi[b] = rpi;
     p.ring[b] = 1;
   }
+  p.tree_group = 0;
   return p;
 }
 

@@ -284,10 +284,10 @@ def test_duplicates_and_tiny_leaves(ctx, orc):
                                                   ("fp8", "gauss_d37_1500", 100, 4),
                                                   ("bf16", "mnist_u8_3000", 300, 4)])
 def test_autotune_and_item_plans(ctx, prec, case, leaf_max, T):
-    """drpt_autotune returns a valid per-bucket plan; builds with the tuned plan
-    and with forced plans (leaves cut into 64-row items, register pipeline,
-    mixed pipelines across buckets: one launch per bucket) equal the default
-    build bit for bit."""
+    """drpt_autotune returns a valid plan; builds with the tuned plan and with
+    forced plans (leaves cut into 64-row items, register pipeline, mixed
+    pipelines across buckets: one launch per bucket; 2 / 3 trees per pooled
+    pass) equal the default build bit for bit."""
     _log(f"[autotune] {prec} {case} leaf_max={leaf_max}")
     X = CASES[case]()
     n, d = X.shape
@@ -303,10 +303,11 @@ def test_autotune_and_item_plans(ctx, prec, case, leaf_max, T):
         _log(f"  plan {plan}")
         assert all(r in (64, 128, 256, 1 << 30) for r in plan["rows_per_item"])
         assert all(g in (0, 1) for g in plan["ring"])
+        assert 1 <= plan["tree_group"] <= T
         forced = [plan,
-                  {"rows_per_item": [64] * 4, "ring": [0] * 4},
-                  {"rows_per_item": [64, 128, 256, 64], "ring": [1, 0, 1, 0]},
-                  {"rows_per_item": [1 << 30] * 4, "ring": [0, 1, 0, 1]}]
+                  {"rows_per_item": [64] * 4, "ring": [0] * 4, "tree_group": 2},
+                  {"rows_per_item": [64, 128, 256, 64], "ring": [1, 0, 1, 0], "tree_group": 3},
+                  {"rows_per_item": [1 << 30] * 4, "ring": [0, 1, 0, 1], "tree_group": 0}]
         for pl in forced:
             c.set_item_plan(p, pl)
             oi, od, _ = c.build_graph(p, Xd)
