@@ -75,6 +75,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: run `python -m drpt_amd.build` first")
+    # torch ships its own HIP / HSA runtime libraries. Loaded after them, this
+    # library shares torch's runtime; loaded before them, its HIP calls found no
+    # device once torch had initialised the GPU. So torch goes first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, i32, i64, u32, f32, dbl = (ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_uint32,
                                    ctypes.c_float, ctypes.c_double)

@@ -8,8 +8,10 @@
 // (:247-251). One process drives one GPU; the build runs through the C ABI.
 //
 // Options beyond the reference: -seed (the reference draws its seeds from
-// std::random_device), -precision fp32|bf16, -kprime, -leaf-max (split nodes
-// larger than this instead of the reference local stage), -depth, -device.
+// std::random_device), -precision fp32|bf16|fp8, -kprime, -leaf-max (split
+// nodes larger than this instead of the reference local stage), -depth,
+// -device, -tree-group (trees per pooled filter pass of a bf16/fp8 build),
+// -autotune 1 (tune the work-item plan on this input before the build).
 // -locality is accepted and has no effect on one GPU.
 #include <hip/hip_runtime_api.h>
 
@@ -50,7 +52,7 @@ int main(int argc, char* argv[]) {
   double tree_depth_ratio = 0.6, density = 0;
   uint32_t seed = 1;
   std::string precision = "fp32";
-  int kprime = 0, leaf_max = 0, depth_override = 0, device = 0;
+  int kprime = 0, leaf_max = 0, depth_override = 0, device = 0, tree_group = 0, autotune = 0;
   for (int p = 0; p + 1 < argc; p++) {
     const char* a = argv[p];
     const char* v = argv[p + 1];
@@ -73,6 +75,8 @@ int main(int argc, char* argv[]) {
     else if (!std::strcmp(a, "-leaf-max")) leaf_max = std::atoi(v);
     else if (!std::strcmp(a, "-depth")) depth_override = std::atoi(v);
     else if (!std::strcmp(a, "-device")) device = std::atoi(v);
+    else if (!std::strcmp(a, "-tree-group")) tree_group = std::atoi(v);
+    else if (!std::strcmp(a, "-autotune")) autotune = std::atoi(v);
   }
   if (input_path.empty()) die("Valid input path needed!...");
   if (output_path.empty()) die("Valid out path needed!...");
@@ -91,6 +95,7 @@ int main(int argc, char* argv[]) {
   p.density = (float)density;
   p.seed = seed;
   p.kprime = kprime;
+  p.tree_group = tree_group;
   if (precision == "fp32") p.precision = DRPT_FP32;
   else if (precision == "bf16") p.precision = DRPT_BF16;
   else if (precision == "fp8") p.precision = DRPT_FP8;
@@ -132,6 +137,12 @@ int main(int argc, char* argv[]) {
                 hipMemcpyHostToDevice) != hipSuccess)
     die("upload failed");
   drpt_timings tm;
+  if (autotune && p.precision != DRPT_FP32) {
+    int32_t plan[9];
+    check(drpt_autotune(ctx, &p, dX, plan), "drpt_autotune");
+    std::printf("Work-item plan: rows per item %d %d %d %d, ring %d %d %d %d, trees per group %d\n",
+                plan[0], plan[1], plan[2], plan[3], plan[4], plan[5], plan[6], plan[7], plan[8]);
+  }
   check(drpt_build_graph(ctx, &p, dX, dI, dD, &tm), "drpt_build_graph");
   std::vector<int32_t> idx((size_t)data_set_size * nn);
   if (hipMemcpy(idx.data(), dI, idx.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)

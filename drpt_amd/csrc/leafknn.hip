@@ -462,7 +462,7 @@ struct LowpShared;
   int lid[kLeafStage];              /* point id of every leaf position */           \
   float lnb[kLeafStage];            /* |x~|^2 of every leaf position */             \
   int8_t lexp[kLeafStage];          /* fp8: scale exponent of every leaf position */ \
-  uint16_t cpos[kRows][kCap];       /* candidate leaf positions per query row */    \
+  uint16_t cpos[kRows][kCap + 2];   /* candidate leaf positions per query row */    \
   int ccnt[kRows * 4];              /* per row (tree) or per (row, quarter) (pool) */ \
   float cacut[kRows];                                                               \
   float red_n[kThreads / 64], red_e[kThreads / 64];                                 \
@@ -479,7 +479,10 @@ struct LowpShared<true> {
   int cpre[kRows + 1];
 };
 
-constexpr int kKnownLds = 16;  // pool mode: known ids per row staged in LDS (larger K: global)
+constexpr int kKnownLds = 16;  // pool mode: known ids per row staged in LDS (larger K: left to the row merge)
+// row pitches in LDS one bank apart (33 / 17 dwords), so the rows a wave
+// appends to or scans in the same step hit different banks
+constexpr int kKnownPitch = kKnownLds + 1;
 
 template <>
 struct LowpShared<false> {
@@ -488,7 +491,7 @@ struct LowpShared<false> {
     unsigned char ring[kRing][kCols * kRingRow];
   };
   DRPT_LOWP_COMMON
-  int known[kRows * kKnownLds];   // pool mode: the block's known ids
+  int known[kRows * kKnownPitch];  // pool mode: the block's known ids
   // pool mode: some list is half full after a tile of that parity (a wave may
   // set the next tile's flag while a slower one still reads this one)
   int flush_flag[2];
@@ -991,7 +994,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
           if (K <= kKnownLds)  // read at the flush, after the Gram pass
             for (int j = tid; j < rows * K; j += kThreads) {
               const int r = j / K;
-              S.known[r * kKnownLds + (j - r * K)] = A.known[(int64_t)S.lid[row0 + r] * K + (j - r * K)];
+              S.known[r * kKnownPitch + (j - r * K)] = A.known[(int64_t)S.lid[row0 + r] * K + (j - r * K)];
             }
         }
       }
@@ -1012,7 +1015,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
           S.ccnt[q * 4 + sub] = nsub;  // the owner's count, visible after the barrier
           __syncthreads();
           const int r = tid >> 2, qs = tid & 3;
-          const int* kn = &S.known[r * kKnownLds];
+          const int* kn = &S.known[r * kKnownPitch];
           const int cnt = r < rows ? S.ccnt[r * 4 + qs] : 0;
           uint16_t* lst = &S.cpos[r][qs * kSubCap];
           int keep = 0;

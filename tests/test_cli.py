@@ -68,3 +68,27 @@ def test_cli_end_to_end_matches_oracle(tmp_path, orc):
     ri, _, _, _ = orc.build_graph(X, 4, D, 10, local_mode=0, seed=21)
     got = open(out + "results.txt").read()
     assert got == expected_edges(ri, 9)
+
+
+@pytest.mark.gpu
+def test_cli_fbin_bf16_autotune_matches_oracle(tmp_path, orc):
+    """fbin input, leaf-max trees, BF16 candidates with the autotuner and a fixed
+    tree group: the edge list equals the oracle's (the same lists as FP32)."""
+    from drpt_amd import datagen
+    X = datagen.int_mixture(4000)
+    path = tmp_path / "int.fbin"
+    datagen.write_fbin(str(path), X)
+    out = str(tmp_path) + "/"
+    print("[cli] fbin bf16 autotune", flush=True)
+    r = _run(["-input", str(path), "-output", out, "-data-set-size", "4000", "-dimension", "128",
+              "-ntrees", "6", "-nn", "10", "-seed", "5", "-data-file-format", "1",
+              "-bytes-for-data-type", "4", "-data-starting-index", "8", "-precision", "bf16",
+              "-leaf-max", "300", "-tree-group", "3", "-autotune", "1"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Work-item plan" in r.stdout
+    import drpt_amd
+    p = drpt_amd.leafmax_params(4000, 128, 6, 10, leaf_max=300, seed=5)  # the driver's depth rule
+    ri, _, _, _ = orc.build_graph(X, 6, p.depth, 10, local_mode=1, leaf_max=300, seed=5,
+                                  density=p.density)
+    got = open(out + "results.txt").read()
+    assert got == expected_edges(ri, 9)
