@@ -452,31 +452,33 @@ __device__ __forceinline__ float acut_of(float B, float nb_i, float e_i, double 
 constexpr int kRing = 3;          // leafmate tile stages in flight (ring path)
 constexpr int kRingRow = 256;     // bytes per leafmate row in a ring stage (d <= 128)
 
-// LDS of the low-precision kernel. Tree mode: register-staged double buffer
-// plus the candidates' exact distances. Candidate / pool modes: the tile ring
-// (also holds the double buffer for inputs the ring does not cover).
-template <bool kTree>
+// LDS of the low-precision kernel, for NR query rows per block (4 threads per
+// row). Tree mode (NR = 64): register-staged double buffer plus the
+// candidates' exact distances. Candidate / pool modes (NR = 128): the tile
+// ring, or for inputs the ring does not cover three 64-row tile buffers
+// (leafmate double buffer; query rows staged in buffers 1-2).
+template <bool kTree, int NR>
 struct LowpShared;
 
-#define DRPT_LOWP_COMMON                                                              \
+#define DRPT_LOWP_COMMON(NR)                                                          \
   int lid[kLeafStage];              /* point id of every leaf position */           \
   float lnb[kLeafStage];            /* |x~|^2 of every leaf position */             \
   int8_t lexp[kLeafStage];          /* fp8: scale exponent of every leaf position */ \
-  uint16_t cpos[kRows][kCap + 2];   /* candidate leaf positions per query row */    \
-  int ccnt[kRows * 4];              /* per row (tree) or per (row, quarter) (pool) */ \
-  float cacut[kRows];                                                               \
-  float red_n[kThreads / 64], red_e[kThreads / 64];                                 \
+  uint16_t cpos[NR][kCap + 2];      /* candidate leaf positions per query row */    \
+  int ccnt[NR * 4];                 /* per row (tree) or per (row, quarter) (pool) */ \
+  float cacut[NR];                                                                  \
+  float red_n[NR / 16], red_e[NR / 16]; /* one per wave */                          \
   int ticket[2];
 
-template <>
-struct LowpShared<true> {
+template <int NR>
+struct LowpShared<true, NR> {
   // leafmate tiles (row bytes), double-buffered; for rows longer than
   // kChunkBytes buffer 1 holds the query rows' chunk and buffer 0 the
   // leafmates' (single-buffered)
   unsigned char Bs[2][kCols][kTilePitchB];
-  DRPT_LOWP_COMMON
-  float cdist[kRows][kCap];         // exact distances of the candidates
-  int cpre[kRows + 1];
+  DRPT_LOWP_COMMON(NR)
+  float cdist[NR][kCap];            // exact distances of the candidates
+  int cpre[NR + 1];
 };
 
 constexpr int kKnownLds = 16;  // pool mode: known ids per row staged in LDS (larger K: left to the row merge)
@@ -484,69 +486,84 @@ constexpr int kKnownLds = 16;  // pool mode: known ids per row staged in LDS (la
 // appends to or scans in the same step hit different banks
 constexpr int kKnownPitch = kKnownLds + 1;
 
-template <>
-struct LowpShared<false> {
+template <int NR>
+struct LowpShared<false, NR> {
   union {
-    unsigned char Bs[2][kCols][kTilePitchB];
+    unsigned char Bs[1 + NR / kCols][kCols][kTilePitchB];
     unsigned char ring[kRing][kCols * kRingRow];
   };
-  DRPT_LOWP_COMMON
-  int known[kRows * kKnownPitch];  // pool mode: the block's known ids
+  DRPT_LOWP_COMMON(NR)
+  int known[NR * kKnownPitch];      // pool mode: the block's known ids
   // pool mode: some list is half full after a tile of that parity (a wave may
   // set the next tile's flag while a slower one still reads this one)
   int flush_flag[2];
 };
 #undef DRPT_LOWP_COMMON
 
+// Wave roles for NR query rows: wave w computes the 32 x 32 block of query
+// rows qh .. qh + 31 and tile columns ch .. ch + 31 of each 64-column tile.
+template <int NR>
+struct WaveRole {
+  static constexpr int kGroups = NR / 32;  // query groups of 32 rows
+  int qh, ch;
+  __device__ explicit WaveRole(int wave) : qh((wave % kGroups) * 32), ch((wave / kGroups) * 32) {}
+};
+
 // Query operand of the block (rows of at most kChunkBytes): lane l of wave w
-// holds, for every k-step, its operand of query row (w & 1) * 32 + l % 32,
-// i.e. the MFMA's B operand, for all passes (32 VGPRs for either element type).
+// holds, for every k-step, its operand of query row qh + l % 32, i.e. the
+// MFMA's B operand, for all passes (32 VGPRs for either element type).
 template <typename EL>
 struct QueryOp {
   typename EL::op_t v[kChunkBytes / (2 * EL::kOpBytes)];
 };
 
-template <typename EL, typename SH>
+// query rows staged in buffers 1.. (buffer 0 takes the first leafmate tile
+// while other waves may still read their query operands)
+template <typename EL, int NR, typename SH>
 __device__ __forceinline__ void load_query(SH& S, const uint8_t* __restrict__ Xq, int rb,
                                            int row0, int rows, QueryOp<EL>& qo) {
+  constexpr int NT = 4 * NR;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   unsigned char(*As)[kTilePitchB] = S.Bs[1];
 #pragma unroll
-  for (int s4 = 0; s4 < 4; ++s4) {
-    const int c = tid + kThreads * s4;
+  for (int s4 = 0; s4 < NR * 16 / NT; ++s4) {
+    const int c = tid + NT * s4;
     const int r = c >> 4, part = c & 15;
     *reinterpret_cast<uint4*>(&As[r][part * 16]) =
         load16(Xq, r < rows ? S.lid[row0 + r] : -1, rb, part * 16);
   }
   __syncthreads();
-  const int qh = (wave & 1) * 32;
+  const WaveRole<NR> role(wave);
 #pragma unroll
   for (int ks = 0; ks < kChunkBytes / (2 * EL::kOpBytes); ++ks)
-    qo.v[ks] = EL::lds(&As[qh + (lane & 31)][op_offset<EL>(ks, lane)]);
-  // buffer 1 is next written after gram_pass's first barrier
+    qo.v[ks] = EL::lds(&As[role.qh + (lane & 31)][op_offset<EL>(ks, lane)]);
+  // buffers 1.. are next written after gram_pass's first barrier
 }
 
-// One pass of Gram tiles of the query block [row0, row0 + 64) against every
+// One pass of Gram tiles of the query block [row0, row0 + NR) against every
 // leafmate tile; epi(c0, acc) sees each thread's 16 accumulators; hook(t) runs
 // on every thread after tile t (and a barrier) when another tile follows. With the
 // query operand in registers (rows of at most kChunkBytes) the leafmate tiles
 // are double-buffered: one barrier per tile, the next tile's loads in flight
 // during this tile's MFMAs and epilogue. rb = bytes per row, d = elements.
-template <typename EL, typename SH, typename Epi, typename Hook>
+template <typename EL, int NR, typename SH, typename Epi, typename Hook>
 __device__ __forceinline__ void gram_pass(SH& S, const uint8_t* __restrict__ Xq, int d, int rb,
                                           int m, int row0, int rows, const QueryOp<EL>& qo,
                                           Epi&& epi, Hook&& hook) {
+  constexpr int NT = 4 * NR;
+  constexpr int kTileLoads = kCols * 16 / NT;  // 16-byte pieces of a tile per thread
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int qh = (wave & 1) * 32, ch = (wave >> 1) * 32;
+  const WaveRole<NR> role(wave);
+  const int qh = role.qh, ch = role.ch;
   constexpr int kStepBytes = 2 * EL::kOpBytes;  // bytes of a row per k-step
   if (rb <= kChunkBytes) {
     const int ksteps = (d + 15) / 16;
-    uint4 rbv[4];
+    uint4 rbv[kTileLoads];
 #pragma unroll
-    for (int s4 = 0; s4 < 4; ++s4) {
-      const int c = tid + kThreads * s4;
+    for (int s4 = 0; s4 < kTileLoads; ++s4) {
+      const int c = tid + NT * s4;
       const int r = c >> 4, part = c & 15;
       *reinterpret_cast<uint4*>(&S.Bs[0][r][part * 16]) =
           load16(Xq, r < m ? S.lid[r] : -1, rb, part * 16);
@@ -557,8 +574,8 @@ __device__ __forceinline__ void gram_pass(SH& S, const uint8_t* __restrict__ Xq,
       const bool more = c0 + kCols < m;
       if (more) {  // next leafmate tile in flight during this one
 #pragma unroll
-        for (int s4 = 0; s4 < 4; ++s4) {
-          const int c = tid + kThreads * s4;
+        for (int s4 = 0; s4 < kTileLoads; ++s4) {
+          const int c = tid + NT * s4;
           const int r = c >> 4, part = c & 15;
           const int col = c0 + kCols + r;
           rbv[s4] = load16(Xq, col < m ? S.lid[col] : -1, rb, part * 16);
@@ -577,8 +594,8 @@ __device__ __forceinline__ void gram_pass(SH& S, const uint8_t* __restrict__ Xq,
       epi(c0, acc);
       if (more) {
 #pragma unroll
-        for (int s4 = 0; s4 < 4; ++s4) {
-          const int c = tid + kThreads * s4;
+        for (int s4 = 0; s4 < kTileLoads; ++s4) {
+          const int c = tid + NT * s4;
           *reinterpret_cast<uint4*>(&S.Bs[b ^ 1][c >> 4][(c & 15) * 16]) = rbv[s4];
         }
       }
@@ -587,7 +604,7 @@ __device__ __forceinline__ void gram_pass(SH& S, const uint8_t* __restrict__ Xq,
     }
     return;
   }
-  // longer rows: chunks of kChunkBytes, query chunk in buffer 1, leafmates in buffer 0
+  // longer rows: chunks of kChunkBytes, query chunk in buffers 1.., leafmates in buffer 0
   unsigned char(*As)[kTilePitchB] = S.Bs[1];
   unsigned char(*Bs)[kTilePitchB] = S.Bs[0];
   for (int c0 = 0; c0 < m; c0 += kCols) {
@@ -596,15 +613,33 @@ __device__ __forceinline__ void gram_pass(SH& S, const uint8_t* __restrict__ Xq,
     for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
     for (int o0 = 0; o0 < rb; o0 += kChunkBytes) {
       __syncthreads();
+      if constexpr (NR == kCols) {  // query and leafmate pieces loaded together
 #pragma unroll
-      for (int s4 = 0; s4 < 4; ++s4) {
-        const int c = tid + kThreads * s4;
-        const int r = c >> 4, part = c & 15;
-        const int col = c0 + r;
-        const uint4 va = load16(Xq, r < rows ? S.lid[row0 + r] : -1, rb, o0 + part * 16);
-        const uint4 vb = load16(Xq, col < m ? S.lid[col] : -1, rb, o0 + part * 16);
-        *reinterpret_cast<uint4*>(&As[r][part * 16]) = va;
-        *reinterpret_cast<uint4*>(&Bs[r][part * 16]) = vb;
+        for (int s4 = 0; s4 < kTileLoads; ++s4) {
+          const int c = tid + NT * s4;
+          const int r = c >> 4, part = c & 15;
+          const int col = c0 + r;
+          const uint4 va = load16(Xq, r < rows ? S.lid[row0 + r] : -1, rb, o0 + part * 16);
+          const uint4 vb = load16(Xq, col < m ? S.lid[col] : -1, rb, o0 + part * 16);
+          *reinterpret_cast<uint4*>(&As[r][part * 16]) = va;
+          *reinterpret_cast<uint4*>(&Bs[r][part * 16]) = vb;
+        }
+      } else {
+#pragma unroll
+        for (int s4 = 0; s4 < NR * 16 / NT; ++s4) {
+          const int c = tid + NT * s4;
+          const int r = c >> 4, part = c & 15;
+          *reinterpret_cast<uint4*>(&As[r][part * 16]) =
+              load16(Xq, r < rows ? S.lid[row0 + r] : -1, rb, o0 + part * 16);
+        }
+#pragma unroll
+        for (int s4 = 0; s4 < kTileLoads; ++s4) {
+          const int c = tid + NT * s4;
+          const int r = c >> 4, part = c & 15;
+          const int col = c0 + r;
+          *reinterpret_cast<uint4*>(&Bs[r][part * 16]) =
+              load16(Xq, col < m ? S.lid[col] : -1, rb, o0 + part * 16);
+        }
       }
       __syncthreads();
       // k-steps covering the chunk's elements (zeros past the row end)
@@ -653,30 +688,32 @@ __device__ __forceinline__ void ring_barrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// Ring path of the Gram pass (d % 8 == 0, d <= 128): leafmate tiles go
-// straight from HBM into LDS (global_load_lds_dwordx4, 16 B per lane, lane l
-// of an instruction writes 16 l bytes after its wave-uniform base), kRing
-// stages, two tiles in flight during each tile's MFMAs. Stage layout: row r
-// of the tile at r * 256 bytes, its 16-byte chunk c at physical chunk
-// c ^ (r & 15) (bank spread for the MFMA operand reads); chunks past the
-// row's bytes stay zero. Every wave issues exactly 4 load instructions per tile
-// (each covers 4 whole rows, at least one valid chunk per row), so
-// vmcnt <= 4 means "all but the newest tile".
-template <typename EL, typename Epi, typename Hook>
-__device__ __forceinline__ void gram_pass_ring(LowpShared<false>& S,
+// Ring path of the Gram pass (rows of 16-byte multiples up to 256 bytes):
+// leafmate tiles go straight from HBM into LDS (global_load_lds_dwordx4, 16 B
+// per lane, lane l of an instruction writes 16 l bytes after its wave-uniform
+// base), kRing stages, two tiles in flight during each tile's MFMAs. Stage
+// layout: row r of the tile at r * 256 bytes, its 16-byte chunk c at physical
+// chunk c ^ (r & 15) (bank spread for the MFMA operand reads); chunks past the
+// row's bytes stay zero. Every wave issues exactly 16 / (NR / 16) load
+// instructions per tile (each covers 4 whole rows, at least one valid chunk
+// per row), so vmcnt <= that count means "all but the newest tile".
+template <typename EL, int NR, typename Epi, typename Hook>
+__device__ __forceinline__ void gram_pass_ring(LowpShared<false, NR>& S,
                                                const uint8_t* __restrict__ Xq, int d, int rb,
                                                int m, const QueryOp<EL>& qo, Epi&& epi,
                                                Hook&& hook) {
+  constexpr int NT = 4 * NR, NW = NT / 64;
+  constexpr int kIssues = 16 / NW;  // load instructions per wave per tile
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int ch = (wave >> 1) * 32;
+  const int ch = WaveRole<NR>(wave).ch;
   const int L = rb >> 4;           // 16-byte chunks per row
   const int ksteps = (d + 15) / 16;
   const int ntiles = (m + kCols - 1) / kCols;
   constexpr int kSteps = kChunkBytes / (2 * EL::kOpBytes);
   __syncthreads();  // query staging reads done (they share the ring's bytes)
   if (L < 16) {     // zero the chunks no load writes
-    for (int j = tid; j < kRing * kCols * 16; j += kThreads) {
+    for (int j = tid; j < kRing * kCols * 16; j += NT) {
       const int st = j / (kCols * 16), slot = j % (kCols * 16);
       const int r = slot >> 4, c = (slot & 15) ^ (r & 15);
       if (c >= L) *reinterpret_cast<uint4*>(&S.ring[st][slot * 16]) = make_uint4(0, 0, 0, 0);
@@ -685,8 +722,8 @@ __device__ __forceinline__ void gram_pass_ring(LowpShared<false>& S,
   auto issue = [&](int j) {
     unsigned char* base = S.ring[j % kRing];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int blk = i * 4 + wave;  // 64 consecutive slots = rows 4 blk .. 4 blk + 3
+    for (int i = 0; i < kIssues; ++i) {
+      const int blk = i * NW + wave;  // 64 consecutive slots = rows 4 blk .. 4 blk + 3
       const int slot = blk * 64 + lane;
       const int r = slot >> 4, c = (slot & 15) ^ (r & 15);
       const int col = min(j * kCols + r, m - 1);  // rows past the leaf: any valid row
@@ -698,7 +735,7 @@ __device__ __forceinline__ void gram_pass_ring(LowpShared<false>& S,
   issue(0);
   if (ntiles > 1) issue(1);
   for (int t = 0; t < ntiles; ++t) {
-    if (t + 1 < ntiles) wait_vmcnt<4>(); else wait_vmcnt<0>();
+    if (t + 1 < ntiles) wait_vmcnt<kIssues>(); else wait_vmcnt<0>();
     ring_barrier();  // tile t visible; every wave is done with tile t - 1's stage
     if (t + 2 < ntiles) issue(t + 2);
     if (t > 0) hook(t - 1);  // touches neither the ring nor the operand registers
@@ -773,11 +810,23 @@ constexpr int kSubCap = 16;  // pool mode: list slots per thread (4 threads per 
 //   kModePool  filter pass against an external bound B (leaf-ordered), skipping
 //              the row's known ids; survivors are appended to the row's pool
 
+// Block shape: 4 threads per query row, 64 rows (256 threads, two blocks per
+// CU). The passes also take 128 rows (512 threads, one block per CU: each
+// staged leafmate tile feeds twice the MFMA work); measured on the 1M x 128,
+// 1M x 960 and 10M x 96 workloads that was 0-8% slower, as one block's tile
+// barriers then stall every wave of the CU instead of half of them.
+template <int MODE>
+constexpr int lowp_rows() { return 64; }
+
 template <typename XT, typename EL, int T, int MODE, bool RING>
-__global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
+__global__ void __launch_bounds__(4 * lowp_rows<MODE>(), lowp_rows<MODE>() == 64 ? 2 : 1)
+    k_knn_lowp(const KnnArgs A) {
+  constexpr int NR = lowp_rows<MODE>();
+  constexpr int NT = 4 * NR, NW = NT / 64;
+  static_assert(MODE != kModeTree || NR == kRows, "tree mode keeps the 64-row layout");
   constexpr int KP = 4 * T;  // widened candidates per row in the bound pass
   constexpr bool kScaled = EL::kBytes == 1;  // fp8: per-row power-of-two scales
-  __shared__ __align__(16) LowpShared<MODE == kModeTree> S;
+  __shared__ __align__(16) LowpShared<MODE == kModeTree, NR> S;
   const XT* __restrict__ X = static_cast<const XT*>(A.X);
   const uint8_t* __restrict__ Xq = static_cast<const uint8_t*>(A.Xq);
   const int d = A.d, K = A.K;
@@ -787,9 +836,10 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
   const int wave = tid >> 6, lane = tid & 63;
   const bool a_resident = rb <= kChunkBytes;
   // this thread's query row within the block and quarter of the columns
-  const int q = (wave & 1) * 32 + (lane & 31);
-  const int sub = (wave >> 1) * 2 + (lane >> 5);
-  const int cq = (wave >> 1) * 32 + 4 * (lane >> 5);  // + 8 * (e / 4) + e % 4
+  const WaveRole<NR> role(wave);
+  const int q = role.qh + (lane & 31);
+  const int sub = (role.ch / 32) * 2 + (lane >> 5);
+  const int cq = role.ch + 4 * (lane >> 5);  // + 8 * (e / 4) + e % 4
   const int qx = blockIdx.x % kQueues;
   const int64_t chunk = (A.range[1] - A.range[0] + kQueues - 1) / kQueues;
   const int64_t begin = A.range[0] + qx * chunk;
@@ -816,7 +866,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
     }
     if (m > kLeafStage) {  // too large to stage
       if constexpr (MODE == kModeTree) {  // exact path for these rows
-        for (int r = it.row0 + tid; r < row_end; r += kThreads) {
+        for (int r = it.row0 + tid; r < row_end; r += NT) {
           const int slot = atomicAdd(A.nfallback, 1);
           if (slot < A.fb_cap) {
             A.fallback[2 * slot] = (int32_t)item;
@@ -824,11 +874,11 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
           }
         }
       } else if constexpr (MODE == kModeCand) {  // no widened candidates
-        for (int r = it.row0 + tid; r < row_end; r += kThreads)
+        for (int r = it.row0 + tid; r < row_end; r += NT)
           for (int s = 0; s < KP; ++s)
             A.out_key[((int64_t)(it.tree - A.tree0) * n + pl[r]) * KP + s] = ~0ull;
       } else {  // every row redone exactly over all trees
-        for (int r = it.row0 + tid; r < row_end; r += kThreads) {
+        for (int r = it.row0 + tid; r < row_end; r += NT) {
           const int id = pl[r];
           if (atomicExch(&A.ovf_flag[id], 1) == 0) A.ovf_list[atomicAdd(A.novf, 1)] = id;
         }
@@ -839,7 +889,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
     }
     // stage ids and norms of the leaf (leaf-ordered arrays: coalesced); leaf maxima
     float mn = 0.0f, me_ = 0.0f;
-    for (int j = tid; j < m; j += kThreads) {
+    for (int j = tid; j < m; j += NT) {
       const float v = A.lnbT[lofs + j];
       S.lid[j] = pl[j];
       S.lnb[j] = v;
@@ -859,19 +909,19 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
     __syncthreads();
     float nmax = S.red_n[0], emax = S.red_e[0];
 #pragma unroll
-    for (int w = 1; w < kThreads / 64; ++w) {
+    for (int w = 1; w < NW; ++w) {
       nmax = fmaxf(nmax, S.red_n[w]);
       emax = fmaxf(emax, S.red_e[w]);
     }
     const double nm = sqrt((double)nmax * (1.0 + (d + 4.0) * 5.9604644775390625e-08));
 
-    for (int row0 = it.row0; row0 < row_end; row0 += kRows) {
-      const int rows = min(kRows, row_end - row0);
+    for (int row0 = it.row0; row0 < row_end; row0 += NR) {
+      const int rows = min(NR, row_end - row0);
       const bool qv = q < rows;
       const int qpos = row0 + q;
       const float rn = qv ? S.lnb[qpos] : 0.0f;
       QueryOp<EL> qo;
-      if (a_resident) load_query<EL>(S, Xq, rb, row0, rows, qo);
+      if (a_resident) load_query<EL, NR>(S, Xq, rb, row0, rows, qo);
       // fp8: the tile's Gram values scaled back by 2^(e_query + e_leafmate)
       // (exact: powers of two), so the epilogues see x~_i . x~_j as for bf16
       const int eq = kScaled && qv ? (int)S.lexp[qpos] : 0;
@@ -894,8 +944,8 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
       // register-staged tile load is pending in its loop
       auto gram = [&](auto&& epi, auto&& hook) {
         auto e2 = scaled(epi);
-        if constexpr (RING) gram_pass_ring<EL>(S, Xq, d, rb, m, qo, e2, hook);
-        else gram_pass<EL>(S, Xq, d, rb, m, row0, rows, qo, e2, hook);
+        if constexpr (RING) gram_pass_ring<EL, NR>(S, Xq, d, rb, m, qo, e2, hook);
+        else gram_pass<EL, NR>(S, Xq, d, rb, m, row0, rows, qo, e2, hook);
       };
       uint32_t key[T];
 #pragma unroll
@@ -947,8 +997,8 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
         }
         __syncthreads();
         const int npair = rows * KP;
-        for (int p = tid; p < npair; p += 2 * kThreads) {
-          const int p1 = p + kThreads;
+        for (int p = tid; p < npair; p += 2 * NT) {
+          const int p1 = p + NT;
           const int r0_ = p / KP, s0 = p - r0_ * KP;
           const uint16_t pos0 = S.cpos[r0_][s0];
           const bool two = p1 < npair;
@@ -992,7 +1042,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
         }
         if constexpr (MODE == kModePool) {
           if (K <= kKnownLds)  // read at the flush, after the Gram pass
-            for (int j = tid; j < rows * K; j += kThreads) {
+            for (int j = tid; j < rows * K; j += NT) {
               const int r = j / K;
               S.known[r * kKnownPitch + (j - r * K)] = A.known[(int64_t)S.lid[row0 + r] * K + (j - r * K)];
             }
@@ -1126,11 +1176,11 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
         }
         __syncthreads();
         const int total = S.cpre[rows];
-        for (int p = tid; p < total; p += 2 * kThreads) {
+        for (int p = tid; p < total; p += 2 * NT) {
           int r_[2], s_[2];
   #pragma unroll
           for (int h = 0; h < 2; ++h) {
-            const int pp = min(p + h * kThreads, total - 1);
+            const int pp = min(p + h * NT, total - 1);
             int lo = 0, hi = rows;  // largest r with cpre[r] <= pp
             while (hi - lo > 1) {
               const int mid = (lo + hi) >> 1;
@@ -1149,7 +1199,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_knn_lowp(const KnnArgs A) {
             D1 = exact_dist(X, a1, b1, d);
           }
           S.cdist[r_[0]][s_[0]] = D0;
-          if (p + kThreads < total) S.cdist[r_[1]][s_[1]] = D1;
+          if (p + NT < total) S.cdist[r_[1]][s_[1]] = D1;
         }
         __syncthreads();
         // the K best of each row's list, or the exact fallback on overflow
@@ -1453,11 +1503,35 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
 }
 
 // Exact K best distinct leafmates over every tree, for rows whose pool
-// overflowed. k_row_exact_tree: one block per (row, tree) pair, the row's
-// exact KP best leafmates in that tree (per-thread lists, pairwise merged in
-// LDS); k_row_exact_merge: per row, the T sorted lists merged without
-// duplicates.
+// overflowed or whose leaf is too large to stage. k_row_exact_tree: one block
+// per (row, tree) pair, the row's exact KP best leafmates in that tree (each
+// wave sorts its share by (D, id), one wave merges the waves' best);
+// k_row_exact_merge: per row, the T sorted lists merged without duplicates.
 constexpr int kFbThreadsRow = 256;
+// one wave: exact keys (D bits << 32 | id, non-negative D orders like its
+// bits, ties by id) of leafmates c = e * kFbThreadsRow + tid of pl[0, m),
+// sorted over the wave; key[0] of lanes 0 .. 63 then holds the wave's 64
+// smallest in order
+template <typename XT, int E>
+__device__ __forceinline__ void row_exact_wave(const XT* __restrict__ X, int d, const XT* xm, int me,
+                                               const int32_t* __restrict__ pl, int m, int tid,
+                                               int lane, uint64_t& k0) {
+  uint64_t key[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int c = e * kFbThreadsRow + tid;
+    key[e] = ~0ull;
+    if (c < m) {
+      const int id = pl[c];
+      if (id != me)
+        key[e] = ((uint64_t)__float_as_uint(exact_dist8(xm, X + (int64_t)id * d, d)) << 32) |
+                 (uint32_t)id;
+    }
+  }
+  bitonic_wave<uint64_t, E>(key, lane);
+  k0 = key[0];
+}
+
 template <typename XT, int KP>
 __global__ void __launch_bounds__(kFbThreadsRow)
 k_row_exact_tree(const XT* __restrict__ X, int d, const int32_t* __restrict__ perm, int64_t n,
@@ -1465,9 +1539,11 @@ k_row_exact_tree(const XT* __restrict__ X, int d, const int32_t* __restrict__ pe
                  const LeafItem* __restrict__ items, const int32_t* __restrict__ rows,
                  const int32_t* __restrict__ nrows, int64_t nrows_max,
                  int32_t* __restrict__ tmp_i, float* __restrict__ tmp_d) {
-  __shared__ float ld[kFbThreadsRow][KP];
-  __shared__ int li[kFbThreadsRow][KP];
-  const int tid = threadIdx.x;
+  constexpr int NW = kFbThreadsRow / 64;
+  constexpr int kChunk = 8 * kFbThreadsRow;  // leafmates sorted per round
+  static_assert((NW + 1) * KP <= 256, "the merge sorts four keys per lane");
+  __shared__ uint64_t best[NW + 1][KP];      // per wave, then the running list
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int64_t total = min((int64_t)*nrows, nrows_max) * T;
   for (int64_t p = blockIdx.x; p < total; p += gridDim.x) {
     const int64_t e = p / T;
@@ -1476,36 +1552,37 @@ k_row_exact_tree(const XT* __restrict__ X, int d, const int32_t* __restrict__ pe
     const int pos = invperm[(int64_t)t * n + me];
     const LeafItem it = items[rowitem[(int64_t)t * n + pos]];
     const int32_t* pl = perm + (int64_t)t * n + it.start;
-    TopK<KP> top;
-    top.init();
-    for (int c = tid; c < it.size; c += kFbThreadsRow) {  // leafmates of one tree are distinct
-      const int id = pl[c];
-      if (id != me) top.insert(exact_dist8(X + (int64_t)me * d, X + (int64_t)id * d, d), id);
-    }
+    const XT* xm = X + (int64_t)me * d;
+    const int m = it.size;
+    if (tid < KP) best[NW][tid] = ~0ull;
+    // leafmates of one tree are distinct; per round each wave sorts its share
+    // of kChunk of them, then wave 0 keeps the KP smallest of the waves' KP
+    // smallest and the running list
+    for (int c0 = 0; c0 < m; c0 += kChunk) {
+      const int mc = min(m - c0, kChunk);
+      uint64_t k0;
+      if (mc <= 2 * kFbThreadsRow) row_exact_wave<XT, 2>(X, d, xm, me, pl + c0, mc, tid, lane, k0);
+      else if (mc <= 4 * kFbThreadsRow) row_exact_wave<XT, 4>(X, d, xm, me, pl + c0, mc, tid, lane, k0);
+      else row_exact_wave<XT, 8>(X, d, xm, me, pl + c0, mc, tid, lane, k0);
+      if (lane < KP) best[w][lane] = k0;
+      __syncthreads();
+      if (w == 0) {
+        uint64_t k4[4];
 #pragma unroll
-    for (int s = 0; s < KP; ++s) {
-      ld[tid][s] = top.d[s];
-      li[tid][s] = top.i[s];
-    }
-    __syncthreads();
-    for (int h = kFbThreadsRow / 2; h > 0; h >>= 1) {
-      if (tid < h) {
-        for (int s = 0; s < KP; ++s) {  // partner's list is sorted
-          const int pi = li[tid + h][s];
-          if (pi == INT_MAX || !better(ld[tid + h][s], pi, top.d[KP - 1], top.i[KP - 1])) break;
-          top.insert(ld[tid + h][s], pi);
+        for (int h = 0; h < 4; ++h) {
+          const int j = h * 64 + lane;
+          k4[h] = j < (NW + 1) * KP ? best[j / KP][j % KP] : ~0ull;
         }
-#pragma unroll
-        for (int s = 0; s < KP; ++s) {
-          ld[tid][s] = top.d[s];
-          li[tid][s] = top.i[s];
-        }
+        bitonic_wave<uint64_t, 4>(k4, lane);
+        if (lane < KP) best[NW][lane] = k4[0];
       }
       __syncthreads();
     }
     if (tid < KP) {
-      tmp_i[p * KP + tid] = li[0][tid];
-      tmp_d[p * KP + tid] = ld[0][tid];
+      const uint64_t k = best[NW][tid];
+      const bool ok = k != ~0ull;
+      tmp_i[p * KP + tid] = ok ? (int32_t)(uint32_t)k : INT_MAX;
+      tmp_d[p * KP + tid] = ok ? __uint_as_float((uint32_t)(k >> 32)) : INFINITY;
     }
     __syncthreads();
   }
@@ -1713,9 +1790,9 @@ int persistent_grid(drpt_ctx* ctx, T kernel, int threads, unsigned* grid) {
 template <typename XT, typename EL, int T, int MODE, bool RING>
 int launch_lowp_r(drpt_ctx* ctx, const KnnArgs& a) {
   unsigned grid;
-  DRPT_CHECK(persistent_grid(ctx, k_knn_lowp<XT, EL, T, MODE, RING>, kThreads, &grid));
-  hipLaunchKernelGGL((k_knn_lowp<XT, EL, T, MODE, RING>), dim3(grid), dim3(kThreads), 0,
-                     ctx->stream, a);
+  constexpr int NT = 4 * lowp_rows<MODE>();
+  DRPT_CHECK(persistent_grid(ctx, k_knn_lowp<XT, EL, T, MODE, RING>, NT, &grid));
+  hipLaunchKernelGGL((k_knn_lowp<XT, EL, T, MODE, RING>), dim3(grid), dim3(NT), 0, ctx->stream, a);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }

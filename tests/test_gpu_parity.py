@@ -334,3 +334,22 @@ def test_lowp_k32_dense_clusters_equal_fp32(ctx, prec):
     _log(f"  overflow rows {out[prec][2].n_fallback_rows}")
     assert np.array_equal(out["fp32"][0], out[prec][0])
     assert np.array_equal(_bits(out["fp32"][1]), _bits(out[prec][1]))
+
+
+@pytest.mark.parametrize("prec,K", [("bf16", 10), ("bf16", 24)])
+def test_lowp_leaves_beyond_stage_equal_fp32(ctx, prec, K):
+    """Leaves larger than the 2048 points a block stages: their rows take the
+    exact row path (leafmates sorted in 2048-point rounds); the lists still
+    equal the FP32 build's."""
+    _log(f"[big leaves] {prec} K={K}")
+    X = datagen.int_mixture(9000)
+    n, d = X.shape
+    out = {}
+    for pr in ("fp32", prec):
+        p = drpt_amd.leafmax_params(n, d, 3, K, leaf_max=4500, precision=pr, seed=9)
+        oi, od, tm = ctx.build_graph(p, _dev(X))
+        out[pr] = (oi.cpu().numpy(), od.cpu().numpy(), tm)
+    _log(f"  largest leaf {out[prec][2].max_leaf}, exact rows {out[prec][2].n_fallback_rows}")
+    assert out[prec][2].max_leaf > 2048
+    assert np.array_equal(out["fp32"][0], out[prec][0])
+    assert np.array_equal(_bits(out["fp32"][1]), _bits(out[prec][1]))
