@@ -1304,6 +1304,21 @@ __device__ __forceinline__ void bitonic_wave(U (&v)[E], int lane) {
   }
 }
 
+// ascending sort of the first nvalid of the 64 E values (the rest are the
+// largest value): only the registers that hold valid values are sorted
+template <typename U, int E>
+__device__ __forceinline__ void bitonic_wave_prefix(U (&v)[E], int lane, int nvalid) {
+  if (E == 1 || nvalid <= 64) {
+    bitonic_wave<U, 1>(*reinterpret_cast<U(*)[1]>(&v[0]), lane);
+  } else if (E == 2 || nvalid <= 128) {
+    bitonic_wave<U, (E < 2 ? E : 2)>(*reinterpret_cast<U(*)[(E < 2 ? E : 2)]>(&v[0]), lane);
+  } else if (E == 4 || nvalid <= 256) {
+    bitonic_wave<U, (E < 4 ? E : 4)>(*reinterpret_cast<U(*)[(E < 4 ? E : 4)]>(&v[0]), lane);
+  } else {
+    bitonic_wave<U, E>(v, lane);
+  }
+}
+
 // exact D of (row, cand): the fma chain, 8 vector loads per batch (one
 // 128-byte line of each row for FP32 input)
 template <typename XT>
@@ -1448,7 +1463,7 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
         bitonic_wave<uint32_t, 1>(k1, lane);
         key[0] = lane < A.rerank ? (uint64_t)dn[k1[0] & 0x7FFu] : ~0ull;
       } else {
-        bitonic_wave<uint64_t, E>(key, lane);
+        bitonic_wave_prefix<uint64_t, E>(key, lane, cnt_new);
       }
       neval = A.rerank;
     }
@@ -1463,13 +1478,7 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
         key[e] = ~0ull;
       }
     }
-    if (E == 1 || neval <= 64) {
-      uint64_t k1[1] = {key[0]};
-      bitonic_wave<uint64_t, 1>(k1, lane);
-      key[0] = k1[0];
-    } else {
-      bitonic_wave<uint64_t, E>(key, lane);
-    }
+    bitonic_wave_prefix<uint64_t, E>(key, lane, neval);
     // merge the K best new entries (lanes < K of key[0]) with the known list
     const uint64_t nk = lane < K ? key[0] : ~0ull;
     const uint64_t kk = kid >= 0 ? (((uint64_t)__float_as_uint(kd) << 32) | (uint32_t)kid) : ~0ull;
