@@ -635,6 +635,7 @@ static ItemPlan lowp_plan_of(drpt_ctx* ctx, const drpt_params* p) {
     plan.ring[b] = it->second[kSizeBuckets + b];
   }
   plan.tree_group = it->second[2 * kSizeBuckets];
+  plan.first_trees = it->second[2 * kSizeBuckets + 1];
   return plan;
 }
 
@@ -654,7 +655,9 @@ static int lowp_setup(drpt_ctx* ctx, const drpt_params* p, const void* X,
     return DRPT_ERR_ARG;
   }
   r->KP = lowp_slots(&r->pp);
-  r->first = std::max(1, std::min(std::min(r->T, kFirstTrees), 256 / r->KP));
+  r->first = std::max(1, std::min(std::min(r->T, plan.first_trees > 0 ? plan.first_trees
+                                                                      : kFirstTrees),
+                                  256 / r->KP));
   r->pool_cap = pool_cap_of(r->K);
   const int64_t n = r->n;
   DRPT_CHECK(group_impl(ctx, p, leaf_heap, plan, r->first, &r->G));
@@ -970,15 +973,28 @@ int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* p
       v[kSizeBuckets + b] = best.ring[b];
     }
     v[2 * kSizeBuckets] = best.tree_group;
+    v[2 * kSizeBuckets + 1] = best.first_trees;
     ctx->plans[plan_key(p)] = v;
     for (int i = 0; i < kPlanInts; ++i) plan_out[i] = v[i];
   };
+  int32_t* oi;
+  float* od;
+  DRPT_CHECK(ws(ctx, "tune.out_idx", (size_t)p->n * p->nn, &oi));
+  DRPT_CHECK(ws(ctx, "tune.out_dist", (size_t)p->n * p->nn, &od));
+  // one low-precision stage with the stored plan (the first run sizes the workspace)
+  auto time_stage = [&](double* ms) -> int {
+    store();
+    for (int rep = 0; rep < 2; ++rep) {
+      DRPT_HIP(hipEventRecord(e0, s));
+      DRPT_CHECK(graph_knn_lowp(ctx, p, X, heap, oi, od, nullptr));
+      DRPT_HIP(hipEventRecord(e1, s));
+      DRPT_HIP(hipEventSynchronize(e1));
+      if (rep > 0) *ms = ms_between(e0, e1);
+    }
+    return DRPT_OK;
+  };
+  const int T = p->ntrees;
   if (p->tree_group == 0) {
-    int32_t* oi;
-    float* od;
-    DRPT_CHECK(ws(ctx, "tune.out_idx", (size_t)p->n * p->nn, &oi));
-    DRPT_CHECK(ws(ctx, "tune.out_dist", (size_t)p->n * p->nn, &od));
-    const int T = p->ntrees;
     const int cands[7] = {T, 8, 6, 5, 4, 3, 2};
     double best_ms = -1.0;
     int best_g = 0;
@@ -987,21 +1003,32 @@ int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* p
       if (g < 1 || g > T || std::find(tried.begin(), tried.end(), g) != tried.end()) continue;
       tried.push_back(g);
       best.tree_group = g;
-      store();
       double ms = 0.0;
-      for (int rep = 0; rep < 2; ++rep) {  // the first run sizes the workspace
-        DRPT_HIP(hipEventRecord(e0, s));
-        DRPT_CHECK(graph_knn_lowp(ctx, p, X, heap, oi, od, nullptr));
-        DRPT_HIP(hipEventRecord(e1, s));
-        DRPT_HIP(hipEventSynchronize(e1));
-        if (rep > 0) ms = ms_between(e0, e1);
-      }
+      DRPT_CHECK(time_stage(&ms));
       if (best_ms < 0.0 || ms < 0.97 * best_ms) {
         best_ms = ms;
         best_g = g;
       }
     }
     best.tree_group = best_g;
+  }
+  // trees whose widened candidates give the first bounds: fewer trees, a
+  // cheaper candidate pass and first merge, looser bounds for the pools
+  {
+    const int cands[3] = {kFirstTrees, 2, 1};
+    double best_ms = -1.0;
+    int best_f = 0;
+    for (int f : cands) {
+      if (f > T) continue;
+      best.first_trees = f;
+      double ms = 0.0;
+      DRPT_CHECK(time_stage(&ms));
+      if (best_ms < 0.0 || ms < 0.97 * best_ms) {
+        best_ms = ms;
+        best_f = f;
+      }
+    }
+    best.first_trees = best_f;
   }
   store();
   DRPT_HIP(hipStreamSynchronize(s));
@@ -1023,11 +1050,12 @@ int drpt_set_item_plan(drpt_ctx* ctx, const drpt_params* p, const int32_t* plan)
     v[b] = plan[b];
     v[kSizeBuckets + b] = plan[kSizeBuckets + b];
   }
-  if (plan[2 * kSizeBuckets] < 0) {
-    set_error("set_item_plan: tree_group must be >= 0");
+  if (plan[2 * kSizeBuckets] < 0 || plan[2 * kSizeBuckets + 1] < 0) {
+    set_error("set_item_plan: tree_group and first_trees must be >= 0");
     return DRPT_ERR_ARG;
   }
   v[2 * kSizeBuckets] = plan[2 * kSizeBuckets];
+  v[2 * kSizeBuckets + 1] = plan[2 * kSizeBuckets + 1];
   ctx->plans[plan_key(p)] = v;
   return DRPT_OK;
 }
