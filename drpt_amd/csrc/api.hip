@@ -147,6 +147,10 @@ int drpt_create(drpt_ctx** out, int device) {
   ctx->device = device;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
+      lds > 0)
+    ctx->max_lds = lds;
   if (const char* g = getenv("DRPT_GRAPHS")) ctx->use_graphs = strcmp(g, "0") != 0;
   if (hipStreamCreate(&ctx->stream) != hipSuccess) {
     delete ctx;

@@ -37,6 +37,8 @@ void set_error(const std::string& msg);
 struct drpt_ctx {
   int device = 0;
   int num_cus = 256;
+  int max_lds = 65536;      // LDS bytes a workgroup may use
+  bool split_lds_big = false;  // split kernels cleared for more than 64 KB of dynamic LDS
   hipStream_t stream = nullptr;
   std::map<std::string, std::pair<void*, size_t>> bufs;
   hipEvent_t ev[12] = {};

@@ -35,8 +35,11 @@ namespace drpt {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kLdsSumNodes = 2048;   // nodes per level kept in LDS for the sums
-constexpr int kLdsHistNodes = 512;   // nodes per level kept in LDS for histograms
+// per-node LDS arrays of the LDS variants (bytes): sums 12 (sum1: count +
+// int64 sum; sum2: 8), histograms 64 (16 slots); used while they fit
+// kSumLdsMax / kHistLdsMax and the device's per-workgroup LDS
+constexpr size_t kSumLdsMax = 96 * 1024;
+constexpr size_t kHistLdsMax = 64 * 1024;
 
 __device__ __forceinline__ void col_exponents(uint32_t cm, int bitlen, int& kq1, int& kq2) {
   int E = (int)((cm >> 23) & 0xffu);
@@ -426,6 +429,17 @@ int grow_trees_impl(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64
   DRPT_CHECK(ws(ctx, "split.E", T * NMAX * 16, &E));
   DRPT_CHECK(ws(ctx, "split.H", T * NMAX * 16, &H));
   const int bitlen = bitlen_of(n);
+  // LDS sums beyond 64 KB of dynamic LDS where the device allows it
+  if (!ctx->split_lds_big && ctx->max_lds > 64 * 1024) {
+    const int bytes = (int)std::min<size_t>((size_t)ctx->max_lds, kSumLdsMax);
+    if (hipFuncSetAttribute((const void*)k_sum1<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            bytes) == hipSuccess &&
+        hipFuncSetAttribute((const void*)k_sum2<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            bytes) == hipSuccess)
+      ctx->split_lds_big = true;
+    else
+      (void)hipGetLastError();
+  }
   // the per-level launches (4 fills + 7 kernels per level) as one hipGraph,
   // replayed while sizes, parameters and buffers stay the same
   char key[512];
@@ -455,12 +469,14 @@ int split_levels(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t 
     DRPT_CHECK(fill_u32(ctx, (uint32_t*)S2, 0u, 2 * TN));
     DRPT_CHECK(fill_u32(ctx, H, 0u, 16 * TN));
     unsigned ngrid = (unsigned)((TN + 255) / 256);
-    bool lds_sum = NN <= kLdsSumNodes;
-    bool lds_hist = NN <= kLdsHistNodes;
+    const size_t lds_max = std::min<size_t>((size_t)ctx->max_lds,
+                                            ctx->split_lds_big ? kSumLdsMax : 64 * 1024);
+    bool lds_sum = (size_t)NN * 12 <= lds_max;
+    bool lds_hist = (size_t)NN * 64 <= std::min(lds_max, kHistLdsMax);
     // points per block: with per-node LDS arrays, enough points to amortise their
     // flush; with global atomics, small blocks so every CU has work
     const int64_t ppb_sum = lds_sum ? std::max<int64_t>(8192, NN * 8) : 8192;
-    const int64_t ppb_hist = lds_hist ? std::max<int64_t>(8192, NN * 16) : 8192;
+    const int64_t ppb_hist = lds_hist ? std::max<int64_t>(8192, NN * 32) : 8192;
     const dim3 sgrid((unsigned)((n + ppb_sum - 1) / ppb_sum), (unsigned)T);
     const dim3 hgrid((unsigned)((n + ppb_hist - 1) / ppb_hist), (unsigned)T);
     const int64_t ppb = ppb_sum;

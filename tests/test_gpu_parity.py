@@ -79,6 +79,23 @@ def test_grow_trees_bitexact(ctx, orc, case, mode, leaf_max):
         assert np.array_equal(heap[t], ref), f"tree {t}: {(heap[t] != ref).sum()} points differ"
 
 
+@pytest.mark.parametrize("mode,D,leaf_max", [(0, 14, 0), (1, 15, 3)])
+def test_grow_trees_deep_levels_bitexact(ctx, orc, mode, D, leaf_max):
+    """Levels with 1024 - 8192 nodes per tree: per-node sums and histograms in
+    large LDS arrays, then global atomics; the trees equal the oracle's."""
+    _log(f"[grow deep] mode={mode} D={D}")
+    X = datagen.int_mixture(40000)
+    n, d = X.shape
+    T = 2
+    R = orc.build_R(d, T * D, 1.0 / np.sqrt(d), 31)
+    P0, cm0 = orc.project(X, R)
+    heap = ctx.grow_trees(_dev(P0), _dev(cm0.view(np.int32)), T, D, mode, leaf_max)
+    heap = heap.cpu().numpy().view(np.uint32)
+    for t in range(T):
+        ref = orc.grow_tree(P0, cm0, D, t, mode, leaf_max, 0)
+        assert np.array_equal(heap[t], ref), f"tree {t}: {(heap[t] != ref).sum()} points differ"
+
+
 def test_split_graph_replay(ctx, orc):
     """The split levels run as a captured hipGraph: replays into the same buffers
     (with eager work and a clobbered output between them), a recapture for a
