@@ -173,19 +173,21 @@ class Context:
         """drpt_autotune: the fastest work-item layout per leaf-size bucket
         (sizes <= 64, <= 256, <= 1024, larger) for this BF16/FP8 workload, kept
         by the context for later builds of the same shape. Returns
-        {"rows_per_item": [4], "ring": [4], "tree_group": int, "first_trees": int}."""
+        {"rows_per_item": [4], "ring": [4], "tree_group": int, "first_trees": int,
+        "rerank": int}."""
         torch = _torch()
-        plan = (ctypes.c_int32 * 10)()
+        plan = (ctypes.c_int32 * 11)()
         torch.cuda.synchronize(X.device)
         _lib.check(self.lib.drpt_autotune(self.h, ctypes.byref(params), _ptr(X),
                                           ctypes.cast(plan, ctypes.c_void_p)))
         return {"rows_per_item": list(plan[:4]), "ring": list(plan[4:8]), "tree_group": plan[8],
-                "first_trees": plan[9]}
+                "first_trees": plan[9], "rerank": plan[10]}
 
     def set_item_plan(self, params, plan):
         """drpt_set_item_plan: restore a plan returned by autotune()."""
-        arr = (ctypes.c_int32 * 10)(*(list(plan["rows_per_item"]) + list(plan["ring"]) +
-                                      [plan.get("tree_group", 0), plan.get("first_trees", 0)]))
+        arr = (ctypes.c_int32 * 11)(*(list(plan["rows_per_item"]) + list(plan["ring"]) +
+                                      [plan.get("tree_group", 0), plan.get("first_trees", 0),
+                                       plan.get("rerank", 0)]))
         _lib.check(self.lib.drpt_set_item_plan(self.h, ctypes.byref(params),
                                                ctypes.cast(arr, ctypes.c_void_p)))
 

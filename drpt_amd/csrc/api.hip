@@ -640,6 +640,7 @@ static ItemPlan lowp_plan_of(drpt_ctx* ctx, const drpt_params* p) {
   }
   plan.tree_group = it->second[2 * kSizeBuckets];
   plan.first_trees = it->second[2 * kSizeBuckets + 1];
+  plan.rerank = it->second[2 * kSizeBuckets + 2];
   return plan;
 }
 
@@ -737,7 +738,9 @@ static int lowp_known(drpt_ctx* ctx, LowpRun* r, const void* X, const ItemPlan& 
   m.src_tree_stride = r->n * r->KP;
   m.S = r->KP;
   m.ntsrc = r->first;
-  m.rerank = std::min(2 * r->K, r->first * r->KP);
+  // at least K exact distances per row: the K-th of them bounds the final K-th
+  const int rr = plan.rerank > 0 ? std::max(r->K, std::min(plan.rerank, 2 * r->K)) : 2 * r->K;
+  m.rerank = std::min(rr, r->first * r->KP);
   m.out_idx = r->ki;
   m.out_dist = r->kd;
   m.order = r->G.perm;  // tree 0's leaf order
@@ -978,6 +981,7 @@ int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* p
     }
     v[2 * kSizeBuckets] = best.tree_group;
     v[2 * kSizeBuckets + 1] = best.first_trees;
+    v[2 * kSizeBuckets + 2] = best.rerank;
     ctx->plans[plan_key(p)] = v;
     for (int i = 0; i < kPlanInts; ++i) plan_out[i] = v[i];
   };
@@ -1034,6 +1038,25 @@ int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* p
     }
     best.first_trees = best_f;
   }
+  // exact re-rank depth of the first bounds: fewer gathers in the first merge,
+  // looser bounds for the pools
+  {
+    const int K = p->nn;
+    const int cands[3] = {2 * K, (3 * K + 1) / 2, K};
+    double best_ms = -1.0;
+    int best_r = 0;
+    for (int i = 0; i < 3; ++i) {
+      if (i > 0 && cands[i] == cands[i - 1]) continue;
+      best.rerank = cands[i];
+      double ms = 0.0;
+      DRPT_CHECK(time_stage(&ms));
+      if (best_ms < 0.0 || ms < 0.97 * best_ms) {
+        best_ms = ms;
+        best_r = cands[i];
+      }
+    }
+    best.rerank = best_r;
+  }
   store();
   DRPT_HIP(hipStreamSynchronize(s));
   return DRPT_OK;
@@ -1054,12 +1077,14 @@ int drpt_set_item_plan(drpt_ctx* ctx, const drpt_params* p, const int32_t* plan)
     v[b] = plan[b];
     v[kSizeBuckets + b] = plan[kSizeBuckets + b];
   }
-  if (plan[2 * kSizeBuckets] < 0 || plan[2 * kSizeBuckets + 1] < 0) {
-    set_error("set_item_plan: tree_group and first_trees must be >= 0");
+  if (plan[2 * kSizeBuckets] < 0 || plan[2 * kSizeBuckets + 1] < 0 ||
+      plan[2 * kSizeBuckets + 2] < 0) {
+    set_error("set_item_plan: tree_group, first_trees and rerank must be >= 0");
     return DRPT_ERR_ARG;
   }
   v[2 * kSizeBuckets] = plan[2 * kSizeBuckets];
   v[2 * kSizeBuckets + 1] = plan[2 * kSizeBuckets + 1];
+  v[2 * kSizeBuckets + 2] = plan[2 * kSizeBuckets + 2];
   ctx->plans[plan_key(p)] = v;
   return DRPT_OK;
 }

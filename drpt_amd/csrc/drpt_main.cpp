@@ -138,12 +138,12 @@ int main(int argc, char* argv[]) {
     die("upload failed");
   drpt_timings tm;
   if (autotune && p.precision != DRPT_FP32) {
-    int32_t plan[10];
+    int32_t plan[11];
     check(drpt_autotune(ctx, &p, dX, plan), "drpt_autotune");
     std::printf("Work-item plan: rows per item %d %d %d %d, ring %d %d %d %d, trees per group %d, "
-                "first trees %d\n",
+                "first trees %d, re-rank %d\n",
                 plan[0], plan[1], plan[2], plan[3], plan[4], plan[5], plan[6], plan[7], plan[8],
-                plan[9]);
+                plan[9], plan[10]);
   }
   check(drpt_build_graph(ctx, &p, dX, dI, dD, &tm), "drpt_build_graph");
   std::vector<int32_t> idx((size_t)data_set_size * nn);

@@ -106,15 +106,18 @@ struct LeafItem {   // one work item: a block of rows of one leaf (or the whole 
 // Gram passes over them use the LDS tile ring (ring[b] = 1, where the row
 // length allows it) or the register double buffer (0); tree_group = trees per
 // pooled filter pass (0: the rule of drpt_params.tree_group = 0); first_trees =
-// trees whose widened candidates give the first bounds (0: 4). Chosen by
-// drpt_autotune; the default is whole leaves, the ring and the rules.
+// trees whose widened candidates give the first bounds (0: 4); rerank = their
+// best approximate candidates whose exact distances give those bounds, in
+// [K, 2K] (0: 2K). Chosen by drpt_autotune; the default is whole leaves, the
+// ring and the rules.
 constexpr int kSizeBuckets = 4;
-constexpr int kPlanInts = 2 * kSizeBuckets + 2;  // rpi, ring, tree_group, first_trees
+constexpr int kPlanInts = 2 * kSizeBuckets + 3;  // rpi, ring, tree_group, first_trees, rerank
 struct ItemPlan {
   int rpi[kSizeBuckets];
   int ring[kSizeBuckets];
   int tree_group;
   int first_trees;
+  int rerank;
 };
 __host__ __device__ inline int size_bucket(int m) {
   return m <= 64 ? 0 : m <= 256 ? 1 : m <= 1024 ? 2 : 3;
@@ -133,6 +136,7 @@ inline ItemPlan uniform_plan(int rpi) {
   }
   p.tree_group = 0;
   p.first_trees = 0;
+  p.rerank = 0;
   return p;
 }
 

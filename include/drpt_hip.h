@@ -134,20 +134,22 @@ int drpt_build_graph(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t
  * per leaf-size bucket (sizes <= 64, <= 256, <= 1024, larger) over the rows a
  * work item takes (whole leaf, 256, 128, 64) and the Gram pipeline (LDS tile
  * ring or register double buffer), then the whole low-precision stage per
- * trees-per-group (ntrees, 8, 6, 5, 4, 3, 2; when p->tree_group is 0) and per
- * number of trees whose widened candidates give the first bounds (4, 2, 1).
- * Keeps the fastest in the context and writes it to plan_out[10]: rows per item
- * of buckets 0-3, their ring flags, trees per group, first trees. Later
- * drpt_build_graph calls
+ * trees-per-group (ntrees, 8, 6, 5, 4, 3, 2; when p->tree_group is 0), per
+ * number of trees whose widened candidates give the first bounds (4, 2, 1) and
+ * per number of their best approximate candidates re-ranked exactly for those
+ * bounds (2 nn, 1.5 nn, nn). Keeps the fastest in the context and writes it to
+ * plan_out[11]: rows per item of buckets 0-3, their ring flags, trees per
+ * group, first trees, re-rank depth. Later drpt_build_graph calls
  * with the same n, d, dtype, precision, nn, ntrees, depth, leaf_max,
  * local_mode, kprime and tree_group use it; results are unchanged by the
  * plan. */
 int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* plan_out);
 
 /* Sets the work-item plan of a workload shape, in drpt_autotune's plan_out
- * layout (rows per item >= 1, ring flags 0 / 1, trees per group and first
- * trees >= 0 with 0 for the automatic rules): restores a plan saved from an
- * earlier drpt_autotune. No counterpart in the reference. */
+ * layout (rows per item >= 1, ring flags 0 / 1, trees per group, first trees
+ * and re-rank depth >= 0 with 0 for the automatic rules; the depth is clamped
+ * to [nn, 2 nn]): restores a plan saved from an earlier drpt_autotune. No
+ * counterpart in the reference. */
 int drpt_set_item_plan(drpt_ctx* ctx, const drpt_params* p, const int32_t* plan);
 
 #ifdef __cplusplus

@@ -322,8 +322,10 @@ def test_autotune_and_item_plans(ctx, prec, case, leaf_max, T):
         assert all(g in (0, 1) for g in plan["ring"])
         assert 1 <= plan["tree_group"] <= T
         assert plan["first_trees"] in (1, 2, 4)
+        assert 10 <= plan["rerank"] <= 20
         forced = [plan,
-                  {"rows_per_item": [64] * 4, "ring": [0] * 4, "tree_group": 2, "first_trees": 1},
+                  {"rows_per_item": [64] * 4, "ring": [0] * 4, "tree_group": 2, "first_trees": 1,
+                   "rerank": 10},
                   {"rows_per_item": [64, 128, 256, 64], "ring": [1, 0, 1, 0], "tree_group": 3,
                    "first_trees": 2},
                   {"rows_per_item": [1 << 30] * 4, "ring": [0, 1, 0, 1], "tree_group": 0}]
