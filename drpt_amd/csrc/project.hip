@@ -44,8 +44,8 @@ k_project(const XT* __restrict__ X, int64_t n, int d, int L, const int32_t* __re
   const int nnz = colptr[L];
   uint32_t* lmax = reinterpret_cast<uint32_t*>(smem);
   const int32_t* lcp = colptr;
-  const float* lw = w;
   const uint16_t* lri = rowidx;
+  const float* lw = w;
   if constexpr (RL) {
     int32_t* scp = reinterpret_cast<int32_t*>(lmax + L);
     float* sw = reinterpret_cast<float*>(scp + L + 1);
@@ -59,6 +59,7 @@ k_project(const XT* __restrict__ X, int64_t n, int d, int L, const int32_t* __re
     lw = sw;
     lri = sri;
   }
+  auto wv = [&](int k) -> double { return (double)lw[k]; };
   XT* tile = reinterpret_cast<XT*>(smem + r_bytes(L, nnz, RL));
   for (int c = threadIdx.x; c < L; c += kThreads) lmax[c] = 0u;
   constexpr int kSlots = 64 / TR;  // columns a wave works on at once
@@ -100,15 +101,17 @@ k_project(const XT* __restrict__ X, int64_t n, int d, int L, const int32_t* __re
         double acc = 0.0;
         const XT* xr = tile + r * pitch;
         int k = beg;
+        // x * w is exact in double, so one fused multiply-add per term rounds
+        // as the separate product and sum would
         for (; k + 4 <= end; k += 4) {  // independent loads, sums still in ascending k
           const double x0 = (double)xr[lri[k]], x1 = (double)xr[lri[k + 1]];
           const double x2 = (double)xr[lri[k + 2]], x3 = (double)xr[lri[k + 3]];
-          acc = __dadd_rn(acc, __dmul_rn(x0, (double)lw[k]));
-          acc = __dadd_rn(acc, __dmul_rn(x1, (double)lw[k + 1]));
-          acc = __dadd_rn(acc, __dmul_rn(x2, (double)lw[k + 2]));
-          acc = __dadd_rn(acc, __dmul_rn(x3, (double)lw[k + 3]));
+          acc = __fma_rn(x0, wv(k), acc);
+          acc = __fma_rn(x1, wv(k + 1), acc);
+          acc = __fma_rn(x2, wv(k + 2), acc);
+          acc = __fma_rn(x3, wv(k + 3), acc);
         }
-        for (; k < end; ++k) acc = __dadd_rn(acc, __dmul_rn((double)xr[lri[k]], (double)lw[k]));
+        for (; k < end; ++k) acc = __fma_rn((double)xr[lri[k]], wv(k), acc);
         pv = (float)acc;
         if (valid_row) P[(int64_t)c * n + row0 + r] = pv;
       }
