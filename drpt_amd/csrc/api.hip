@@ -583,8 +583,9 @@ constexpr int kFirstTrees = 4;
 // Pool slots per row (more survivors send the row to the exact row path): the
 // pool pass drops a row's known ids when K <= 16 (ids staged in LDS); with
 // larger K they stay in the pool and the row merge drops them, so the pool
-// is larger.
-inline int pool_cap_of(int K) { return K <= 16 ? 256 : 512; }
+// is larger (a row merge of up to 1024 entries is still far cheaper than the
+// exact path over every leafmate of every tree).
+inline int pool_cap_of(int K) { return K <= 16 ? 256 : 1024; }
 
 // Low-precision graph (build_graph): the final lists straight from the leaves.
 //  1. kModeCand on trees < a: k' widened candidates per row and tree (with

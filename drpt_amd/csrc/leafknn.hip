@@ -1143,15 +1143,24 @@ __global__ void __launch_bounds__(4 * lowp_rows<MODE>(), lowp_rows<MODE>() == 64
               const int slot = atomicAdd(&S.ccnt[q], 1);
               if (slot < kCap) S.cpos[q][slot] = (uint16_t)c;
             }
-          } else {  // this thread's list (known ids dropped at the flush)
-            if (nsub < kSubCap) S.cpos[q][sub * kSubCap + nsub] = (uint16_t)c;
+          } else if (nsub < kSubCap) {  // this thread's list (known ids dropped at the flush)
+            S.cpos[q][sub * kSubCap + nsub] = (uint16_t)c;
             ++nsub;
+          } else {  // list full within this tile: straight to the row's pool (the row
+                    // merge drops known ids)
+            const int id = S.lid[qpos];
+            const int slot = atomicAdd(&A.pool_cnt[id], 1);
+            if (slot < A.pool_cap) {
+              A.out_idx[(int64_t)id * A.pool_cap + slot] = S.lid[c];
+            } else if (atomicExch(&A.ovf_flag[id], 1) == 0) {
+              A.ovf_list[atomicAdd(A.novf, 1)] = id;
+            }
           }
         }
         if constexpr (MODE == kModePool) {
           // flushed at half capacity: a tile adds at most 16 entries per
-          // thread but rarely more than a few; a list that still overflows
-          // sends its row to the exact row path
+          // thread but rarely more than a few; entries that still do not fit
+          // go to the pool directly
           if (nsub > kSubCap / 2) S.flush_flag[(c0 / kCols) & 1] = 1;
         }
       }, hook);
@@ -1314,6 +1323,8 @@ __device__ __forceinline__ void bitonic_wave_prefix(U (&v)[E], int lane, int nva
     bitonic_wave<U, (E < 2 ? E : 2)>(*reinterpret_cast<U(*)[(E < 2 ? E : 2)]>(&v[0]), lane);
   } else if (E == 4 || nvalid <= 256) {
     bitonic_wave<U, (E < 4 ? E : 4)>(*reinterpret_cast<U(*)[(E < 4 ? E : 4)]>(&v[0]), lane);
+  } else if (E == 8 || nvalid <= 512) {
+    bitonic_wave<U, (E < 8 ? E : 8)>(*reinterpret_cast<U(*)[(E < 8 ? E : 8)]>(&v[0]), lane);
   } else {
     bitonic_wave<U, E>(v, lane);
   }
@@ -1353,7 +1364,7 @@ template <typename XT, int E>
 __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeArgs A) {
   // per-wave open-addressing table for distinct ids: more slots than the K
   // known ids + 64 E insertions, so probing ends
-  constexpr int kHashBits = E <= 4 ? 9 : 10;
+  constexpr int kHashBits = E <= 4 ? 9 : E <= 8 ? 10 : 11;
   constexpr int kHashSlots = 1 << kHashBits;
   __shared__ uint32_t tab[kMergeWaves][kHashSlots];
   __shared__ uint64_t dense[kMergeWaves][64 * E];
@@ -1948,9 +1959,9 @@ int launch_leaf_gather_val(drpt_ctx* ctx, const int32_t* perm, int64_t total, co
 
 int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a) {
   const int slots = a.S * a.ntsrc;
-  if (a.K < 1 || a.K > 64 || slots > 512 || a.rerank > 64 * 4 || a.ntsrc > 64 ||
+  if (a.K < 1 || a.K > 64 || slots > 1024 || a.rerank > 64 * 4 || a.ntsrc > 64 ||
       (a.src_key != nullptr && slots > 256)) {
-    set_error("row_merge: unsupported sizes (nn in [1, 64], at most 512 candidate slots, "
+    set_error("row_merge: unsupported sizes (nn in [1, 64], at most 1024 candidate slots, "
               "256 with approximate keys)");
     return DRPT_ERR_UNSUPPORTED;
   }
@@ -1959,8 +1970,11 @@ int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a) {
                                          (int64_t)ctx->num_cus * 16);
   const unsigned grid = (unsigned)(((std::max<int64_t>(want, 8) + 7) / 8) * 8);
   const bool u8 = dtype == DRPT_U8;
-  const int E = slots <= 64 ? 1 : slots <= 128 ? 2 : slots <= 256 ? 4 : 8;
-  if (E == 8) {
+  const int E = slots <= 64 ? 1 : slots <= 128 ? 2 : slots <= 256 ? 4 : slots <= 512 ? 8 : 16;
+  if (E == 16) {
+    if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 16>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+    else hipLaunchKernelGGL((k_row_merge<float, 16>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+  } else if (E == 8) {
     if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 8>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
     else hipLaunchKernelGGL((k_row_merge<float, 8>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
   } else if (E == 1) {
