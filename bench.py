@@ -256,7 +256,8 @@ def main():
         path = ("ring" if rings == {1} else "regs" if rings == {0} else "mixed") \
             if (2 * d) % 16 == 0 and 2 * d <= 256 else "regs"
         kernel = f"k_knn_lowp<float, ElBf16, {t}, kModePool, {path}>"
-        gsz = p.tree_group or max(1, min(p.ntrees, max(4, 160 // p.nn)))
+        gsz = p.tree_group or (plan or {}).get("tree_group", 0) or \
+            max(1, min(p.ntrees, max(4, 160 // p.nn)))
         res["pool_launches"] = -(-p.ntrees // gsz)
     peak = PEAK["fp32" if prec == "fp32" else "bf16"]
     achieved = fl / (kms / 1e3) / 1e12
