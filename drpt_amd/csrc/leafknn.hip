@@ -547,7 +547,7 @@ __device__ __forceinline__ void load_query(SH& S, const uint8_t* __restrict__ Xq
 // query operand in registers (rows of at most kChunkBytes) the leafmate tiles
 // are double-buffered: one barrier per tile, the next tile's loads in flight
 // during this tile's MFMAs and epilogue. rb = bytes per row, d = elements.
-template <typename EL, int NR, typename SH, typename Epi, typename Hook>
+template <typename EL, int NR, bool kPipe, typename SH, typename Epi, typename Hook>
 __device__ __forceinline__ void gram_pass(SH& S, const uint8_t* __restrict__ Xq, int d, int rb,
                                           int m, int row0, int rows, const QueryOp<EL>& qo,
                                           Epi&& epi, Hook&& hook) {
@@ -607,6 +607,55 @@ __device__ __forceinline__ void gram_pass(SH& S, const uint8_t* __restrict__ Xq,
   // longer rows: chunks of kChunkBytes, query chunk in buffers 1.., leafmates in buffer 0
   unsigned char(*As)[kTilePitchB] = S.Bs[1];
   unsigned char(*Bs)[kTilePitchB] = S.Bs[0];
+  if constexpr (kPipe && NR == kCols) {
+    // the next chunk's query and leafmate pieces (this tile's next chunk, else
+    // the next tile's first) are loaded into registers during this chunk's
+    // MFMAs (and the tile's epilogue)
+    uint4 pa[kTileLoads], pb[kTileLoads];
+    auto fetch = [&](int c0, int o0) {
+#pragma unroll
+      for (int s4 = 0; s4 < kTileLoads; ++s4) {
+        const int c = tid + NT * s4;
+        const int r = c >> 4, part = c & 15;
+        const int col = c0 + r;
+        pa[s4] = load16(Xq, r < rows ? S.lid[row0 + r] : -1, rb, o0 + part * 16);
+        pb[s4] = load16(Xq, col < m ? S.lid[col] : -1, rb, o0 + part * 16);
+      }
+    };
+    fetch(0, 0);
+    for (int c0 = 0; c0 < m; c0 += kCols) {
+      floatx16 acc;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+      for (int o0 = 0; o0 < rb; o0 += kChunkBytes) {
+        __syncthreads();  // every wave done with the previous chunk
+#pragma unroll
+        for (int s4 = 0; s4 < kTileLoads; ++s4) {
+          const int c = tid + NT * s4;
+          const int r = c >> 4, part = c & 15;
+          *reinterpret_cast<uint4*>(&As[r][part * 16]) = pa[s4];
+          *reinterpret_cast<uint4*>(&Bs[r][part * 16]) = pb[s4];
+        }
+        __syncthreads();
+        const bool tile_end = o0 + kChunkBytes >= rb;
+        if (!tile_end) fetch(c0, o0 + kChunkBytes);
+        else if (c0 + kCols < m) fetch(c0 + kCols, 0);
+        const int ksteps = (min(kChunkBytes, rb - o0) + kStepBytes - 1) / kStepBytes;
+        for (int ks = 0; ks < ksteps; ++ks) {
+          const typename EL::op_t a = EL::lds(&Bs[ch + (lane & 31)][op_offset<EL>(ks, lane)]);
+          const typename EL::op_t bq = EL::lds(&As[qh + (lane & 31)][op_offset<EL>(ks, lane)]);
+          acc = EL::mma(a, bq, acc);
+        }
+      }
+      epi(c0, acc);
+      if (c0 + kCols < m) {
+        __syncthreads();
+        hook(c0 / kCols);
+      }
+    }
+    __syncthreads();
+    return;
+  }
   for (int c0 = 0; c0 < m; c0 += kCols) {
     floatx16 acc;
 #pragma unroll
@@ -945,7 +994,7 @@ __global__ void __launch_bounds__(4 * lowp_rows<MODE>(), lowp_rows<MODE>() == 64
       auto gram = [&](auto&& epi, auto&& hook) {
         auto e2 = scaled(epi);
         if constexpr (RING) gram_pass_ring<EL, NR>(S, Xq, d, rb, m, qo, e2, hook);
-        else gram_pass<EL, NR>(S, Xq, d, rb, m, row0, rows, qo, e2, hook);
+        else gram_pass<EL, NR, MODE != kModeTree>(S, Xq, d, rb, m, row0, rows, qo, e2, hook);
       };
       uint32_t key[T];
 #pragma unroll
