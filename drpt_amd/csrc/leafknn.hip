@@ -608,49 +608,71 @@ __device__ __forceinline__ void gram_pass(SH& S, const uint8_t* __restrict__ Xq,
   unsigned char(*As)[kTilePitchB] = S.Bs[1];
   unsigned char(*Bs)[kTilePitchB] = S.Bs[0];
   if constexpr (kPipe && NR == kCols) {
-    // the next chunk's query and leafmate pieces (this tile's next chunk, else
-    // the next tile's first) are loaded into registers during this chunk's
-    // MFMAs (and the tile's epilogue)
-    uint4 pa[kTileLoads], pb[kTileLoads];
+    // Two leafmate tiles per step: every wave multiplies its query operand
+    // against columns ch.. of both tiles (one query read for two MFMAs), over
+    // chunks of kPairChunk bytes (query and the two tiles fit the tile
+    // buffers). The next chunk's pieces (this pair's next chunk, else the next
+    // pair's first) are loaded into registers during this chunk's MFMAs and
+    // the pair's epilogues.
+    constexpr int kPC = 128;                              // bytes of a row per step
+    constexpr int kPP = kPC + 16;                         // LDS row pitch
+    constexpr int kQL = NR * (kPC / 16) / NT;             // query pieces per thread
+    constexpr int kBL = 2 * kCols * (kPC / 16) / NT;      // leafmate pieces per thread
+    static_assert((NR + 2 * kCols) * kPP <= (int)sizeof(S.Bs), "pair tiles fit the tile buffers");
+    unsigned char* const Aq = &S.Bs[0][0][0];
+    unsigned char* const Bl = Aq + NR * kPP;
+    uint4 pa[kQL], pb[kBL];
     auto fetch = [&](int c0, int o0) {
 #pragma unroll
-      for (int s4 = 0; s4 < kTileLoads; ++s4) {
-        const int c = tid + NT * s4;
-        const int r = c >> 4, part = c & 15;
+      for (int s = 0; s < kQL; ++s) {
+        const int c = tid + NT * s;
+        const int r = c >> 3, part = c & 7;
+        pa[s] = load16(Xq, r < rows ? S.lid[row0 + r] : -1, rb, o0 + part * 16);
+      }
+#pragma unroll
+      for (int s = 0; s < kBL; ++s) {
+        const int c = tid + NT * s;
+        const int r = c >> 3, part = c & 7;
         const int col = c0 + r;
-        pa[s4] = load16(Xq, r < rows ? S.lid[row0 + r] : -1, rb, o0 + part * 16);
-        pb[s4] = load16(Xq, col < m ? S.lid[col] : -1, rb, o0 + part * 16);
+        pb[s] = load16(Xq, col < m ? S.lid[col] : -1, rb, o0 + part * 16);
       }
     };
     fetch(0, 0);
-    for (int c0 = 0; c0 < m; c0 += kCols) {
-      floatx16 acc;
+    for (int c0 = 0; c0 < m; c0 += 2 * kCols) {
+      floatx16 acc0, acc1;
 #pragma unroll
-      for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
-      for (int o0 = 0; o0 < rb; o0 += kChunkBytes) {
+      for (int e = 0; e < 16; ++e) acc0[e] = acc1[e] = 0.0f;
+      for (int o0 = 0; o0 < rb; o0 += kPC) {
         __syncthreads();  // every wave done with the previous chunk
 #pragma unroll
-        for (int s4 = 0; s4 < kTileLoads; ++s4) {
-          const int c = tid + NT * s4;
-          const int r = c >> 4, part = c & 15;
-          *reinterpret_cast<uint4*>(&As[r][part * 16]) = pa[s4];
-          *reinterpret_cast<uint4*>(&Bs[r][part * 16]) = pb[s4];
+        for (int s = 0; s < kQL; ++s) {
+          const int c = tid + NT * s;
+          *reinterpret_cast<uint4*>(Aq + (c >> 3) * kPP + (c & 7) * 16) = pa[s];
+        }
+#pragma unroll
+        for (int s = 0; s < kBL; ++s) {
+          const int c = tid + NT * s;
+          *reinterpret_cast<uint4*>(Bl + (c >> 3) * kPP + (c & 7) * 16) = pb[s];
         }
         __syncthreads();
-        const bool tile_end = o0 + kChunkBytes >= rb;
-        if (!tile_end) fetch(c0, o0 + kChunkBytes);
-        else if (c0 + kCols < m) fetch(c0 + kCols, 0);
-        const int ksteps = (min(kChunkBytes, rb - o0) + kStepBytes - 1) / kStepBytes;
+        if (o0 + kPC < rb) fetch(c0, o0 + kPC);
+        else if (c0 + 2 * kCols < m) fetch(c0 + 2 * kCols, 0);
+        const int ksteps = (min(kPC, rb - o0) + kStepBytes - 1) / kStepBytes;
         for (int ks = 0; ks < ksteps; ++ks) {
-          const typename EL::op_t a = EL::lds(&Bs[ch + (lane & 31)][op_offset<EL>(ks, lane)]);
-          const typename EL::op_t bq = EL::lds(&As[qh + (lane & 31)][op_offset<EL>(ks, lane)]);
-          acc = EL::mma(a, bq, acc);
+          const int o = op_offset<EL>(ks, lane);
+          const typename EL::op_t bq = EL::lds(Aq + (qh + (lane & 31)) * kPP + o);
+          const typename EL::op_t a0 = EL::lds(Bl + (ch + (lane & 31)) * kPP + o);
+          const typename EL::op_t a1 = EL::lds(Bl + (kCols + ch + (lane & 31)) * kPP + o);
+          acc0 = EL::mma(a0, bq, acc0);
+          acc1 = EL::mma(a1, bq, acc1);
         }
       }
-      epi(c0, acc);
-      if (c0 + kCols < m) {
+      epi(c0, acc0);
+      if (c0 + kCols < m) epi(c0 + kCols, acc1);
+      if (c0 + 2 * kCols < m) {
         __syncthreads();
         hook(c0 / kCols);
+        hook(c0 / kCols + 1);
       }
     }
     __syncthreads();
