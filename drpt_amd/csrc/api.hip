@@ -776,6 +776,12 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
     DRPT_CHECK(ws(ctx, "knn.known_idx2", (size_t)n * K, &ki2));
     DRPT_CHECK(ws(ctx, "knn.known_dist2", (size_t)n * K, &kd2));
   }
+  int32_t* heavy = nullptr;  // rows whose pools hold more than 256 entries
+  int32_t* nheavy = nullptr;
+  if (r.pool_cap > 256) {
+    DRPT_CHECK(ws(ctx, "knn.heavy", (size_t)n, &heavy));
+    DRPT_CHECK(ws(ctx, "knn.nheavy", 1, &nheavy));
+  }
   while ((int)ctx->ev_pass.size() < 2 * ngroups) {
     hipEvent_t e;
     DRPT_HIP(hipEventCreate(&e));
@@ -811,6 +817,8 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
     f.out_idx = last ? out_idx : (kin == r.ki ? ki2 : r.ki);
     f.out_dist = last ? out_dist : (kdn == r.kd ? kd2 : r.kd);
     f.order = G.perm;
+    f.heavy = heavy;
+    f.nheavy = nheavy;
     DRPT_CHECK(launch_row_merge(ctx, p->dtype, f));
     kin = f.out_idx;
     kdn = f.out_dist;

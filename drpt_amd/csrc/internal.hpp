@@ -235,6 +235,12 @@ struct RowMergeArgs {
   const int32_t* order;      // optional permutation of the rows (processing order)
   int32_t* out_idx;          // [n][K]
   float* out_dist;
+  // optional, one source with more than 256 slots: rows with at most 256
+  // candidates are merged by a light pass (fewer registers per wave, more waves
+  // in flight), the others listed in heavy[0 .. *nheavy) for the full pass
+  int32_t* heavy;            // [n]
+  int32_t* nheavy;           // [1]
+  const int32_t* nrows_dev;  // kernel: rows to process read on the device (order = heavy)
 };
 int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a);
 // rows listed in ovf_list: exact K best over every leafmate in every tree

@@ -1449,12 +1449,22 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
   // candidates), one contiguous chunk per XCD (blocks go round-robin over 8)
   const int xcd = blockIdx.x & 7;
   const int64_t nblk = (gridDim.x - xcd + 7) >> 3;  // blocks on this XCD
-  const int64_t chunk = (A.n + 7) >> 3;
-  const int64_t lo = xcd * chunk, hi = min(A.n, lo + chunk);
+  const int64_t nr = A.nrows_dev != nullptr ? (int64_t)*A.nrows_dev : A.n;
+  const int64_t chunk = (nr + 7) >> 3;
+  const int64_t lo = xcd * chunk, hi = min(nr, lo + chunk);
   const int nslots = A.S * A.ntsrc;
   for (int64_t i = lo + (int64_t)(blockIdx.x >> 3) * kMergeWaves + w; i < hi;
        i += nblk * kMergeWaves) {
     const int64_t row = A.order != nullptr ? (int64_t)A.order[i] : i;
+    if (A.heavy != nullptr) {  // light pass (one source): rows beyond 64 E slots listed
+      if (A.skip_flag == nullptr || A.skip_flag[row] == 0) {
+        const int c = min(A.cnt_tree[row], A.S);
+        if (c > 64 * E) {
+          if (lane == 0) A.heavy[atomicAdd(A.nheavy, 1)] = (int32_t)row;
+          continue;
+        }
+      }
+    }
     if (A.skip_flag != nullptr && A.skip_flag[row] != 0) {
       // left to the exact row path; the known list passes through (a valid
       // bound for later passes)
@@ -2041,6 +2051,24 @@ int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a) {
                                          (int64_t)ctx->num_cus * 16);
   const unsigned grid = (unsigned)(((std::max<int64_t>(want, 8) + 7) / 8) * 8);
   const bool u8 = dtype == DRPT_U8;
+  if (a.heavy != nullptr && a.nheavy != nullptr && a.ntsrc == 1 && a.cnt_tree != nullptr &&
+      a.src != nullptr && a.rerank == 0 && slots > 256) {
+    // light pass over every row (rows of at most 256 candidates merged, the
+    // others listed), then the full pass over the listed rows only
+    DRPT_CHECK(fill_u32(ctx, (uint32_t*)a.nheavy, 0u, 1));
+    RowMergeArgs l = a;
+    l.nrows_dev = nullptr;
+    if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 4>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, l);
+    else hipLaunchKernelGGL((k_row_merge<float, 4>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, l);
+    DRPT_HIP(hipGetLastError());
+    a.order = a.heavy;
+    a.nrows_dev = a.nheavy;
+    a.heavy = nullptr;
+    a.nheavy = nullptr;
+  } else {
+    a.heavy = nullptr;
+    a.nheavy = nullptr;
+  }
   const int E = slots <= 64 ? 1 : slots <= 128 ? 2 : slots <= 256 ? 4 : slots <= 512 ? 8 : 16;
   if (E == 16) {
     if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 16>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
