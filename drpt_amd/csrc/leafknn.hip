@@ -1549,14 +1549,20 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
     for (int e = 0; e < E; ++e) key[e] = (e * 64 + lane < cnt_new) ? dn[e * 64 + lane] : ~0ull;
     int neval = cnt_new;
     if (A.rerank > 0 && cnt_new > A.rerank) {
-      if (cnt_new <= 64) {
-        // approximate bits (low 11 bits zero) | dense index: a 32-bit sort
-        uint32_t k1[1] = {lane < cnt_new ? (uint32_t)(key[0] >> 32) | (uint32_t)lane : 0xFFFFFFFFu};
-        bitonic_wave<uint32_t, 1>(k1, lane);
-        key[0] = lane < A.rerank ? (uint64_t)dn[k1[0] & 0x7FFu] : ~0ull;
-      } else {
-        bitonic_wave_prefix<uint64_t, E>(key, lane, cnt_new);
+      // approximate bits (low 11 bits zero) | dense index (< 64 E <= 1024): a
+      // 32-bit sort; ties in the approximate bits go by dense index (any
+      // choice among them is sound: the pooled passes re-find every leafmate
+      // that can still enter the top K)
+      uint32_t k32[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int j = e * 64 + lane;
+        k32[e] = j < cnt_new ? (uint32_t)(key[e] >> 32) | (uint32_t)j : 0xFFFFFFFFu;
       }
+      bitonic_wave_prefix<uint32_t, E>(k32, lane, cnt_new);
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        key[e] = e * 64 + lane < A.rerank ? dn[k32[e] & 0x7FFu] : ~0ull;
       neval = A.rerank;
     }
     // exact D, keys (D bits << 32 | id)
@@ -1574,6 +1580,29 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
     // merge the K best new entries (lanes < K of key[0]) with the known list
     const uint64_t nk = lane < K ? key[0] : ~0ull;
     const uint64_t kk = kid >= 0 ? (((uint64_t)__float_as_uint(kd) << 32) | (uint32_t)kid) : ~0ull;
+    int32_t* oi = A.out_idx + row * K;
+    float* od = A.out_dist + row * K;
+    if (K <= 32) {
+      // the known list ascending in lanes 0-31 and the new K best descending
+      // in lanes 32-63 form a bitonic sequence; its merge network sorts the
+      // union (distinct ids: known ids are never fresh) and lanes < K hold
+      // the K best
+      const uint64_t nrev = __shfl(nk, 63 - lane);
+      uint64_t v = lane < 32 ? kk : nrev;
+#pragma unroll
+      for (int j = 32; j > 0; j >>= 1) {
+        const uint64_t o = xor_lane(v, j);
+        const bool lower = (lane & j) == 0;
+        v = lower ? (o < v ? o : v) : (o < v ? v : o);
+      }
+      if (lane < K) {
+        oi[lane] = v == ~0ull ? -1 : (int32_t)(uint32_t)v;
+        od[lane] = v == ~0ull ? INFINITY : __uint_as_float((uint32_t)(v >> 32));
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      continue;
+    }
     int rn = lane, rk = lane;  // rank of my new entry / of my known entry
     for (int s = 0; s < K; ++s) {
       const uint64_t o_new = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(nk >> 32), s) << 32) |
@@ -1583,8 +1612,6 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
       rn += o_kn < nk ? 1 : 0;
       rk += o_new < kk ? 1 : 0;
     }
-    int32_t* oi = A.out_idx + row * K;
-    float* od = A.out_dist + row * K;
     if (lane < K && nk != ~0ull && rn < K) {
       oi[rn] = (int32_t)(uint32_t)nk;
       od[rn] = __uint_as_float((uint32_t)(nk >> 32));
