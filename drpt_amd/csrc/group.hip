@@ -44,6 +44,52 @@ __global__ void k_scatter(const uint32_t* __restrict__ heap, int64_t n, int HS,
   }
 }
 
+// LDS-private variants, for trees whose heap slots fit a workgroup's LDS (4 B
+// per slot): a block counts its contiguous range of points in LDS and adds
+// each touched slot's count to the global array with one atomic, instead of
+// one global atomic per point. The scatter counts again, reserves each
+// touched slot's range of positions with one global atomic, and places its
+// points through LDS cursors.
+constexpr int kGroupThreads = 1024;
+
+__global__ void __launch_bounds__(kGroupThreads)
+k_count_lds(const uint32_t* __restrict__ heap, int64_t n, int HS, int64_t ppb,
+            uint32_t* __restrict__ hcount) {
+  extern __shared__ uint32_t lc[];
+  const int t = blockIdx.y;
+  for (int k = threadIdx.x; k < HS; k += kGroupThreads) lc[k] = 0;
+  __syncthreads();
+  const uint32_t* h = heap + (int64_t)t * n;
+  const int64_t b0 = (int64_t)blockIdx.x * ppb, b1 = min(n, b0 + ppb);
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += kGroupThreads) atomicAdd(&lc[h[i]], 1u);
+  __syncthreads();
+  uint32_t* hc = hcount + (int64_t)t * HS;
+  for (int k = threadIdx.x; k < HS; k += kGroupThreads)
+    if (lc[k]) atomicAdd(&hc[k], lc[k]);
+}
+
+__global__ void __launch_bounds__(kGroupThreads)
+k_scatter_lds(const uint32_t* __restrict__ heap, int64_t n, int HS, int64_t ppb,
+              const int64_t* __restrict__ hoff, uint32_t* __restrict__ cursor,
+              int32_t* __restrict__ perm) {
+  extern __shared__ uint32_t lc[];
+  const int t = blockIdx.y;
+  for (int k = threadIdx.x; k < HS; k += kGroupThreads) lc[k] = 0;
+  __syncthreads();
+  const uint32_t* h = heap + (int64_t)t * n;
+  const int64_t b0 = (int64_t)blockIdx.x * ppb, b1 = min(n, b0 + ppb);
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += kGroupThreads) atomicAdd(&lc[h[i]], 1u);
+  __syncthreads();
+  const int64_t* ho = hoff + (int64_t)t * HS;
+  uint32_t* cu = cursor + (int64_t)t * HS;
+  for (int k = threadIdx.x; k < HS; k += kGroupThreads)
+    if (lc[k]) lc[k] = (uint32_t)(ho[k] + atomicAdd(&cu[k], lc[k]));  // first position (< n)
+  __syncthreads();
+  int32_t* pm = perm + (int64_t)t * n;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += kGroupThreads)
+    pm[atomicAdd(&lc[h[i]], 1u)] = (int32_t)i;
+}
+
 // Ascending sort of each segment in LDS (bitonic network, size padded to a
 // power of two with INT_MAX). One block per segment.
 constexpr int kSortMax = 8192;
@@ -230,11 +276,38 @@ unsigned grid_for(int64_t n) {
 
 }  // namespace
 
+// LDS bytes of the private variants' slot array when the device allows it
+// (beyond 64 KB of dynamic LDS with the function attribute raised), else 0
+static size_t group_lds_bytes(drpt_ctx* ctx, const void* f, int HS, int64_t n) {
+  const size_t bytes = (size_t)HS * 4;
+  if (n < 16384 || bytes > (size_t)ctx->max_lds) return 0;
+  if (bytes > 64 * 1024 &&
+      hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return bytes;
+}
+
+// blocks per tree of the private variants: about one per CU over all trees,
+// at least 4096 points each
+static int64_t group_ppb(drpt_ctx* ctx, int64_t n, int T) {
+  const int64_t bx = std::max<int64_t>(1, std::min<int64_t>((ctx->num_cus + T - 1) / T, n / 4096));
+  return (n + bx - 1) / bx;
+}
+
 int launch_count_leaves(drpt_ctx* ctx, const uint32_t* heap, int64_t n, int T, int D,
                         uint32_t* hcount) {
   const int HS = 1 << D;
   DRPT_CHECK(fill_u32(ctx, hcount, 0u, (size_t)T * HS));
-  hipLaunchKernelGGL(k_count, dim3(grid_for(n), T), dim3(256), 0, ctx->stream, heap, n, HS, hcount);
+  const size_t lds = group_lds_bytes(ctx, (const void*)k_count_lds, HS, n);
+  if (lds > 0) {
+    const int64_t ppb = group_ppb(ctx, n, T);
+    hipLaunchKernelGGL(k_count_lds, dim3((unsigned)((n + ppb - 1) / ppb), T), dim3(kGroupThreads),
+                       lds, ctx->stream, heap, n, HS, ppb, hcount);
+  } else {
+    hipLaunchKernelGGL(k_count, dim3(grid_for(n), T), dim3(256), 0, ctx->stream, heap, n, HS, hcount);
+  }
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }
@@ -243,8 +316,15 @@ int launch_scatter_leaves(drpt_ctx* ctx, const uint32_t* heap, int64_t n, int T,
                           const int64_t* hoff, uint32_t* cursor, int32_t* perm) {
   const int HS = 1 << D;
   DRPT_CHECK(fill_u32(ctx, cursor, 0u, (size_t)T * HS));
-  hipLaunchKernelGGL(k_scatter, dim3(grid_for(n), T), dim3(256), 0, ctx->stream, heap, n, HS, hoff,
-                     cursor, perm);
+  const size_t lds = group_lds_bytes(ctx, (const void*)k_scatter_lds, HS, n);
+  if (lds > 0) {
+    const int64_t ppb = group_ppb(ctx, n, T);
+    hipLaunchKernelGGL(k_scatter_lds, dim3((unsigned)((n + ppb - 1) / ppb), T),
+                       dim3(kGroupThreads), lds, ctx->stream, heap, n, HS, ppb, hoff, cursor, perm);
+  } else {
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for(n), T), dim3(256), 0, ctx->stream, heap, n, HS,
+                       hoff, cursor, perm);
+  }
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }
