@@ -148,7 +148,15 @@ int run(drpt_ctx* ctx, const XT* X, int64_t n, int d, const int32_t* colptr, con
     set_error("project: LDS tile too large for this dimension / column count");
     return DRPT_ERR_UNSUPPORTED;
   }
-  int64_t blocks = std::min<int64_t>((n + TR - 1) / TR, (int64_t)ctx->num_cus * 8);
+  // as many blocks as are resident at once (LDS bounds them): every block
+  // strides over the row tiles from the start, no late round of blocks
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_project<XT, TR, RL>, kThreads,
+                                                   lds) != hipSuccess || per_cu < 1) {
+    (void)hipGetLastError();
+    per_cu = 2;
+  }
+  int64_t blocks = std::min<int64_t>((n + TR - 1) / TR, (int64_t)ctx->num_cus * per_cu);
   hipLaunchKernelGGL((k_project<XT, TR, RL>), dim3((unsigned)blocks), dim3(kThreads), lds,
                      ctx->stream, X, n, d, L, colptr, rowidx, w, P, colmax, pitch);
   DRPT_HIP(hipGetLastError());
