@@ -59,6 +59,22 @@ __device__ __forceinline__ long long wave_sum_i64(long long v) {
   return v;
 }
 
+// Points per thread whose node and value words are loaded before any is used
+// (independent loads in flight; sums and counts do not depend on the order)
+constexpr int kUnroll = 4;
+
+__device__ __forceinline__ void load_points(const uint32_t* __restrict__ nt,
+                                            const float* __restrict__ Pc, int64_t base,
+                                            int64_t b1, uint32_t (&hu)[kUnroll],
+                                            float (&pu)[kUnroll]) {
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    const int64_t i = base + u * kThreads + threadIdx.x;
+    hu[u] = i < b1 ? nt[i] : 0u;
+    pu[u] = i < b1 ? Pc[i] : 0.0f;
+  }
+}
+
 // Points [b0, b1) of tree blockIdx.y handled by this block.
 __device__ __forceinline__ void block_range(int64_t n, int64_t ppb, int64_t& b0, int64_t& b1) {
   b0 = (int64_t)blockIdx.x * ppb;
@@ -89,40 +105,46 @@ k_sum1(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n
   int64_t b0, b1;
   block_range(n, ppb, b0, b1);
   const int lane = threadIdx.x & 63;
-  for (int64_t base = b0; base < b1; base += kThreads) {
-    int64_t i = base + threadIdx.x;
-    bool act = false;
-    int nd = 0;
-    long long q = 0;
-    if (i < b1) {
-      uint32_t h = nt[i];
-      act = (h >> l) == 1u;
-      nd = (int)(h - (uint32_t)NN);
-      if (act) q = quant(Pc[i], kq1);
-    }
-    unsigned long long am = __ballot(act);
-    if (am == 0) continue;
-    int leader = __ffsll((long long)am) - 1;
-    int ndl = __shfl(nd, leader);
-    unsigned long long same = __ballot(act && nd == ndl);
-    if (same == am) {
-      long long s = wave_sum_i64(q);
-      if (lane == leader) {
-        if (kLds) {
-          atomicAdd(&lc[ndl], (uint32_t)__popcll(am));
-          atomicAdd(&lS[ndl], (unsigned long long)s);
-        } else {
-          atomicAdd(&gc[ndl], (uint32_t)__popcll(am));
-          atomicAdd(&gS[ndl], (unsigned long long)s);
-        }
+  for (int64_t base = b0; base < b1; base += kThreads * kUnroll) {
+    uint32_t hu[kUnroll];
+    float pu[kUnroll];
+    load_points(nt, Pc, base, b1, hu, pu);
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int64_t i = base + u * kThreads + threadIdx.x;
+      bool act = false;
+      int nd = 0;
+      long long q = 0;
+      if (i < b1) {
+        uint32_t h = hu[u];
+        act = (h >> l) == 1u;
+        nd = (int)(h - (uint32_t)NN);
+        if (act) q = quant(pu[u], kq1);
       }
-    } else if (act) {
-      if (kLds) {
-        atomicAdd(&lc[nd], 1u);
-        atomicAdd(&lS[nd], (unsigned long long)q);
-      } else {
-        atomicAdd(&gc[nd], 1u);
-        atomicAdd(&gS[nd], (unsigned long long)q);
+      unsigned long long am = __ballot(act);
+      if (am == 0) continue;
+      int leader = __ffsll((long long)am) - 1;
+      int ndl = __shfl(nd, leader);
+      unsigned long long same = __ballot(act && nd == ndl);
+      if (same == am) {
+        long long s = wave_sum_i64(q);
+        if (lane == leader) {
+          if (kLds) {
+            atomicAdd(&lc[ndl], (uint32_t)__popcll(am));
+            atomicAdd(&lS[ndl], (unsigned long long)s);
+          } else {
+            atomicAdd(&gc[ndl], (uint32_t)__popcll(am));
+            atomicAdd(&gS[ndl], (unsigned long long)s);
+          }
+        }
+      } else if (act) {
+        if (kLds) {
+          atomicAdd(&lc[nd], 1u);
+          atomicAdd(&lS[nd], (unsigned long long)q);
+        } else {
+          atomicAdd(&gc[nd], 1u);
+          atomicAdd(&gS[nd], (unsigned long long)q);
+        }
       }
     }
   }
@@ -177,36 +199,42 @@ k_sum2(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n
   int64_t b0, b1;
   block_range(n, ppb, b0, b1);
   const int lane = threadIdx.x & 63;
-  for (int64_t base = b0; base < b1; base += kThreads) {
-    int64_t i = base + threadIdx.x;
-    bool act = false;
-    int nd = 0;
-    long long q = 0;
-    if (i < b1) {
-      uint32_t h = nt[i];
-      act = (h >> l) == 1u;
-      nd = (int)(h - (uint32_t)NN);
-      if (act) act = st[nd] != 0u;
-      if (act) {
-        float w = __fsub_rn(Pc[i], mt[nd]);
-        float w2 = __fmul_rn(w, w);
-        q = quant(w2, kq2);
+  for (int64_t base = b0; base < b1; base += kThreads * kUnroll) {
+    uint32_t hu[kUnroll];
+    float pu[kUnroll];
+    load_points(nt, Pc, base, b1, hu, pu);
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int64_t i = base + u * kThreads + threadIdx.x;
+      bool act = false;
+      int nd = 0;
+      long long q = 0;
+      if (i < b1) {
+        uint32_t h = hu[u];
+        act = (h >> l) == 1u;
+        nd = (int)(h - (uint32_t)NN);
+        if (act) act = st[nd] != 0u;
+        if (act) {
+          float w = __fsub_rn(pu[u], mt[nd]);
+          float w2 = __fmul_rn(w, w);
+          q = quant(w2, kq2);
+        }
       }
-    }
-    unsigned long long am = __ballot(act);
-    if (am == 0) continue;
-    int leader = __ffsll((long long)am) - 1;
-    int ndl = __shfl(nd, leader);
-    unsigned long long same = __ballot(act && nd == ndl);
-    if (same == am) {
-      long long s = wave_sum_i64(q);
-      if (lane == leader) {
-        if (kLds) atomicAdd(&lS[ndl], (unsigned long long)s);
-        else atomicAdd(&gS[ndl], (unsigned long long)s);
+      unsigned long long am = __ballot(act);
+      if (am == 0) continue;
+      int leader = __ffsll((long long)am) - 1;
+      int ndl = __shfl(nd, leader);
+      unsigned long long same = __ballot(act && nd == ndl);
+      if (same == am) {
+        long long s = wave_sum_i64(q);
+        if (lane == leader) {
+          if (kLds) atomicAdd(&lS[ndl], (unsigned long long)s);
+          else atomicAdd(&gS[ndl], (unsigned long long)s);
+        }
+      } else if (act) {
+        if (kLds) atomicAdd(&lS[nd], (unsigned long long)q);
+        else atomicAdd(&gS[nd], (unsigned long long)q);
       }
-    } else if (act) {
-      if (kLds) atomicAdd(&lS[nd], (unsigned long long)q);
-      else atomicAdd(&gS[nd], (unsigned long long)q);
     }
   }
   if (kLds) {
@@ -296,46 +324,52 @@ k_hist(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n
   int64_t b0, b1;
   block_range(n, ppb, b0, b1);
   const int lane = threadIdx.x & 63;
-  for (int64_t base = b0; base < b1; base += kThreads) {
-    int64_t i = base + threadIdx.x;
-    bool act = false;
-    int nd = 0, b = 0;
-    if (i < b1) {
-      uint32_t h = nt[i];
-      act = (h >> l) == 1u;
-      nd = (int)(h - (uint32_t)NN);
-      if (act) act = st[nd] != 0u;
-      if (act) {
-        float e[16];
-        const float4* src = reinterpret_cast<const float4*>(Et + (int64_t)nd * 16);
-        float4 a0 = src[0], a1 = src[1], a2 = src[2], a3 = src[3];
-        e[0] = a0.x; e[1] = a0.y; e[2] = a0.z; e[3] = a0.w;
-        e[4] = a1.x; e[5] = a1.y; e[6] = a1.z; e[7] = a1.w;
-        e[8] = a2.x; e[9] = a2.y; e[10] = a2.z; e[11] = a2.w;
-        e[12] = a3.x; e[13] = a3.y; e[14] = a3.z; e[15] = a3.w;
-        b = slot_of(Pc[i], e);
-      }
-    }
-    unsigned long long am = __ballot(act);
-    if (am == 0) continue;
-    int leader = __ffsll((long long)am) - 1;
-    int ndl = __shfl(nd, leader);
-    unsigned long long same = __ballot(act && nd == ndl);
-    if (same == am) {
-      // wave-ballot histogram: lane k < 16 adds the population of slot k
-      uint32_t mine = 0;
+  for (int64_t base = b0; base < b1; base += kThreads * kUnroll) {
+    uint32_t hu[kUnroll];
+    float pu[kUnroll];
+    load_points(nt, Pc, base, b1, hu, pu);
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        uint32_t c = (uint32_t)__popcll(__ballot(act && b == k));
-        mine = (lane == k) ? c : mine;
+    for (int u = 0; u < kUnroll; ++u) {
+      const int64_t i = base + u * kThreads + threadIdx.x;
+      bool act = false;
+      int nd = 0, b = 0;
+      if (i < b1) {
+        uint32_t h = hu[u];
+        act = (h >> l) == 1u;
+        nd = (int)(h - (uint32_t)NN);
+        if (act) act = st[nd] != 0u;
+        if (act) {
+          float e[16];
+          const float4* src = reinterpret_cast<const float4*>(Et + (int64_t)nd * 16);
+          float4 a0 = src[0], a1 = src[1], a2 = src[2], a3 = src[3];
+          e[0] = a0.x; e[1] = a0.y; e[2] = a0.z; e[3] = a0.w;
+          e[4] = a1.x; e[5] = a1.y; e[6] = a1.z; e[7] = a1.w;
+          e[8] = a2.x; e[9] = a2.y; e[10] = a2.z; e[11] = a2.w;
+          e[12] = a3.x; e[13] = a3.y; e[14] = a3.z; e[15] = a3.w;
+          b = slot_of(pu[u], e);
+        }
       }
-      if (lane < 16 && mine) {
-        if (kLds) atomicAdd(&lH[ndl * 16 + lane], mine);
-        else atomicAdd(&gH[(int64_t)ndl * 16 + lane], mine);
+      unsigned long long am = __ballot(act);
+      if (am == 0) continue;
+      int leader = __ffsll((long long)am) - 1;
+      int ndl = __shfl(nd, leader);
+      unsigned long long same = __ballot(act && nd == ndl);
+      if (same == am) {
+        // wave-ballot histogram: lane k < 16 adds the population of slot k
+        uint32_t mine = 0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          uint32_t c = (uint32_t)__popcll(__ballot(act && b == k));
+          mine = (lane == k) ? c : mine;
+        }
+        if (lane < 16 && mine) {
+          if (kLds) atomicAdd(&lH[ndl * 16 + lane], mine);
+          else atomicAdd(&gH[(int64_t)ndl * 16 + lane], mine);
+        }
+      } else if (act) {
+        if (kLds) atomicAdd(&lH[nd * 16 + b], 1u);
+        else atomicAdd(&gH[(int64_t)nd * 16 + b], 1u);
       }
-    } else if (act) {
-      if (kLds) atomicAdd(&lH[nd * 16 + b], 1u);
-      else atomicAdd(&gH[(int64_t)nd * 16 + b], 1u);
     }
   }
   if (kLds) {
