@@ -1430,8 +1430,9 @@ __device__ __forceinline__ float exact_dist8(const XT* __restrict__ xr, const XT
 }
 
 constexpr int kMergeWaves = 4;    // rows in flight per block (one per wave)
+constexpr int kMergeRowLds = 1024; // elements of a row staged in LDS per wave (16 KB per block)
 
-template <typename XT, int E>
+template <typename XT, int E, bool RL>
 __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeArgs A) {
   // per-wave open-addressing table for distinct ids: more slots than the K
   // known ids + 64 E insertions, so probing ends
@@ -1439,6 +1440,10 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
   constexpr int kHashSlots = 1 << kHashBits;
   __shared__ uint32_t tab[kMergeWaves][kHashSlots];
   __shared__ uint64_t dense[kMergeWaves][64 * E];
+  // RL: the row staged in LDS for its exact distances (a separate instantiation:
+  // shorter rows keep the registers and LDS of the plain kernel)
+  constexpr bool kRowLds = RL;
+  __shared__ __align__(16) float qrow[kMergeWaves][kRowLds ? kMergeRowLds : 4];
   const XT* __restrict__ X = static_cast<const XT*>(A.X);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int K = A.K, d = A.d;
@@ -1567,13 +1572,45 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
     }
     // exact D, keys (D bits << 32 | id)
     const XT* xr = X + row * d;
+    if (kRowLds) {
+      // the row staged once in LDS as float: every lane reads it by broadcast
+      // instead of loading it again beside its candidate (same fma chain)
+      float* qf = qrow[w];
+      for (int k = lane * 4; k < d; k += 256) *reinterpret_cast<float4*>(&qf[k]) = load4f(xr + k);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      if (e * 64 + lane < neval) {
-        const uint32_t id = (uint32_t)key[e];
-        key[e] = ((uint64_t)__float_as_uint(exact_dist8(xr, X + (int64_t)id * d, d)) << 32) | id;
-      } else {
-        key[e] = ~0ull;
+      for (int e = 0; e < E; ++e) {
+        if (e * 64 + lane < neval) {
+          const uint32_t id = (uint32_t)key[e];
+          const XT* xc = X + (int64_t)id * d;
+          float acc = 0.0f;
+#pragma unroll 8
+          for (int k = 0; k < d; k += 4) {
+            const float4 a = *reinterpret_cast<const float4*>(&qf[k]), b = load4f(xc + k);
+            float t = __fsub_rn(a.x, b.x);
+            acc = fmaf(t, t, acc);
+            t = __fsub_rn(a.y, b.y);
+            acc = fmaf(t, t, acc);
+            t = __fsub_rn(a.z, b.z);
+            acc = fmaf(t, t, acc);
+            t = __fsub_rn(a.w, b.w);
+            acc = fmaf(t, t, acc);
+          }
+          key[e] = ((uint64_t)__float_as_uint(acc) << 32) | id;
+        } else {
+          key[e] = ~0ull;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        if (e * 64 + lane < neval) {
+          const uint32_t id = (uint32_t)key[e];
+          key[e] = ((uint64_t)__float_as_uint(exact_dist8(xr, X + (int64_t)id * d, d)) << 32) | id;
+        } else {
+          key[e] = ~0ull;
+        }
       }
     }
     bitonic_wave_prefix<uint64_t, E>(key, lane, neval);
@@ -2065,6 +2102,19 @@ int launch_leaf_gather_val(drpt_ctx* ctx, const int32_t* perm, int64_t total, co
   return DRPT_OK;
 }
 
+// the row staged in LDS for rows of 128 to kMergeRowLds elements (multiples
+// of 4) at E <= 4: measured faster there (1M x 128, 1M x 960), slower at d = 96
+template <typename XT, int E>
+void launch_rm(drpt_ctx* ctx, unsigned grid, const RowMergeArgs& a) {
+  if constexpr (E <= 4) {
+    if ((a.d & 3) == 0 && a.d >= 128 && a.d <= kMergeRowLds) {
+      hipLaunchKernelGGL((k_row_merge<XT, E, true>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_row_merge<XT, E, false>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+}
+
 int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a) {
   const int slots = a.S * a.ntsrc;
   if (a.K < 1 || a.K > 64 || slots > 1024 || a.rerank > 64 * 4 || a.ntsrc > 64 ||
@@ -2085,8 +2135,8 @@ int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a) {
     DRPT_CHECK(fill_u32(ctx, (uint32_t*)a.nheavy, 0u, 1));
     RowMergeArgs l = a;
     l.nrows_dev = nullptr;
-    if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 4>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, l);
-    else hipLaunchKernelGGL((k_row_merge<float, 4>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, l);
+    if (u8) launch_rm<uint8_t, 4>(ctx, grid, l);
+    else launch_rm<float, 4>(ctx, grid, l);
     DRPT_HIP(hipGetLastError());
     a.order = a.heavy;
     a.nrows_dev = a.nheavy;
@@ -2098,20 +2148,20 @@ int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a) {
   }
   const int E = slots <= 64 ? 1 : slots <= 128 ? 2 : slots <= 256 ? 4 : slots <= 512 ? 8 : 16;
   if (E == 16) {
-    if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 16>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
-    else hipLaunchKernelGGL((k_row_merge<float, 16>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+    if (u8) launch_rm<uint8_t, 16>(ctx, grid, a);
+    else launch_rm<float, 16>(ctx, grid, a);
   } else if (E == 8) {
-    if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 8>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
-    else hipLaunchKernelGGL((k_row_merge<float, 8>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+    if (u8) launch_rm<uint8_t, 8>(ctx, grid, a);
+    else launch_rm<float, 8>(ctx, grid, a);
   } else if (E == 1) {
-    if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 1>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
-    else hipLaunchKernelGGL((k_row_merge<float, 1>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+    if (u8) launch_rm<uint8_t, 1>(ctx, grid, a);
+    else launch_rm<float, 1>(ctx, grid, a);
   } else if (E == 2) {
-    if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 2>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
-    else hipLaunchKernelGGL((k_row_merge<float, 2>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+    if (u8) launch_rm<uint8_t, 2>(ctx, grid, a);
+    else launch_rm<float, 2>(ctx, grid, a);
   } else {
-    if (u8) hipLaunchKernelGGL((k_row_merge<uint8_t, 4>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
-    else hipLaunchKernelGGL((k_row_merge<float, 4>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+    if (u8) launch_rm<uint8_t, 4>(ctx, grid, a);
+    else launch_rm<float, 4>(ctx, grid, a);
   }
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
