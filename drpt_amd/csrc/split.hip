@@ -419,14 +419,23 @@ k_part(const float* __restrict__ P, uint32_t* __restrict__ node, int64_t n, int 
   uint32_t* nt = node + (int64_t)t * n;
   const float* mt = med + (int64_t)t * NN;
   const uint32_t* st = split + (int64_t)t * NN;
-  int64_t stride = (int64_t)gridDim.x * kThreads;
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
-    uint32_t h = nt[i];
-    if ((h >> l) != 1u) continue;
-    int nd = (int)(h - (uint32_t)NN);
-    if (!st[nd]) continue;
-    float v = Pc[i];
-    nt[i] = 2u * h + ((v <= mt[nd]) ? 0u : 1u);
+  // block ranges of kUnroll * kThreads points; each thread loads its points'
+  // node and value words before using any
+  const int64_t span = (int64_t)kThreads * kUnroll;
+  for (int64_t base = (int64_t)blockIdx.x * span; base < n; base += (int64_t)gridDim.x * span) {
+    const int64_t b1 = min(n, base + span);
+    uint32_t hu[kUnroll];
+    float pu[kUnroll];
+    load_points(nt, Pc, base, b1, hu, pu);
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int64_t i = base + u * kThreads + threadIdx.x;
+      const uint32_t h = hu[u];
+      if (i >= b1 || (h >> l) != 1u) continue;
+      const int nd = (int)(h - (uint32_t)NN);
+      if (!st[nd]) continue;
+      nt[i] = 2u * h + ((pu[u] <= mt[nd]) ? 0u : 1u);
+    }
   }
 }
 
