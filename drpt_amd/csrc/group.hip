@@ -44,50 +44,80 @@ __global__ void k_scatter(const uint32_t* __restrict__ heap, int64_t n, int HS,
   }
 }
 
-// LDS-private variants, for trees whose heap slots fit a workgroup's LDS (4 B
-// per slot): a block counts its contiguous range of points in LDS and adds
-// each touched slot's count to the global array with one atomic, instead of
-// one global atomic per point. The scatter counts again, reserves each
-// touched slot's range of positions with one global atomic, and places its
-// points through LDS cursors.
+// LDS-private variants: heap slots [blockIdx.z * CH, + CH) of tree blockIdx.y
+// in LDS (4 B each; CH a power of two, the whole tree when it fits). A block
+// counts its contiguous range of points in LDS and adds each touched slot's
+// count to the global array with one atomic, instead of one global atomic per
+// point; trees with more slots take several chunks, each reading the block's
+// points again and skipping those of other chunks. The scatter counts again,
+// reserves each touched slot's range of positions with one global atomic, and
+// places its points through LDS cursors. Each thread loads kGroupLoads heap
+// words before using any (loads in flight behind the LDS atomics).
 constexpr int kGroupThreads = 1024;
+constexpr int kGroupLoads = 8;
+
+__device__ __forceinline__ void count_range(const uint32_t* __restrict__ h, int64_t b0, int64_t b1,
+                                            uint32_t lo, int CH, uint32_t* lc) {
+  for (int64_t base = b0; base < b1; base += (int64_t)kGroupThreads * kGroupLoads) {
+    uint32_t hv[kGroupLoads];
+#pragma unroll
+    for (int u = 0; u < kGroupLoads; ++u) {
+      const int64_t i = base + u * kGroupThreads + threadIdx.x;
+      hv[u] = i < b1 ? h[i] - lo : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int u = 0; u < kGroupLoads; ++u)
+      if (hv[u] < (uint32_t)CH) atomicAdd(&lc[hv[u]], 1u);
+  }
+}
 
 __global__ void __launch_bounds__(kGroupThreads)
-k_count_lds(const uint32_t* __restrict__ heap, int64_t n, int HS, int64_t ppb,
+k_count_lds(const uint32_t* __restrict__ heap, int64_t n, int HS, int CH, int64_t ppb,
             uint32_t* __restrict__ hcount) {
   extern __shared__ uint32_t lc[];
   const int t = blockIdx.y;
-  for (int k = threadIdx.x; k < HS; k += kGroupThreads) lc[k] = 0;
+  const uint32_t lo = (uint32_t)blockIdx.z * (uint32_t)CH;
+  for (int k = threadIdx.x; k < CH; k += kGroupThreads) lc[k] = 0;
   __syncthreads();
-  const uint32_t* h = heap + (int64_t)t * n;
   const int64_t b0 = (int64_t)blockIdx.x * ppb, b1 = min(n, b0 + ppb);
-  for (int64_t i = b0 + threadIdx.x; i < b1; i += kGroupThreads) atomicAdd(&lc[h[i]], 1u);
+  count_range(heap + (int64_t)t * n, b0, b1, lo, CH, lc);
   __syncthreads();
-  uint32_t* hc = hcount + (int64_t)t * HS;
-  for (int k = threadIdx.x; k < HS; k += kGroupThreads)
+  uint32_t* hc = hcount + (int64_t)t * HS + lo;
+  for (int k = threadIdx.x; k < CH; k += kGroupThreads)
     if (lc[k]) atomicAdd(&hc[k], lc[k]);
 }
 
 __global__ void __launch_bounds__(kGroupThreads)
-k_scatter_lds(const uint32_t* __restrict__ heap, int64_t n, int HS, int64_t ppb,
+k_scatter_lds(const uint32_t* __restrict__ heap, int64_t n, int HS, int CH, int64_t ppb,
               const int64_t* __restrict__ hoff, uint32_t* __restrict__ cursor,
               int32_t* __restrict__ perm) {
   extern __shared__ uint32_t lc[];
   const int t = blockIdx.y;
-  for (int k = threadIdx.x; k < HS; k += kGroupThreads) lc[k] = 0;
+  const uint32_t lo = (uint32_t)blockIdx.z * (uint32_t)CH;
+  for (int k = threadIdx.x; k < CH; k += kGroupThreads) lc[k] = 0;
   __syncthreads();
   const uint32_t* h = heap + (int64_t)t * n;
   const int64_t b0 = (int64_t)blockIdx.x * ppb, b1 = min(n, b0 + ppb);
-  for (int64_t i = b0 + threadIdx.x; i < b1; i += kGroupThreads) atomicAdd(&lc[h[i]], 1u);
+  count_range(h, b0, b1, lo, CH, lc);
   __syncthreads();
-  const int64_t* ho = hoff + (int64_t)t * HS;
-  uint32_t* cu = cursor + (int64_t)t * HS;
-  for (int k = threadIdx.x; k < HS; k += kGroupThreads)
+  const int64_t* ho = hoff + (int64_t)t * HS + lo;
+  uint32_t* cu = cursor + (int64_t)t * HS + lo;
+  for (int k = threadIdx.x; k < CH; k += kGroupThreads)
     if (lc[k]) lc[k] = (uint32_t)(ho[k] + atomicAdd(&cu[k], lc[k]));  // first position (< n)
   __syncthreads();
   int32_t* pm = perm + (int64_t)t * n;
-  for (int64_t i = b0 + threadIdx.x; i < b1; i += kGroupThreads)
-    pm[atomicAdd(&lc[h[i]], 1u)] = (int32_t)i;
+  for (int64_t base = b0; base < b1; base += (int64_t)kGroupThreads * kGroupLoads) {
+    uint32_t hv[kGroupLoads];
+#pragma unroll
+    for (int u = 0; u < kGroupLoads; ++u) {
+      const int64_t i = base + u * kGroupThreads + threadIdx.x;
+      hv[u] = i < b1 ? h[i] - lo : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int u = 0; u < kGroupLoads; ++u)
+      if (hv[u] < (uint32_t)CH)
+        pm[atomicAdd(&lc[hv[u]], 1u)] = (int32_t)(base + u * kGroupThreads + threadIdx.x);
+  }
 }
 
 // Ascending sort of each segment in LDS (bitonic network, size padded to a
@@ -276,23 +306,27 @@ unsigned grid_for(int64_t n) {
 
 }  // namespace
 
-// LDS bytes of the private variants' slot array when the device allows it
-// (beyond 64 KB of dynamic LDS with the function attribute raised), else 0
-static size_t group_lds_bytes(drpt_ctx* ctx, const void* f, int HS, int64_t n) {
-  const size_t bytes = (size_t)HS * 4;
-  if (n < 16384 || bytes > (size_t)ctx->max_lds) return 0;
-  if (bytes > 64 * 1024 &&
-      hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
+// heap slots per LDS chunk of the private variants (a power of two, at most
+// HS, within the device's LDS; the function attribute raised beyond 64 KB; at
+// most 8 chunks), else 0: global atomics
+static int group_chunk(drpt_ctx* ctx, const void* f, int HS, int64_t n) {
+  if (n < 16384) return 0;
+  int ch = HS;
+  while (ch > 4096 && (size_t)ch * 4 > (size_t)ctx->max_lds) ch >>= 1;
+  if ((size_t)ch * 4 > (size_t)ctx->max_lds || HS / ch > 8) return 0;
+  if ((size_t)ch * 4 > 64 * 1024 &&
+      hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, ch * 4) != hipSuccess) {
     (void)hipGetLastError();
     return 0;
   }
-  return bytes;
+  return ch;
 }
 
-// blocks per tree of the private variants: about one per CU over all trees,
-// at least 4096 points each
-static int64_t group_ppb(drpt_ctx* ctx, int64_t n, int T) {
-  const int64_t bx = std::max<int64_t>(1, std::min<int64_t>((ctx->num_cus + T - 1) / T, n / 4096));
+// points per block of the private variants: about one block per CU over all
+// trees and chunks, at least 4096 points each
+static int64_t group_ppb(drpt_ctx* ctx, int64_t n, int T, int nch) {
+  const int64_t bx = std::max<int64_t>(
+      1, std::min<int64_t>((ctx->num_cus + T * nch - 1) / (T * nch), n / 4096));
   return (n + bx - 1) / bx;
 }
 
@@ -300,11 +334,13 @@ int launch_count_leaves(drpt_ctx* ctx, const uint32_t* heap, int64_t n, int T, i
                         uint32_t* hcount) {
   const int HS = 1 << D;
   DRPT_CHECK(fill_u32(ctx, hcount, 0u, (size_t)T * HS));
-  const size_t lds = group_lds_bytes(ctx, (const void*)k_count_lds, HS, n);
-  if (lds > 0) {
-    const int64_t ppb = group_ppb(ctx, n, T);
-    hipLaunchKernelGGL(k_count_lds, dim3((unsigned)((n + ppb - 1) / ppb), T), dim3(kGroupThreads),
-                       lds, ctx->stream, heap, n, HS, ppb, hcount);
+  const int ch = group_chunk(ctx, (const void*)k_count_lds, HS, n);
+  if (ch > 0) {
+    const int nch = HS / ch;
+    const int64_t ppb = group_ppb(ctx, n, T, nch);
+    hipLaunchKernelGGL(k_count_lds, dim3((unsigned)((n + ppb - 1) / ppb), T, nch),
+                       dim3(kGroupThreads), (size_t)ch * 4, ctx->stream, heap, n, HS, ch, ppb,
+                       hcount);
   } else {
     hipLaunchKernelGGL(k_count, dim3(grid_for(n), T), dim3(256), 0, ctx->stream, heap, n, HS, hcount);
   }
@@ -316,11 +352,16 @@ int launch_scatter_leaves(drpt_ctx* ctx, const uint32_t* heap, int64_t n, int T,
                           const int64_t* hoff, uint32_t* cursor, int32_t* perm) {
   const int HS = 1 << D;
   DRPT_CHECK(fill_u32(ctx, cursor, 0u, (size_t)T * HS));
-  const size_t lds = group_lds_bytes(ctx, (const void*)k_scatter_lds, HS, n);
-  if (lds > 0) {
-    const int64_t ppb = group_ppb(ctx, n, T);
-    hipLaunchKernelGGL(k_scatter_lds, dim3((unsigned)((n + ppb - 1) / ppb), T),
-                       dim3(kGroupThreads), lds, ctx->stream, heap, n, HS, ppb, hoff, cursor, perm);
+  // one chunk only: with several, the LDS scatter measured slower than the
+  // global cursors (10M x 96, 2^18 slots in 8 chunks: 21.0 against 12.4 ms)
+  int ch = group_chunk(ctx, (const void*)k_scatter_lds, HS, n);
+  if (ch > 0 && ch < HS) ch = 0;
+  if (ch > 0) {
+    const int nch = HS / ch;
+    const int64_t ppb = group_ppb(ctx, n, T, nch);
+    hipLaunchKernelGGL(k_scatter_lds, dim3((unsigned)((n + ppb - 1) / ppb), T, nch),
+                       dim3(kGroupThreads), (size_t)ch * 4, ctx->stream, heap, n, HS, ch, ppb,
+                       hoff, cursor, perm);
   } else {
     hipLaunchKernelGGL(k_scatter, dim3(grid_for(n), T), dim3(256), 0, ctx->stream, heap, n, HS,
                        hoff, cursor, perm);
