@@ -39,7 +39,7 @@ constexpr int kThreads = 256;
 // int64 sum; sum2: 8), histograms 64 (16 slots); used while they fit
 // kSumLdsMax / kHistLdsMax and the device's per-workgroup LDS
 constexpr size_t kSumLdsMax = 96 * 1024;
-constexpr size_t kHistLdsMax = 64 * 1024;
+constexpr size_t kHistLdsMax = 128 * 1024;
 
 __device__ __forceinline__ void col_exponents(uint32_t cm, int bitlen, int& kq1, int& kq2) {
   int E = (int)((cm >> 23) & 0xffu);
@@ -475,10 +475,13 @@ int grow_trees_impl(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64
   // LDS sums beyond 64 KB of dynamic LDS where the device allows it
   if (!ctx->split_lds_big && ctx->max_lds > 64 * 1024) {
     const int bytes = (int)std::min<size_t>((size_t)ctx->max_lds, kSumLdsMax);
+    const int hbytes = (int)std::min<size_t>((size_t)ctx->max_lds, kHistLdsMax);
     if (hipFuncSetAttribute((const void*)k_sum1<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             bytes) == hipSuccess &&
         hipFuncSetAttribute((const void*)k_sum2<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            bytes) == hipSuccess)
+                            bytes) == hipSuccess &&
+        hipFuncSetAttribute((const void*)k_hist<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            hbytes) == hipSuccess)
       ctx->split_lds_big = true;
     else
       (void)hipGetLastError();
@@ -515,7 +518,9 @@ int split_levels(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t 
     const size_t lds_max = std::min<size_t>((size_t)ctx->max_lds,
                                             ctx->split_lds_big ? kSumLdsMax : 64 * 1024);
     bool lds_sum = (size_t)NN * 12 <= lds_max;
-    bool lds_hist = (size_t)NN * 64 <= std::min(lds_max, kHistLdsMax);
+    const size_t hist_max = ctx->split_lds_big ? std::min<size_t>((size_t)ctx->max_lds, kHistLdsMax)
+                                               : std::min<size_t>((size_t)ctx->max_lds, 64 * 1024);
+    bool lds_hist = (size_t)NN * 64 <= hist_max;
     // points per block: with per-node LDS arrays, enough points to amortise their
     // flush; with global atomics, small blocks so every CU has work
     const int64_t ppb_sum = lds_sum ? std::max<int64_t>(8192, NN * 8) : 8192;
