@@ -131,6 +131,22 @@ class Context:
         self.sync()
         return heap
 
+    def grow_trees_noalloc(self, P, colmax, ntrees, depth, tree_mode=DRPT_TREE_FIXED, leaf_max=0,
+                           out=None):
+        """drpt_grow_trees_noalloc: grow_trees with the split state in the
+        library's fixed device array (no device allocation by the library)."""
+        torch = _torch()
+        L, n = P.shape
+        heap = out if out is not None else torch.empty((ntrees, n), dtype=torch.int32,
+                                                       device=P.device)
+        if heap.shape != (ntrees, n) or heap.dtype != torch.int32 or not heap.is_contiguous():
+            raise ValueError("out must be a contiguous int32 tensor of shape (ntrees, n)")
+        torch.cuda.synchronize(P.device)
+        _lib.check(self.lib.drpt_grow_trees_noalloc(self.h, _ptr(P), _ptr(colmax), n, ntrees,
+                                                    depth, tree_mode, leaf_max, _ptr(heap)))
+        self.sync()
+        return heap
+
     def tree_knn(self, params, X, heap):
         torch = _torch()
         T, n, K = params.ntrees, params.n, params.nn

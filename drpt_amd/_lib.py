@@ -27,7 +27,7 @@ EXPORTS = [
     "drpt_last_error", "drpt_version", "drpt_create", "drpt_destroy", "drpt_sync",
     "drpt_get_stream", "drpt_derive_params", "drpt_build_projection", "drpt_project",
     "drpt_grow_trees", "drpt_tree_knn", "drpt_merge", "drpt_build_graph", "drpt_autotune",
-    "drpt_set_item_plan",
+    "drpt_set_item_plan", "drpt_grow_trees_noalloc",
 ]
 
 
@@ -97,6 +97,7 @@ def load():
     L.drpt_build_projection.argtypes = [i32, i32, f32, u32, vp]
     L.drpt_project.argtypes = [vp, vp, i32, i64, i32, vp, i32, vp, vp]
     L.drpt_grow_trees.argtypes = [vp, vp, vp, i64, i32, i32, i32, i32, vp]
+    L.drpt_grow_trees_noalloc.argtypes = [vp, vp, vp, i64, i32, i32, i32, i32, vp]
     L.drpt_tree_knn.argtypes = [vp, ctypes.POINTER(Params), vp, vp, vp, vp, ctypes.POINTER(Timings)]
     L.drpt_merge.argtypes = [vp, i64, i32, i32, vp, vp, vp, vp]
     L.drpt_build_graph.argtypes = [vp, ctypes.POINTER(Params), vp, vp, vp, ctypes.POINTER(Timings)]

@@ -143,6 +143,7 @@ int drpt_create(drpt_ctx** out, int device) {
     return DRPT_ERR_ARG;
   }
   DRPT_HIP(hipSetDevice(device));
+  DRPT_CHECK(noalloc_setup(device));
   drpt_ctx* ctx = new drpt_ctx();
   ctx->device = device;
   hipDeviceProp_t prop;

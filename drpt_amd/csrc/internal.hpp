@@ -89,6 +89,9 @@ int launch_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, co
                    uint32_t* colmax);
 
 // split.hip
+// device address of the allocation-free entry points' scratch array on
+// `device` (current), looked up once per device by drpt_create
+int noalloc_setup(int device);
 int grow_trees_impl(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t n, int T, int D,
                     int mode, int leaf_max, uint32_t* node);
 

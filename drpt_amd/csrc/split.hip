@@ -28,6 +28,9 @@
 //
 // Roofline: HBM-bound; algorithmic bytes 12 per point per tree per level
 // (4 B value + 4 B node read + 4 B node write).
+#include <cmath>
+#include <mutex>
+
 #include "internal.hpp"
 
 namespace drpt {
@@ -452,12 +455,35 @@ int bitlen_of(int64_t n) {
 
 }  // namespace
 
-int grow_trees_impl(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t n, int T, int D,
-                    int mode, int leaf_max, uint32_t* node) {
+// LDS sums and histograms beyond 64 KB of dynamic LDS where the device allows
+// it (a function attribute, set once per context)
+static void raise_split_lds(drpt_ctx* ctx) {
+  if (!ctx->split_lds_big && ctx->max_lds > 64 * 1024) {
+    const int bytes = (int)std::min<size_t>((size_t)ctx->max_lds, kSumLdsMax);
+    const int hbytes = (int)std::min<size_t>((size_t)ctx->max_lds, kHistLdsMax);
+    if (hipFuncSetAttribute((const void*)k_sum1<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            bytes) == hipSuccess &&
+        hipFuncSetAttribute((const void*)k_sum2<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            bytes) == hipSuccess &&
+        hipFuncSetAttribute((const void*)k_hist<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            hbytes) == hipSuccess)
+      ctx->split_lds_big = true;
+    else
+      (void)hipGetLastError();
+  }
+}
+
+static int check_tree_args(int64_t n, int T, int D) {
   if (D < 1 || D > 24 || T < 1 || n < 1) {
     set_error("grow_trees: depth must be in [1, 24], ntrees >= 1, n >= 1");
     return DRPT_ERR_ARG;
   }
+  return DRPT_OK;
+}
+
+int grow_trees_impl(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t n, int T, int D,
+                    int mode, int leaf_max, uint32_t* node) {
+  DRPT_CHECK(check_tree_args(n, T, D));
   if (D == 1) return fill_u32(ctx, node, 1u, (size_t)T * n);
   const int64_t NMAX = (int64_t)1 << (D - 2);
   uint32_t *cnt, *split, *H;
@@ -472,20 +498,7 @@ int grow_trees_impl(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64
   DRPT_CHECK(ws(ctx, "split.E", T * NMAX * 16, &E));
   DRPT_CHECK(ws(ctx, "split.H", T * NMAX * 16, &H));
   const int bitlen = bitlen_of(n);
-  // LDS sums beyond 64 KB of dynamic LDS where the device allows it
-  if (!ctx->split_lds_big && ctx->max_lds > 64 * 1024) {
-    const int bytes = (int)std::min<size_t>((size_t)ctx->max_lds, kSumLdsMax);
-    const int hbytes = (int)std::min<size_t>((size_t)ctx->max_lds, kHistLdsMax);
-    if (hipFuncSetAttribute((const void*)k_sum1<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            bytes) == hipSuccess &&
-        hipFuncSetAttribute((const void*)k_sum2<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            bytes) == hipSuccess &&
-        hipFuncSetAttribute((const void*)k_hist<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            hbytes) == hipSuccess)
-      ctx->split_lds_big = true;
-    else
-      (void)hipGetLastError();
-  }
+  raise_split_lds(ctx);
   // the per-level launches (4 fills + 7 kernels per level) as one hipGraph,
   // replayed while sizes, parameters and buffers stay the same
   char key[512];
@@ -568,3 +581,99 @@ int split_levels(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t 
 
 }  // namespace
 }  // namespace drpt
+
+// ---- allocation-free variant of drpt_grow_trees -----------------------------
+// The per-node state of the splits (counts, fixed-point sums, mean, edge
+// table, histogram, median: 160 B per node and tree at the deepest split
+// level) lives in this file-scope device array instead of the context's
+// workspace, for callers that cannot allocate device memory (code under
+// stream capture, serving loops). Each GPU the library runs on holds its own
+// copy until the process ends, and every call on that GPU uses it: calls that
+// overlap on one GPU (several contexts or streams) race. Host code never uses
+// the array's name as an address: drpt_create looks up the device address
+// once per device (noalloc_setup) and the entry point passes that pointer.
+constexpr size_t kNoallocScratchBytes = (size_t)32 << 20;
+// trees x nodes of the deepest split level whose state fits the array
+constexpr size_t kNoallocMaxNodes = 196608;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace drpt {
+
+namespace {
+constexpr int kNoallocDevices = 64;
+void* g_scratch_ptr[kNoallocDevices];
+std::mutex g_scratch_mu;
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
+int noalloc_setup(int device) {
+  if (device < 0 || device >= kNoallocDevices) {
+    set_error("device index beyond the " + std::to_string(kNoallocDevices) +
+              " devices the allocation-free entry points track");
+    return DRPT_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return DRPT_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr) {
+    (void)hipGetLastError();
+    set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    return DRPT_ERR_HIP;
+  }
+  g_scratch_ptr[device] = ptr;
+  return DRPT_OK;
+}
+
+}  // namespace drpt
+
+extern "C" int drpt_grow_trees_noalloc(drpt_ctx* ctx, const float* P, const uint32_t* colmax,
+                                       int64_t n, int ntrees, int depth, int tree_mode,
+                                       int leaf_max, uint32_t* leaf_heap) {
+  using namespace drpt;
+  if (!ctx || !P || !colmax || !leaf_heap) {
+    set_error("grow_trees_noalloc: null pointer");
+    return DRPT_ERR_ARG;
+  }
+  if (depth <= 0 || (tree_mode == DRPT_TREE_FIXED && depth > std::log2((double)n))) {
+    set_error("depth should be in range [1,....,log2(rows)]");
+    return DRPT_ERR_DEPTH;
+  }
+  DRPT_CHECK(check_tree_args(n, ntrees, depth));
+  DRPT_HIP(hipSetDevice(ctx->device));
+  if (depth == 1) return fill_u32(ctx, leaf_heap, 1u, (size_t)ntrees * n);
+  const size_t TN = (size_t)ntrees << (depth - 2);
+  // [cnt u32 | flag u32 | S1 u64 | S2 u64 | mu f32 | med f32 | E f32 x16 | H u32 x16]
+  const size_t off_flag = align256(TN * 4);
+  const size_t off_S1 = off_flag + align256(TN * 4);
+  const size_t off_S2 = off_S1 + align256(TN * 8);
+  const size_t off_mu = off_S2 + align256(TN * 8);
+  const size_t off_med = off_mu + align256(TN * 4);
+  const size_t off_E = off_med + align256(TN * 4);
+  const size_t off_H = off_E + align256(TN * 64);
+  const size_t total = off_H + align256(TN * 64);
+  if (TN > kNoallocMaxNodes || total > kNoallocScratchBytes) {
+    set_error("grow_trees_noalloc: " + std::to_string(ntrees) + " trees of depth " +
+              std::to_string(depth) + " need " + std::to_string(total) +
+              " bytes of split state, more than the " + std::to_string(kNoallocScratchBytes) +
+              " bytes this entry point holds (ntrees * 2^(depth-2) must be <= " +
+              std::to_string(kNoallocMaxNodes) + "); use "
+              "drpt_grow_trees");
+    return DRPT_ERR_ARG;
+  }
+  unsigned char* base =
+      ctx->device < kNoallocDevices ? (unsigned char*)g_scratch_ptr[ctx->device] : nullptr;
+  if (!base) {
+    set_error("grow_trees_noalloc: no scratch address for this device (context not set up)");
+    return DRPT_ERR_HIP;
+  }
+  raise_split_lds(ctx);
+  // launched eagerly on the context's stream (a caller may be capturing it);
+  // split_levels clears every per-level array it accumulates into
+  return split_levels(ctx, P, colmax, n, ntrees, depth, tree_mode, leaf_max, bitlen_of(n),
+                      leaf_heap, (uint32_t*)base, (uint32_t*)(base + off_flag),
+                      (uint32_t*)(base + off_H), (unsigned long long*)(base + off_S1),
+                      (unsigned long long*)(base + off_S2), (float*)(base + off_mu),
+                      (float*)(base + off_E), (float*)(base + off_med));
+}

@@ -106,6 +106,20 @@ int drpt_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, cons
 int drpt_grow_trees(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t n, int ntrees,
                     int depth, int tree_mode, int leaf_max, uint32_t* leaf_heap);
 
+/* Allocation-free variant of drpt_grow_trees: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * stream capture, serving loops, harnesses that forbid hipMalloc). Same
+ * arguments and results as drpt_grow_trees (bit for bit); the per-node split
+ * state lives in a 32 MiB device array of the library instead of the
+ * context's workspace, so ntrees * 2^(depth-2) must be <= 196608 (else
+ * DRPT_ERR_ARG), and the launches are issued eagerly on the context's stream
+ * (no hipGraph of its own). The array is one per GPU and holds the last
+ * call's state: calls that overlap on one GPU (several contexts or streams
+ * at once) race. */
+int drpt_grow_trees_noalloc(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t n,
+                            int ntrees, int depth, int tree_mode, int leaf_max,
+                            uint32_t* leaf_heap);
+
 /* Stages local + (c): from leaf_heap (device), form the final leaves
  * (reference mode: MDRPT::grow_local_trees, mdrpt.cpp:321-400 with
  * drpt_local.cpp; leaf-max mode: the leaves as grown) and compute, per tree
