@@ -285,9 +285,9 @@ struct Grouping {
   int64_t host_leaves = 0, host_max = 0;
 };
 
-static int group_impl(drpt_ctx* ctx, const drpt_params* p, const uint32_t* leaf_heap,
-                      const ItemPlan& plan, int first_trees, Grouping* G,
-                      bool items_only = false) {
+static int group_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
+                      const uint32_t* leaf_heap, const ItemPlan& plan, int first_trees,
+                      Grouping* G, bool items_only = false) {
   const int T = p->ntrees, D = p->depth, K = p->nn;
   const int64_t n = p->n;
   const int64_t HS = (int64_t)1 << D;
@@ -355,16 +355,26 @@ static int group_impl(drpt_ctx* ctx, const drpt_params* p, const uint32_t* leaf_
                 std::to_string(gmin) + "): depth should be in range [1,....,log2(rows)]");
       return DRPT_ERR_DEPTH;  // drpt_local.cpp:84-87
     }
+    // the local matrices (seed S+1+t, the int-truncated density,
+    // mdrpt.cpp:348-352): all zero for a density below 1 (the libstdc++ tie
+    // permutation), non-zero for -density >= 1 (the real-key local tree)
     const float dint = (float)(int)p->density;
-    std::vector<float> C((size_t)p->d * ld);
+    const int LL = T * ld;
+    std::vector<float> C((size_t)p->d * ld), Call((size_t)p->d * LL);
+    int nreal = 0;
     for (int t = 0; t < T; ++t) {
       host_build_projection(p->d, ld, dint, p->seed + 1u + (uint32_t)t, C.data());
-      for (float c : C)
-        if (c != 0.0f) {
-          set_error("reference local stage with a non-zero local projection is not supported");
-          return DRPT_ERR_UNSUPPORTED;
-        }
+      bool real = false;
+      for (float c : C) real = real || c != 0.0f;
+      nreal += real;
+      for (int j = 0; j < p->d; ++j)
+        for (int l = 0; l < ld; ++l) Call[(size_t)j * LL + t * ld + l] = C[(size_t)j * ld + l];
     }
+    if (nreal != 0 && nreal != T) {
+      set_error("local stage: some trees have an all-zero local projection and others not");
+      return DRPT_ERR_UNSUPPORTED;
+    }
+    const bool real_local = nreal == T;
     std::vector<int64_t> seg_start;
     std::vector<int32_t> seg_size, seg_tree;
     std::vector<int64_t> seg_tbl;
@@ -376,14 +386,16 @@ static int group_impl(drpt_ctx* ctx, const drpt_params* p, const uint32_t* leaf_
       seg_start.push_back(l.start);
       seg_size.push_back(l.size);
       seg_tree.push_back(l.tree);
-      auto it = tbl_of.find(l.size);
-      if (it == tbl_of.end()) {
-        const int64_t off = (int64_t)tables.size();
-        tables.resize(tables.size() + l.size);
-        host_local_positions(l.size, ld, tables.data() + off);
-        it = tbl_of.emplace(l.size, off).first;
+      if (!real_local) {
+        auto it = tbl_of.find(l.size);
+        if (it == tbl_of.end()) {
+          const int64_t off = (int64_t)tables.size();
+          tables.resize(tables.size() + l.size);
+          host_local_positions(l.size, ld, tables.data() + off);
+          it = tbl_of.emplace(l.size, off).first;
+        }
+        seg_tbl.push_back(it->second);
       }
-      seg_tbl.push_back(it->second);
       host_local_sizes(l.size, ld, sizes.data());
       int64_t acc = l.start;
       for (int32_t sz : sizes) {
@@ -391,19 +403,69 @@ static int group_impl(drpt_ctx* ctx, const drpt_params* p, const uint32_t* leaf_
         acc += sz;
       }
     }
+    // level-by-level segments of the real-key local tree: at depth l each
+    // global leaf holds 2^l nodes with the ceil/floor halving sizes
+    std::vector<int64_t> lv_start, lv_ofs(1, 0);
+    std::vector<int32_t> lv_size, lv_tree;
+    if (real_local) {
+      std::vector<int32_t> hs((size_t)1 << ld);
+      for (int l = 0; l < ld; ++l) {
+        for (auto& lf : leaves) {
+          host_local_sizes(lf.size, l, hs.data());
+          int64_t acc = lf.start;
+          for (int k = 0; k < (1 << l); ++k) {
+            if (hs[k] > 1) {
+              lv_start.push_back(acc);
+              lv_size.push_back(hs[k]);
+              lv_tree.push_back(lf.tree);
+            }
+            acc += hs[k];
+          }
+        }
+        lv_ofs.push_back((int64_t)lv_start.size());
+      }
+    }
     int64_t* dss;
-    int32_t *dsz, *dst, *dtb;
-    int64_t* dtbo;
+    int32_t *dsz, *dst;
     DRPT_CHECK(upload(ctx, "local.start", seg_start, &dss));
     DRPT_CHECK(upload(ctx, "local.size", seg_size, &dsz));
     DRPT_CHECK(upload(ctx, "local.tree", seg_tree, &dst));
-    DRPT_CHECK(upload(ctx, "local.tblofs", seg_tbl, &dtbo));
-    DRPT_CHECK(upload(ctx, "local.tables", tables, &dtb));
-    DRPT_CHECK(launch_sort_segments(ctx, perm, n, dss, dsz, dst, (int)leaves.size(), max_seg));
-    int32_t* perm2;
-    DRPT_CHECK(ws(ctx, "group.perm2", T * n, &perm2));
-    DRPT_CHECK(launch_apply_local(ctx, perm, perm2, n, dss, dst, dtbo, dsz, (int)leaves.size(), dtb));
-    perm_final = perm2;
+    if (!real_local) {
+      int32_t* dtb;
+      int64_t* dtbo;
+      DRPT_CHECK(upload(ctx, "local.tblofs", seg_tbl, &dtbo));
+      DRPT_CHECK(upload(ctx, "local.tables", tables, &dtb));
+      DRPT_CHECK(launch_sort_segments(ctx, perm, n, dss, dsz, dst, (int)leaves.size(), max_seg));
+      int32_t* perm2;
+      DRPT_CHECK(ws(ctx, "group.perm2", T * n, &perm2));
+      DRPT_CHECK(
+          launch_apply_local(ctx, perm, perm2, n, dss, dst, dtbo, dsz, (int)leaves.size(), dtb));
+      perm_final = perm2;
+    } else {
+      // local projections of every point, LP[t * ld + l][i]
+      // (MDRPT::grow_local_trees' multiply_mat per leaf, mdrpt.cpp:365-369)
+      float* LP;
+      uint32_t* lcm;
+      DRPT_CHECK(ws(ctx, "local.P", (size_t)LL * n, &LP));
+      DRPT_CHECK(ws(ctx, "local.colmax", LL, &lcm));
+      DRPT_CHECK(drpt_project(ctx, X, p->dtype, n, p->d, Call.data(), LL, LP, lcm));
+      std::vector<uint32_t> cmh(LL);
+      DRPT_HIP(hipMemcpyAsync(cmh.data(), lcm, 4 * (size_t)LL, hipMemcpyDeviceToHost, s));
+      DRPT_HIP(hipStreamSynchronize(s));
+      for (uint32_t b : cmh)
+        if (b >= 0x7f800000u) {
+          set_error("non-finite local projected values");
+          return DRPT_ERR_NONFINITE;
+        }
+      int64_t* dls;
+      int32_t *dlz, *dlt;
+      DRPT_CHECK(upload(ctx, "local.lv_start", lv_start, &dls));
+      DRPT_CHECK(upload(ctx, "local.lv_size", lv_size, &dlz));
+      DRPT_CHECK(upload(ctx, "local.lv_tree", lv_tree, &dlt));
+      for (int l = 0; l < ld; ++l)
+        DRPT_CHECK(launch_sort_keyed(ctx, perm, n, LP, ld, l, dls + lv_ofs[l], dlz + lv_ofs[l],
+                                     dlt + lv_ofs[l], (int)(lv_ofs[l + 1] - lv_ofs[l]), max_seg));
+    }
     // work items in tree order (tree 0 first), largest leaves first inside a tree
     std::stable_sort(final_leaves.begin(), final_leaves.end(), [](const Leaf& a, const Leaf& b) {
       if (a.tree != b.tree) return a.tree < b.tree;
@@ -525,7 +587,7 @@ static int tree_knn_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
   hipStream_t s = ctx->stream;
   DRPT_HIP(hipEventRecord(ctx->ev[6], s));
   Grouping G;
-  DRPT_CHECK(group_impl(ctx, p, leaf_heap, uniform_plan(64), T, &G));
+  DRPT_CHECK(group_impl(ctx, p, X, leaf_heap, uniform_plan(64), T, &G));
   drpt_params pp = *p;
   pp.kprime = kprime_of(p);
   int32_t *fb = nullptr, *nfb = nullptr;
@@ -667,7 +729,7 @@ static int lowp_setup(drpt_ctx* ctx, const drpt_params* p, const void* X,
                                   256 / r->KP));
   r->pool_cap = pool_cap_of(r->K);
   const int64_t n = r->n;
-  DRPT_CHECK(group_impl(ctx, p, leaf_heap, plan, r->first, &r->G));
+  DRPT_CHECK(group_impl(ctx, p, X, leaf_heap, plan, r->first, &r->G));
   DRPT_CHECK(prep_lowp(ctx, &r->pp, X, r->G.perm, &r->a));
   if (r->pc.precision != r->pp.precision) {
     DRPT_CHECK(prep_lowp(ctx, &r->pc, X, r->G.perm, &r->ac));
@@ -961,7 +1023,7 @@ int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* p
         ItemPlan cand = best;
         cand.rpi[b] = rpis[ir];
         cand.ring[b] = ring;
-        DRPT_CHECK(group_impl(ctx, p, heap, cand, r.first, &r.G, true));
+        DRPT_CHECK(group_impl(ctx, p, X, heap, cand, r.first, &r.G, true));
         double ms = 0.0;
         for (int rep = 0; rep < 3; ++rep) {
           DRPT_CHECK(fill_u32(ctx, (uint32_t*)r.pcnt, 0u, r.n));

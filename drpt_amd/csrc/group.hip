@@ -154,6 +154,61 @@ k_sort_segments(int32_t* __restrict__ perm, int64_t n, const int64_t* __restrict
   for (int i = threadIdx.x; i < m; i += blockDim.x) p[i] = buf[i];
 }
 
+// Real-key local tree level (-density >= 1: non-zero local projections):
+// each segment of perm is sorted by (value, point index), value =
+// LP[col(tree)][point], so its first ceil(m/2) positions hold the left child of
+// drpt_local.cpp:121-161 (nth_element at m/2, left = [begin, end - m/2)). Keys
+// are 64-bit: the float's order-preserving bits (-0 taken as +0, so equal
+// values tie and the index decides) above the index. One block per segment,
+// bitonic network in LDS (size padded to a power of two).
+constexpr int kKeyedSortMax = 8192;
+
+__device__ __forceinline__ uint32_t ordered_bits(float v) {
+  uint32_t b = __float_as_uint(v);
+  if (b == 0x80000000u) b = 0u;
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__global__ void __launch_bounds__(1024)
+k_sort_keyed(int32_t* __restrict__ perm, int64_t n, const float* __restrict__ LP, int ld, int level,
+             const int64_t* __restrict__ seg_start, const int32_t* __restrict__ seg_size,
+             const int32_t* __restrict__ seg_tree) {
+  __shared__ unsigned long long buf[kKeyedSortMax];
+  const int s = blockIdx.x;
+  const int m = seg_size[s];
+  const int t = seg_tree[s];
+  int32_t* p = perm + (int64_t)t * n + seg_start[s];
+  const float* v = LP + ((int64_t)t * ld + level) * n;
+  int P2 = 1;
+  while (P2 < m) P2 <<= 1;
+  for (int i = threadIdx.x; i < P2; i += blockDim.x) {
+    if (i < m) {
+      const int32_t id = p[i];
+      buf[i] = ((unsigned long long)ordered_bits(v[id]) << 32) | (uint32_t)id;
+    } else {
+      buf[i] = ~0ull;
+    }
+  }
+  __syncthreads();
+  for (int k = 2; k <= P2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < P2; i += blockDim.x) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const bool up = (i & k) == 0;
+          const unsigned long long a = buf[i], b = buf[ixj];
+          if ((a > b) == up) {
+            buf[i] = b;
+            buf[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < m; i += blockDim.x) p[i] = (int32_t)(uint32_t)buf[i];
+}
+
 // out[start + k] = in[start + table[k]] for each global leaf.
 __global__ void k_apply_local(const int32_t* __restrict__ in, int32_t* __restrict__ out, int64_t n,
                               const int64_t* __restrict__ seg_start,
@@ -380,6 +435,21 @@ int launch_sort_segments(drpt_ctx* ctx, int32_t* perm, int64_t n, const int64_t*
   }
   hipLaunchKernelGGL(k_sort_segments, dim3(nseg), dim3(1024), 0, ctx->stream, perm, n, seg_start,
                      seg_size, seg_tree);
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
+}
+
+int launch_sort_keyed(drpt_ctx* ctx, int32_t* perm, int64_t n, const float* LP, int ld, int level,
+                      const int64_t* seg_start, const int32_t* seg_size, const int32_t* seg_tree,
+                      int nseg, int max_seg) {
+  if (nseg <= 0) return DRPT_OK;
+  if (max_seg > kKeyedSortMax) {
+    set_error("local random projection: a global leaf exceeds " + std::to_string(kKeyedSortMax) +
+              " points (LDS sort capacity)");
+    return DRPT_ERR_UNSUPPORTED;
+  }
+  hipLaunchKernelGGL(k_sort_keyed, dim3(nseg), dim3(1024), 0, ctx->stream, perm, n, LP, ld, level,
+                     seg_start, seg_size, seg_tree);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }

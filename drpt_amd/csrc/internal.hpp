@@ -157,6 +157,11 @@ int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, con
 int launch_tree_range(drpt_ctx* ctx, const int64_t* base, int t0, int t1, int64_t* range);
 int launch_sort_segments(drpt_ctx* ctx, int32_t* perm, int64_t n, const int64_t* seg_start,
                          const int32_t* seg_size, const int32_t* seg_tree, int nseg, int max_seg);
+// sorts each segment of perm by (LP[seg_tree * ld + level][point], point):
+// one level of the real-key local tree
+int launch_sort_keyed(drpt_ctx* ctx, int32_t* perm, int64_t n, const float* LP, int ld, int level,
+                      const int64_t* seg_start, const int32_t* seg_size, const int32_t* seg_tree,
+                      int nseg, int max_seg);
 int launch_apply_local(drpt_ctx* ctx, const int32_t* perm_in, int32_t* perm_out, int64_t n,
                        const int64_t* seg_start, const int32_t* seg_tree, const int64_t* seg_tbl,
                        const int32_t* seg_size, int nseg, const int32_t* tables);

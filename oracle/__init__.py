@@ -37,6 +37,7 @@ def lib():
         L.orc_grow_tree.argtypes = [vp, i64, i32, i32, i32, i32, i32, vp, vp, vp]
         L.orc_local_perm.argtypes = [i32, i32, vp]
         L.orc_local_sizes.argtypes = [i32, i32, vp]
+        L.orc_local_split.argtypes = [vp, i64, i32, vp, i32]
         L.orc_local_C_nonzeros.argtypes = [i32, i32, f32, u32]
         L.orc_dist.argtypes = [vp, i32, i32, i64, i64]
         L.orc_dist.restype = f32
@@ -117,6 +118,15 @@ def local_perm(m, ld):
 def local_sizes(m, ld):
     out = np.zeros(1 << ld, np.int32)
     lib().orc_local_sizes(m, ld, _p(out))
+    return out
+
+
+def local_split(LP, pts, ld):
+    """Real-key local tree of one global leaf: LP [ld][n] column-major, pts the
+    leaf's point ids; returns them rearranged into local leaves."""
+    LP = np.ascontiguousarray(LP, np.float32)
+    out = np.ascontiguousarray(pts, np.int32).copy()
+    lib().orc_local_split(_p(LP), LP.shape[1], ld, _p(out), len(out))
     return out
 
 

@@ -323,6 +323,40 @@ void orc_local_sizes(int m, int ld, int32_t* out) {
   for (size_t i = 0; i < s.size(); ++i) out[i] = s[i];
 }
 
+// ----------------------------------------------------------------------------
+// True local random projection: with -density k >= 1 (main.cpp:69-72 parses
+// an integer) the int density handed to grow_local_trees is k >= 1
+// (mdrpt.cpp:321-400), so the local matrix C (d x ld, math_operations.cpp:48-76
+// with seed S+1+t) is non-zero and the local tree splits on real keys.
+// Reference: drpt_local.cpp:121-161 -- at depth l a node of s positions is
+// rearranged by std::nth_element(begin, begin + s/2, end, less-by-LP[l]) and
+// its left child is [begin, end - s/2): the ceil(s/2) smallest values; the
+// right child the rest; the left child is recursed first.
+// CANONICAL: ties in LP are broken by point index (the reference leaves them
+// to libstdc++'s introselect), and each node is fully sorted (only the sets
+// matter downstream). LP is column-major [ld][n] (P of the local matrix),
+// pts the m points of one global leaf; on return each local leaf is
+// contiguous, in the order and sizes of orc_local_sizes.
+// ----------------------------------------------------------------------------
+static void local_split_rec(const float* LP, int64_t n, int32_t* b, int32_t* e, int depth,
+                            int ld) {
+  if (depth == ld) return;
+  const float* v = LP + (size_t)depth * n;
+  std::sort(b, e, [v](int32_t x, int32_t y) {
+    if (v[x] < v[y]) return true;
+    if (v[y] < v[x]) return false;
+    return x < y;
+  });
+  const int64_t size = e - b;
+  int32_t* mid = e - size / 2;
+  local_split_rec(LP, n, b, mid, depth + 1, ld);
+  local_split_rec(LP, n, mid, e, depth + 1, ld);
+}
+
+void orc_local_split(const float* LP, int64_t n, int ld, int32_t* pts, int m) {
+  local_split_rec(LP, n, pts, pts + m, 0, ld);
+}
+
 // Local matrix check: the reference generates C with the truncated density
 // (mdrpt.cpp:348-352); returns the number of non-zeros (0 in practice).
 int orc_local_C_nonzeros(int d, int ld, float density_int, uint32_t seed) {
@@ -598,20 +632,34 @@ int orc_build_graph(const orc_params* pp, const void* X, int32_t* out_idx, float
     int ld = orc_local_depth(gmin, K, p.local_offset);
     if (ld <= 0) return -2;  // reference throws (drpt_local.cpp:84-87)
     float dint = (float)(int)p.density;
-    for (int t = 0; t < T; ++t)
-      if (orc_local_C_nonzeros(p.d, ld, dint, p.seed + 1u + (uint32_t)t) != 0) return -3;
     int nl = 1 << ld;
+    std::vector<float> C((size_t)p.d * ld), LP, Rt;
+    std::vector<uint32_t> lcm(ld);
     for (int t = 0; t < T; ++t) {
+      // the tree's local matrix: all zero (density < 1 truncated to 0) gives
+      // the libstdc++ tie permutation, non-zero the real-key local tree
+      orc_build_R(p.d, ld, dint, p.seed + 1u + (uint32_t)t, C.data());
+      bool real = false;
+      for (float c : C) real = real || c != 0.0f;
+      if (real) {
+        LP.resize((size_t)ld * n);
+        orc_project(X, p.x_is_u8, n, p.d, C.data(), ld, LP.data(), lcm.data());
+      }
       std::vector<int32_t> npm(n);
       std::vector<int64_t> nof;
       std::vector<int32_t> tbl, sz(nl);
       for (size_t g = 0; g + 1 < offs[t].size(); ++g) {
         int64_t o = offs[t][g];
         int m = (int)(offs[t][g + 1] - o);
-        tbl.resize(m);
-        orc_local_perm(m, ld, tbl.data());
         orc_local_sizes(m, ld, sz.data());
-        for (int k = 0; k < m; ++k) npm[o + k] = perms[t][o + tbl[k]];
+        if (real) {
+          for (int k = 0; k < m; ++k) npm[o + k] = perms[t][o + k];
+          orc_local_split(LP.data(), n, ld, npm.data() + o, m);
+        } else {
+          tbl.resize(m);
+          orc_local_perm(m, ld, tbl.data());
+          for (int k = 0; k < m; ++k) npm[o + k] = perms[t][o + tbl[k]];
+        }
         int64_t acc = o;
         for (int s = 0; s < nl; ++s) {
           if (sz[s] > 0) nof.push_back(acc);

@@ -182,3 +182,143 @@ def grow_tree(P, colmax, D, t, mode, leaf_max, stat_mode):
     for i, pts in enumerate(cur):
         out[pts] = (1 << (D - 1)) + i
     return out
+
+
+# ---- libstdc++ std::nth_element (bits/stl_algo.h, GCC 4.9 - 13) -------------
+# The reference local tree (cpp/src/drpt/algo/drpt_local.cpp:121-161) calls
+# std::nth_element(begin, begin + n/2, end, less-by-projected-value). In its
+# default configuration every projected value is 0 (mdrpt.cpp:348-352 receive
+# an int density of 0), so the comparator is always false and the leaves are
+# whatever positions libstdc++'s introselect leaves there. This restates that
+# introselect step by step on a Python list, with the comparator as a function
+# of two elements: __introselect with depth limit 2*floor(log2 n), the
+# median-of-three pivot moved to the front (__move_median_to_first), the
+# unguarded Hoare partition, __heap_select for an exhausted depth limit and
+# the final insertion sort of a range of at most 3.
+def _lg(n):
+    return n.bit_length() - 1
+
+
+def _move_median_to_first(a, result, i, j, k, less):
+    if less(a[i], a[j]):
+        if less(a[j], a[k]):
+            a[result], a[j] = a[j], a[result]
+        elif less(a[i], a[k]):
+            a[result], a[k] = a[k], a[result]
+        else:
+            a[result], a[i] = a[i], a[result]
+    elif less(a[i], a[k]):
+        a[result], a[i] = a[i], a[result]
+    elif less(a[j], a[k]):
+        a[result], a[k] = a[k], a[result]
+    else:
+        a[result], a[j] = a[j], a[result]
+
+
+def _unguarded_partition(a, first, last, pivot, less):
+    while True:
+        while less(a[first], a[pivot]):
+            first += 1
+        last -= 1
+        while less(a[pivot], a[last]):
+            last -= 1
+        if not first < last:
+            return first
+        a[first], a[last] = a[last], a[first]
+        first += 1
+
+
+def _adjust_heap(a, base, hole, length, value, less):
+    top = hole
+    child = hole
+    while child < (length - 1) // 2:
+        child = 2 * (child + 1)
+        if less(a[base + child], a[base + child - 1]):
+            child -= 1
+        a[base + hole] = a[base + child]
+        hole = child
+    if length % 2 == 0 and child == (length - 2) // 2:
+        child = 2 * (child + 1)
+        a[base + hole] = a[base + child - 1]
+        hole = child - 1
+    parent = (hole - 1) // 2
+    while hole > top and less(a[base + parent], value):
+        a[base + hole] = a[base + parent]
+        hole = parent
+        parent = (hole - 1) // 2
+    a[base + hole] = value
+
+
+def _make_heap(a, first, last, less):
+    length = last - first
+    if length < 2:
+        return
+    parent = (length - 2) // 2
+    while True:
+        _adjust_heap(a, first, parent, length, a[first + parent], less)
+        if parent == 0:
+            return
+        parent -= 1
+
+
+def _heap_select(a, first, middle, last, less):
+    _make_heap(a, first, middle, less)
+    for i in range(middle, last):
+        if less(a[i], a[first]):
+            value = a[i]
+            a[i] = a[first]
+            _adjust_heap(a, first, 0, middle - first, value, less)
+
+
+def _insertion_sort(a, first, last, less):
+    for i in range(first + 1, last):
+        v = a[i]
+        if less(v, a[first]):
+            a[first + 1:i + 1] = a[first:i]
+            a[first] = v
+        else:
+            j = i
+            while less(v, a[j - 1]):
+                a[j] = a[j - 1]
+                j -= 1
+            a[j] = v
+
+
+def nth_element(a, first, nth, last, less):
+    """std::nth_element(a+first, a+nth, a+last, less) on list a, in place."""
+    if first == last or nth == last:
+        return
+    depth = 2 * _lg(last - first)
+    while last - first > 3:
+        if depth == 0:
+            _heap_select(a, first, nth + 1, last, less)
+            a[first], a[nth] = a[nth], a[first]
+            return
+        depth -= 1
+        mid = first + (last - first) // 2
+        _move_median_to_first(a, first, first + 1, mid, last - 1, less)
+        cut = _unguarded_partition(a, first + 1, last, first, less)
+        if cut <= nth:
+            first = cut
+        else:
+            last = cut
+    _insertion_sort(a, first, last, less)
+
+
+def local_positions(m, ld, less=lambda x, y: False):
+    """Positions of a reference local tree (drpt_local.cpp:121-161): at each
+    level nth_element(begin, begin + size/2, end), then the left child is
+    [begin, end - size/2) and the right child [end - size/2, end)."""
+    a = list(range(m))
+
+    def rec(b, e, depth):
+        if depth == ld:
+            return
+        size = e - b
+        nth_element(a, b, b + size // 2, e, less)
+        mid = e - size // 2
+        rec(b, mid, depth + 1)
+        rec(mid, e, depth + 1)
+
+    rec(0, m, 0)
+    return a

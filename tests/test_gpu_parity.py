@@ -283,6 +283,30 @@ def test_build_graph_vs_oracle(ctx, orc, case, mode):
     assert np.array_equal(_bits(od.cpu().numpy()), _bits(rd))
 
 
+@pytest.mark.parametrize("case,T,prec", [("mnist_u8_3000", 4, "fp32"), ("int128_f32_4000", 3, "fp32"),
+                                         ("mnist_u8_3000", 5, "bf16"), ("gauss_d37_1500", 4, "fp8")])
+def test_build_graph_real_local_tree_vs_oracle(ctx, orc, case, T, prec):
+    """-density 1 (an integer density, main.cpp:69-72): dense R and a dense
+    local matrix per tree, so the reference local tree splits on real keys
+    (drpt_local.cpp:121-161); the GPU's segmented (value, index) sorts give the
+    oracle's local leaves and lists bit for bit."""
+    _log(f"[build_graph real local] {case} T={T} {prec}")
+    X = CASES[case]()
+    n, d = X.shape
+    p = drpt_amd.derive_params(n, d, T, 10)
+    p.seed = 13
+    p.density = 1.0
+    p.precision = drpt_amd.PRECISIONS[prec]
+    p.dtype = drpt_amd.DRPT_U8 if X.dtype == np.uint8 else drpt_amd.DRPT_F32
+    oi, od, tm = ctx.build_graph(p, _dev(X))
+    ri, rd, _, _ = orc.build_graph(X, T, p.depth, 10, local_mode=0, seed=13, density=1.0)
+    assert np.array_equal(oi.cpu().numpy(), ri)
+    assert np.array_equal(_bits(od.cpu().numpy()), _bits(rd))
+    r0, _, _, _ = orc.build_graph(X, T, p.depth, 10, local_mode=0, seed=13,
+                                  density=1.0 / np.sqrt(d))
+    assert not np.array_equal(ri, r0)  # the local tree really changed
+
+
 def test_duplicates_and_tiny_leaves(ctx, orc):
     _log("[edge] duplicates / tiny leaves")
     rng = np.random.default_rng(9)

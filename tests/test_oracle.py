@@ -205,6 +205,109 @@ def test_local_sizes_and_permutation(orc, m, ld):
     assert np.array_equal(p, orc.local_perm(m, ld))
 
 
+def test_local_perm_known_answer(orc):
+    # a 8-point leaf split 3 levels deep on all-equal keys, as the reference's
+    # DRPTLocal arranges it with GCC 11.4's libstdc++
+    assert orc.local_perm(8, 3).tolist() == [6, 5, 4, 7, 0, 3, 2, 1]
+    assert refpy.local_positions(8, 3) == [6, 5, 4, 7, 0, 3, 2, 1]
+
+
+def _local_cases():
+    rng = np.random.default_rng(14)
+    ms = list(range(1, 400)) + sorted(rng.integers(400, 5001, size=60).tolist()) + [938, 1953, 4883]
+    for m in ms:
+        for ld in range(1, 6):
+            if (1 << ld) <= m:
+                yield m, ld
+
+
+def test_local_perm_matches_introselect_restatement(orc):
+    """orc_local_perm (the libstdc++ call) against refpy's step-by-step
+    restatement of libstdc++'s introselect, for every leaf size below 400 and
+    a sample up to 5000, local depths 1-5."""
+    for m, ld in _local_cases():
+        assert orc.local_perm(m, ld).tolist() == refpy.local_positions(m, ld), (m, ld)
+
+
+def test_nth_element_restatement_selects():
+    """The restatement is a selection on real keys too: the nth element is in
+    place, smaller ones before it, larger after (a plausible slip in the
+    partition or the median-of-three breaks this)."""
+    rng = np.random.default_rng(15)
+    for _ in range(400):
+        m = int(rng.integers(1, 300))
+        v = rng.integers(0, 25, size=m).tolist()
+        a = list(range(m))
+        k = int(rng.integers(0, m))
+        refpy.nth_element(a, 0, k, m, lambda x, y: v[x] < v[y])
+        assert sorted(a) == list(range(m))
+        assert v[a[k]] == sorted(v)[k]
+        assert all(v[a[i]] <= v[a[k]] for i in range(k))
+        assert all(v[a[i]] >= v[a[k]] for i in range(k + 1, m))
+
+
+@pytest.mark.parametrize("m,ld,levels", [(7, 2, 3), (64, 3, 3), (117, 3, 4), (500, 4, 4),
+                                         (1000, 5, 5)])
+def test_local_split_real_keys_matches_reference_recursion(orc, m, ld, levels):
+    """Real-key local tree (-density >= 1): the oracle's leaves equal the sets
+    the reference recursion (drpt_local.cpp:121-161: nth_element at size/2,
+    left = [begin, end - size/2), left first) gives with the restated
+    introselect, keys ordered by (value, point index); values drawn from a few
+    integers so that ties are frequent."""
+    rng = np.random.default_rng(m + ld)
+    n = m + 50
+    LP = rng.integers(0, levels, size=(ld, n)).astype(np.float32)
+    pts = np.sort(rng.choice(n, size=m, replace=False)).astype(np.int32)
+    got = orc.local_split(LP, pts, ld)
+    a = pts.tolist()
+
+    def rec(b, e, depth):
+        if depth == ld:
+            return
+        size = e - b
+        less = lambda x, y: (LP[depth, x], x) < (LP[depth, y], y)
+        refpy.nth_element(a, b, b + size // 2, e, less)
+        mid = e - size // 2
+        rec(b, mid, depth + 1)
+        rec(mid, e, depth + 1)
+
+    rec(0, m, 0)
+    sizes = orc.local_sizes(m, ld)
+    o = 0
+    for sz in sizes:
+        assert sorted(got[o:o + sz].tolist()) == sorted(a[o:o + sz])
+        o += sz
+    # brute force: every left child holds the ceil(s/2) smallest (value, index) keys
+    def check(seg, depth):
+        if depth == ld:
+            return
+        keys = sorted((float(LP[depth, p]), int(p)) for p in seg)
+        left = {p for _, p in keys[:len(seg) - len(seg) // 2]}
+        first = len(seg) - len(seg) // 2
+        assert set(seg[:first]) == left
+        check(seg[:first], depth + 1)
+        check(seg[first:], depth + 1)
+
+    check(got.tolist(), 0)
+
+
+def test_build_graph_real_local_tree(orc):
+    """-density 1: dense R and dense local C; the build's lists equal the top-K
+    of each point's local-leafmate union (brute force over the leaves the
+    oracle reports), and differ from the tie-permutation build."""
+    rng = np.random.default_rng(16)
+    X = rng.integers(0, 40, size=(3000, 12)).astype(np.uint8)
+    D = orc.global_depth(orc.tree_depth(3000))
+    oi, od, _, _ = orc.build_graph(X, 3, D, 5, local_mode=0, seed=4, density=1.0)
+    oi0, _, _, _ = orc.build_graph(X, 3, D, 5, local_mode=0, seed=4, density=1.0 / np.sqrt(12))
+    assert (oi >= 0).all() and (oi != np.arange(3000)[:, None]).all()
+    assert np.all(np.diff(od, axis=1) >= 0)
+    for p in range(0, 3000, 97):
+        d2 = sorted(_exact_int_d2(X, p, q) for q in oi[p])
+        assert d2 == sorted(int(v) for v in od[p])
+    assert not np.array_equal(oi, oi0)
+
+
 def test_local_C_is_zero_for_truncated_density(orc):
     # (int)(1/sqrt(d)) == 0: the local projection keeps an entry only for a zero draw
     assert orc.local_C_nonzeros(784, 3, 0.0, 12345) == 0
