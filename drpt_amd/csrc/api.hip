@@ -97,6 +97,11 @@ int validate(const drpt_params* p) {
     set_error("precision not supported by this build");
     return DRPT_ERR_UNSUPPORTED;
   }
+  if (p->precision != DRPT_FP32 && p->nn > 32) {
+    // the exact row paths of the low-precision builds keep at most 32 per row
+    set_error("nn > 32 needs precision FP32 (BF16 / FP8 builds keep at most 32 neighbours)");
+    return DRPT_ERR_UNSUPPORTED;
+  }
   if (p->tree_group < 0) {
     set_error("tree_group must be >= 0");
     return DRPT_ERR_ARG;
@@ -153,6 +158,7 @@ int drpt_create(drpt_ctx** out, int device) {
       lds > 0)
     ctx->max_lds = lds;
   if (const char* g = getenv("DRPT_GRAPHS")) ctx->use_graphs = strcmp(g, "0") != 0;
+  if (const char* b = getenv("DRPT_ROW_EXACT_BATCH")) ctx->row_exact_batch = std::atoll(b);
   if (hipStreamCreate(&ctx->stream) != hipSuccess) {
     delete ctx;
     set_error("hipStreamCreate failed");
@@ -160,6 +166,9 @@ int drpt_create(drpt_ctx** out, int device) {
   }
   for (auto& e : ctx->ev) {
     if (hipEventCreate(&e) != hipSuccess) {
+      (void)hipGetLastError();
+      e = nullptr;
+      drpt_destroy(ctx);  // the stream and the events created so far
       set_error("hipEventCreate failed");
       return DRPT_ERR_HIP;
     }
@@ -1142,8 +1151,9 @@ int drpt_set_item_plan(drpt_ctx* ctx, const drpt_params* p, const int32_t* plan)
   DRPT_CHECK(validate(p));
   std::vector<int> v(kPlanInts);
   for (int b = 0; b < kSizeBuckets; ++b) {
-    if (plan[b] < 1 || (plan[kSizeBuckets + b] != 0 && plan[kSizeBuckets + b] != 1)) {
-      set_error("set_item_plan: rows per item must be >= 1 and ring flags 0 or 1");
+    if (plan[b] < 64 || (plan[kSizeBuckets + b] != 0 && plan[kSizeBuckets + b] != 1)) {
+      // the work-item buffer holds ceil(m / 64) items per leaf
+      set_error("set_item_plan: rows per item must be >= 64 and ring flags 0 or 1");
       return DRPT_ERR_ARG;
     }
     v[b] = plan[b];

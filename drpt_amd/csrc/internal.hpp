@@ -51,6 +51,9 @@ struct drpt_ctx {
   };
   std::map<std::string, Graph> graphs;
   bool use_graphs = true;  // environment DRPT_GRAPHS=0: launch eagerly
+  // environment DRPT_ROW_EXACT_BATCH=r: at most r rows per batch of the exact
+  // row path (its scratch is otherwise capped by bytes)
+  int64_t row_exact_batch = 0;
   // work-item plans found by drpt_autotune, by workload shape (ItemPlan)
   std::map<std::string, std::vector<int>> plans;
 };

@@ -438,7 +438,10 @@ __device__ __forceinline__ float exact_dist4(const XT* __restrict__ X, int64_t a
   return acc;
 }
 
-// A_cut(B) of the header comment, rounded up to float (+inf when B is not finite)
+// A_cut(B) of the header comment, rounded up to float (+inf when B is not finite).
+// The absolute 1e-30 covers errors the relative terms do not: squared norms,
+// Gram entries and FP8 rescalings that fall below the FP32 normal range
+// (at most 2 d 2^-126 < 1e-33 for d < 2^16).
 __device__ __forceinline__ float acut_of(float B, float nb_i, float e_i, double nm, float emax,
                                          int d) {
   if (!(B < INFINITY)) return INFINITY;
@@ -1673,6 +1676,8 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
 // wave sorts its share by (D, id), one wave merges the waves' best);
 // k_row_exact_merge: per row, the T sorted lists merged without duplicates.
 constexpr int kFbThreadsRow = 256;
+// largest per-tree list scratch of one batch of exact rows (bytes)
+constexpr int64_t kRowExactScratch = (int64_t)512 << 20;
 // one wave: exact keys (D bits << 32 | id, non-negative D orders like its
 // bits, ties by id) of leafmates c = e * kFbThreadsRow + tid of pl[0, m),
 // sorted over the wave; key[0] of lanes 0 .. 63 then holds the wave's 64
@@ -2185,30 +2190,42 @@ int launch_row_exact(drpt_ctx* ctx, const drpt_params* p, const void* X, const i
   hipLaunchKernelGGL(k_rowitem, dim3((unsigned)ctx->num_cus * 8), dim3(256), 0, ctx->stream, items,
                      range, n, rowitem);
   DRPT_HIP(hipGetLastError());
+  // per-tree lists of at most kRowExactScratch bytes of rows at a time: the
+  // rows [b0, b0 + batch) of ovf_list per pair of launches (nrows_max is the
+  // host's count of listed rows, so every batch holds listed rows only)
+  const int KPmax = K <= 16 ? 16 : 32;
+  const int64_t per_row = (int64_t)T * KPmax * 8;
+  int64_t batch = std::max<int64_t>(1, std::min<int64_t>(nrows_max, kRowExactScratch / per_row));
+  if (ctx->row_exact_batch > 0) batch = std::min<int64_t>(batch, ctx->row_exact_batch);
   int32_t* ti;
   float* td;
-  DRPT_CHECK(ws(ctx, "knn.rowx_i", (size_t)nrows_max * T * 32, &ti));
-  DRPT_CHECK(ws(ctx, "knn.rowx_d", (size_t)nrows_max * T * 32, &td));
-  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nrows_max * T, (int64_t)ctx->num_cus * 4));
-  const unsigned mgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nrows_max + 255) / 256, 1024));
-  auto go = [&](auto kp) -> int {
-    constexpr int KP = decltype(kp)::value;
-    if (p->dtype == DRPT_U8)
-      hipLaunchKernelGGL((k_row_exact_tree<uint8_t, KP>), dim3(grid), dim3(kFbThreadsRow), 0,
-                         ctx->stream, (const uint8_t*)X, p->d, perm, n, T, inv, rowitem, items,
-                         ovf_list, novf, nrows_max, ti, td);
-    else
-      hipLaunchKernelGGL((k_row_exact_tree<float, KP>), dim3(grid), dim3(kFbThreadsRow), 0,
-                         ctx->stream, (const float*)X, p->d, perm, n, T, inv, rowitem, items,
-                         ovf_list, novf, nrows_max, ti, td);
-    DRPT_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_row_exact_merge<KP>), dim3(mgrid), dim3(256), 0, ctx->stream, T, K,
-                       ovf_list, novf, nrows_max, ti, td, out_idx, out_dist);
-    DRPT_HIP(hipGetLastError());
-    return DRPT_OK;
-  };
-  if (K <= 16) return go(std::integral_constant<int, 16>());
-  return go(std::integral_constant<int, 32>());
+  DRPT_CHECK(ws(ctx, "knn.rowx_i", (size_t)batch * T * KPmax, &ti));
+  DRPT_CHECK(ws(ctx, "knn.rowx_d", (size_t)batch * T * KPmax, &td));
+  for (int64_t b0 = 0; b0 < nrows_max; b0 += batch) {
+    const int64_t nb = std::min(batch, nrows_max - b0);
+    const int32_t* rows = ovf_list + b0;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nb * T, (int64_t)ctx->num_cus * 4));
+    const unsigned mgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nb + 255) / 256, 1024));
+    auto go = [&](auto kp) -> int {
+      constexpr int KP = decltype(kp)::value;
+      if (p->dtype == DRPT_U8)
+        hipLaunchKernelGGL((k_row_exact_tree<uint8_t, KP>), dim3(grid), dim3(kFbThreadsRow), 0,
+                           ctx->stream, (const uint8_t*)X, p->d, perm, n, T, inv, rowitem, items,
+                           rows, novf, nb, ti, td);
+      else
+        hipLaunchKernelGGL((k_row_exact_tree<float, KP>), dim3(grid), dim3(kFbThreadsRow), 0,
+                           ctx->stream, (const float*)X, p->d, perm, n, T, inv, rowitem, items,
+                           rows, novf, nb, ti, td);
+      DRPT_HIP(hipGetLastError());
+      hipLaunchKernelGGL((k_row_exact_merge<KP>), dim3(mgrid), dim3(256), 0, ctx->stream, T, K,
+                         rows, novf, nb, ti, td, out_idx, out_dist);
+      DRPT_HIP(hipGetLastError());
+      return DRPT_OK;
+    };
+    if (K <= 16) DRPT_CHECK(go(std::integral_constant<int, 16>()));
+    else DRPT_CHECK(go(std::integral_constant<int, 32>()));
+  }
+  return DRPT_OK;
 }
 
 int launch_fallback(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,

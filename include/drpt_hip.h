@@ -47,7 +47,8 @@ typedef struct {
   int32_t dtype;        /* DRPT_U8 | DRPT_F32 */
   int32_t ntrees;       /* -ntrees */
   int32_t depth;        /* projection columns per tree: Dg (reference mode) or max depth */
-  int32_t nn;           /* -nn: K = nn non-self neighbours returned per point */
+  int32_t nn;           /* -nn: K = nn non-self neighbours returned per point (1-64;
+                           BF16 / FP8 builds: 1-32) */
   int32_t local_mode;   /* DRPT_LOCAL_REFERENCE | DRPT_LOCAL_LEAFMAX */
   int32_t leaf_max;     /* leaf-max mode: nodes larger than this are split */
   int32_t local_offset; /* -local-tree-offset (reference mode, main.cpp:37) */
@@ -160,7 +161,7 @@ int drpt_build_graph(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t
 int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* plan_out);
 
 /* Sets the work-item plan of a workload shape, in drpt_autotune's plan_out
- * layout (rows per item >= 1, ring flags 0 / 1, trees per group, first trees
+ * layout (rows per item >= 64, ring flags 0 / 1, trees per group, first trees
  * and re-rank depth >= 0 with 0 for the automatic rules; the depth is clamped
  * to [nn, 2 nn]): restores a plan saved from an earlier drpt_autotune. No
  * counterpart in the reference. */
