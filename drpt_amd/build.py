@@ -29,7 +29,7 @@ HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall"]
 CLI_FLAGS = ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
 HIP_SOURCES = ["api.hip", "util.hip", "project.hip", "split.hip", "group.hip", "leafknn.hip",
-               "merge.hip"]
+               "leafpool.hip", "merge.hip"]
 HOST_SOURCES = ["host_ref.cpp"]
 CLI_SOURCES = ["drpt_main.cpp", "io.cpp"]
 

@@ -701,7 +701,13 @@ static std::string plan_key(const drpt_params* p) {
   return k;
 }
 
-static ItemPlan lowp_default_plan() { return uniform_plan(1 << 30); }  // whole leaves, ring
+// whole leaves; the pooled filter pass with each leaf resident in LDS where it
+// fits, else (and for the candidate pass) the LDS tile ring
+static ItemPlan lowp_default_plan() {
+  ItemPlan p = uniform_plan(1 << 30);
+  for (int b = 0; b < kSizeBuckets; ++b) p.ring[b] = 2;
+  return p;
+}
 
 static ItemPlan lowp_plan_of(drpt_ctx* ctx, const drpt_params* p) {
   auto it = ctx->plans.find(plan_key(p));
@@ -736,7 +742,8 @@ static int lowp_setup(drpt_ctx* ctx, const drpt_params* p, const void* X,
   r->first = std::max(1, std::min(std::min(r->T, plan.first_trees > 0 ? plan.first_trees
                                                                       : kFirstTrees),
                                   256 / r->KP));
-  r->pool_cap = pool_cap_of(r->K);
+  // the LDS-resident pool pass leaves known ids to the row merge: larger pools
+  r->pool_cap = pool_leaf_rows(ctx, p->d) > 0 ? std::max(512, pool_cap_of(r->K)) : pool_cap_of(r->K);
   const int64_t n = r->n;
   DRPT_CHECK(group_impl(ctx, p, X, leaf_heap, plan, r->first, &r->G));
   DRPT_CHECK(prep_lowp(ctx, &r->pp, X, r->G.perm, &r->a));
@@ -780,19 +787,31 @@ static int lowp_pass(drpt_ctx* ctx, LowpRun* r, int mode, const ItemPlan& plan,
   const drpt_params* pr = mode == kModeCand ? &r->pc : &r->pp;
   a.items = r->G.items;
   a.range = range;
+  // leaves of sizes [lo, hi] with Gram pipeline `ring`: the LDS-resident pool
+  // pass (ring 2) takes those it can stage, the streaming kernel the rest
+  const int pl_rows = mode == kModePool ? pool_leaf_rows(ctx, pr->d) : 0;
+  auto run = [&](int lo, int hi, int ring) -> int {
+    if (ring == 2 && pl_rows > 0 && lo <= pl_rows) {
+      KnnArgs b = a;
+      b.size_lo = lo;
+      b.size_hi = std::min(hi, pl_rows);
+      DRPT_CHECK(launch_pool_leaf(ctx, b));
+      if (hi <= pl_rows) return DRPT_OK;
+      lo = pl_rows + 1;
+    }
+    a.size_lo = lo;
+    a.size_hi = hi;
+    a.ring = ring == 0 ? 0 : 1;
+    return launch_lowp_knn(ctx, pr, mode, a);
+  };
   bool uniform = true;
   for (int b = 1; b < kSizeBuckets; ++b) uniform = uniform && plan.ring[b] == plan.ring[0];
-  if (bucket < 0 && uniform) {
-    a.size_lo = 0;
-    a.size_hi = INT_MAX;
-    a.ring = plan.ring[0];
-    return launch_lowp_knn(ctx, pr, mode, a);
-  }
+  if (bucket < 0 && uniform) return run(0, INT_MAX, plan.ring[0]);
   for (int b = 0; b < kSizeBuckets; ++b) {
     if (bucket >= 0 && b != bucket) continue;
-    bucket_sizes(b, &a.size_lo, &a.size_hi);
-    a.ring = plan.ring[b];
-    DRPT_CHECK(launch_lowp_knn(ctx, pr, mode, a));
+    int lo, hi;
+    bucket_sizes(b, &lo, &hi);
+    DRPT_CHECK(run(lo, hi, plan.ring[b]));
   }
   return DRPT_OK;
 }
@@ -1021,6 +1040,7 @@ int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* p
   r.a.known = r.ki;
   const int rb = p->d * 2;  // the pooled passes run in BF16
   const bool ring_ok = rb % 16 == 0 && rb <= 256;
+  const bool leaf_ok = pool_leaf_rows(ctx, p->d) > 0;  // whole leaves resident in LDS
   hipEvent_t e0 = ctx->ev[10], e1 = ctx->ev[11];
   // candidates per bucket: rows per item x Gram pipeline; each timed twice
   // after a warm-up, the incumbent kept unless another is 3% faster
@@ -1028,7 +1048,7 @@ int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* p
   for (int b = 0; b < kSizeBuckets; ++b) {
     double best_ms = -1.0;
     for (int ir = 0; ir < 4; ++ir) {
-      for (int ring = 1; ring >= (ring_ok ? 0 : 1); --ring) {
+      for (int ring = ir == 0 && leaf_ok ? 2 : 1; ring >= (ring_ok ? 0 : 1); --ring) {
         ItemPlan cand = best;
         cand.rpi[b] = rpis[ir];
         cand.ring[b] = ring;
@@ -1151,9 +1171,9 @@ int drpt_set_item_plan(drpt_ctx* ctx, const drpt_params* p, const int32_t* plan)
   DRPT_CHECK(validate(p));
   std::vector<int> v(kPlanInts);
   for (int b = 0; b < kSizeBuckets; ++b) {
-    if (plan[b] < 64 || (plan[kSizeBuckets + b] != 0 && plan[kSizeBuckets + b] != 1)) {
+    if (plan[b] < 64 || plan[kSizeBuckets + b] < 0 || plan[kSizeBuckets + b] > 2) {
       // the work-item buffer holds ceil(m / 64) items per leaf
-      set_error("set_item_plan: rows per item must be >= 64 and ring flags 0 or 1");
+      set_error("set_item_plan: rows per item must be >= 64 and ring flags 0, 1 or 2");
       return DRPT_ERR_ARG;
     }
     v[b] = plan[b];

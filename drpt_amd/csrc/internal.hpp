@@ -54,6 +54,7 @@ struct drpt_ctx {
   // environment DRPT_ROW_EXACT_BATCH=r: at most r rows per batch of the exact
   // row path (its scratch is otherwise capped by bytes)
   int64_t row_exact_batch = 0;
+  unsigned pool_leaf_lds_set = 0;  // LDS-resident pool pass instances cleared for their dynamic LDS
   // work-item plans found by drpt_autotune, by workload shape (ItemPlan)
   std::map<std::string, std::vector<int>> plans;
 };
@@ -110,7 +111,9 @@ struct LeafItem {   // one work item: a block of rows of one leaf (or the whole 
 // Work-item layout by leaf size: leaves of bucket b (sizes <= 64, <= 256,
 // <= 1024, larger) are cut into items of rpi[b] rows, and the low-precision
 // Gram passes over them use the LDS tile ring (ring[b] = 1, where the row
-// length allows it) or the register double buffer (0); tree_group = trees per
+// length allows it), the register double buffer (0) or, for the pooled filter
+// pass, the whole leaf resident in LDS (2; leaves too large for it, and other
+// passes, take the ring); tree_group = trees per
 // pooled filter pass (0: the rule of drpt_params.tree_group = 0); first_trees =
 // trees whose widened candidates give the first bounds (0: 4); rerank = their
 // best approximate candidates whose exact distances give those bounds, in
@@ -225,6 +228,11 @@ struct KnnArgs {
 // against an external bound, survivors appended to per-row pools
 int launch_lowp_knn(drpt_ctx* ctx, const drpt_params* p, int mode, KnnArgs a);
 int lowp_slots(const drpt_params* p);  // k' actually used (a multiple of 4)
+// leafpool.hip: the pool pass with each leaf resident in LDS (work-item plan
+// ring flag 2): largest leaf it stages at dimension d (0: not applicable), and
+// the launch over the items of a.range with sizes in [a.size_lo, a.size_hi]
+int pool_leaf_rows(drpt_ctx* ctx, int d);
+int launch_pool_leaf(drpt_ctx* ctx, const KnnArgs& a);
 
 struct RowMergeArgs {
   const void* X;

@@ -1,0 +1,386 @@
+// Stage (c): the pooled filter pass with each leaf resident in LDS.
+//
+// The graph build's certifying pass (MDRPT::calculate_nns,
+// cpp/src/drpt/algo/mdrpt.cpp:123-212, restated as: every leafmate j of row i
+// whose exact distance can be at most the row's bound B(i) goes to row i's
+// pool) for leaves whose bf16 rows fit the LDS. The streaming variant
+// (k_knn_lowp, pool mode) re-reads a leaf's rows once per 64-row query block;
+// here a block stages the whole leaf once (global_load_lds_dwordx4, rows
+// XOR-swizzled by 16-byte chunk for conflict-free MFMA operand reads) and its
+// eight waves take (64 query rows x column range) tasks from an LDS counter,
+// with no barrier until the leaf is done.
+//
+// Filter. The accumulator of the v_mfma_f32_32x32x16_bf16 tile starts at
+// -|x~_j|^2 / 2 (column j, exact halving), so it ends at
+//   acc_ij = G_ij - |x~_j|^2 / 2 = (|x~_i|^2 - a_ij) / 2,
+// and a_ij <= A_cut(B_i) becomes acc_ij >= thr_i with the per-row threshold
+//   thr_i = round_down((|x~_i|^2 - A_cut(B_i) - eta_i) / 2):
+// one compare per element. The extra eta_i = (2d + 16) 2^-24 (n_i + nm)^2 (the
+// bound A_cut already holds once) covers the different rounding of this form
+// (the initialised accumulator, the stored norms and the subtraction, each
+// within (d + 4) 2^-24 (n_i + nm)^2), so every leafmate the streaming pass
+// keeps is kept here as well.
+//
+// Survivors are collected per lane (its query row and 16 columns of each
+// tile) in a register list of 8 leaf positions and appended to the row's pool
+// with one atomic per list; known ids are not dropped here (the row merge's
+// hash drops them before any exact distance).
+//
+// Roofline: MFMA-bound, 2 d m^2 flops per leaf; HBM bytes per leaf m * 2d
+// (rows) + 16 m (ids, norms, errors, bounds).
+#include <climits>
+
+#include "internal.hpp"
+#include "lowp_common.hpp"
+
+namespace drpt {
+
+namespace {
+
+using namespace lowp;
+
+constexpr int kPlWaves = 8;
+constexpr int kPlThreads = 64 * kPlWaves;
+constexpr int kPlPitch = 256;   // LDS bytes per leaf row (16 chunks of 16 bytes)
+constexpr int kPlSteps = 8;     // k-steps of 16 elements: rows of at most 128 bf16
+constexpr int kPlList = 8;      // pool entries a lane holds per query row
+constexpr int kPlQueues = 8;    // ticket queues, one per XCD (blocks go round-robin)
+constexpr int kPlMisc = 32;     // ints of bookkeeping after the leaf arrays
+
+__device__ __forceinline__ uint4 row_chunk(const unsigned char* rows, int r, int c) {
+  return *reinterpret_cast<const uint4*>(rows + r * kPlPitch + ((c ^ (r & 15)) << 4));
+}
+
+// Per-lane list of 8 leaf positions in 4 registers (newest in w[0] bits 0-15).
+struct PosList {
+  uint32_t w[4];
+  int cnt;
+  __device__ __forceinline__ void reset() {
+    w[0] = w[1] = w[2] = w[3] = 0u;
+    cnt = 0;
+  }
+  __device__ __forceinline__ void push(uint32_t c) {
+    w[3] = __builtin_amdgcn_alignbit(w[3], w[2], 16);
+    w[2] = __builtin_amdgcn_alignbit(w[2], w[1], 16);
+    w[1] = __builtin_amdgcn_alignbit(w[1], w[0], 16);
+    w[0] = (w[0] << 16) | c;
+    ++cnt;
+  }
+};
+
+// Appends the list's entries to row id's pool (one atomic); a pool that
+// cannot take them sends the row to the exact row path.
+__device__ __forceinline__ void flush_list(const KnnArgs& A, const int32_t* lid, int id,
+                                           PosList& L) {
+  if (L.cnt <= 0) return;
+  const int base = atomicAdd(&A.pool_cnt[id], L.cnt);
+  if (base + L.cnt <= A.pool_cap) {
+    int32_t* dst = A.out_idx + (int64_t)id * A.pool_cap + base;
+#pragma unroll
+    for (int j = 0; j < kPlList; ++j)
+      if (j < L.cnt) dst[j] = lid[(L.w[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu];
+  } else if (atomicExch(&A.ovf_flag[id], 1) == 0) {
+    A.ovf_list[atomicAdd(A.novf, 1)] = id;
+  }
+  L.reset();
+}
+
+// Both query rows' lists: the two reservations issued before either is awaited.
+__device__ __forceinline__ void flush_pair(const KnnArgs& A, const int32_t* lid, const int (&id)[2],
+                                           PosList (&L)[2]) {
+  int base[2];
+#pragma unroll
+  for (int g = 0; g < 2; ++g) base[g] = L[g].cnt > 0 ? atomicAdd(&A.pool_cnt[id[g]], L[g].cnt) : 0;
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    if (L[g].cnt <= 0) continue;
+    if (base[g] + L[g].cnt <= A.pool_cap) {
+      int32_t* dst = A.out_idx + (int64_t)id[g] * A.pool_cap + base[g];
+#pragma unroll
+      for (int j = 0; j < kPlList; ++j)
+        if (j < L[g].cnt) dst[j] = lid[(L[g].w[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu];
+    } else if (atomicExch(&A.ovf_flag[id[g]], 1) == 0) {
+      A.ovf_list[atomicAdd(A.novf, 1)] = id[g];
+    }
+  }
+}
+
+// Filter bits of one lane's 16 tile elements (bit 4 g + u: column
+// 8 g + u of the lane's half) for threshold thr.
+__device__ __forceinline__ uint32_t pass_bits(const floatx16& acc, float thr) {
+  uint32_t pm = 0;
+#pragma unroll
+  for (int e = 15; e >= 0; --e) pm = pm + pm + (acc[e] >= thr ? 1u : 0u);
+  return pm;
+}
+
+template <int KS>
+__global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, int mmax) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  unsigned char* rows = smem;
+  int32_t* lid = reinterpret_cast<int32_t*>(smem + (size_t)mmax * kPlPitch);
+  float* chalf = reinterpret_cast<float*>(lid + mmax);
+  float* thrs = chalf + mmax;                        // per leaf row: filter threshold
+  int* misc = reinterpret_cast<int*>(thrs + mmax);   // tickets [0, 1], task counter [2]
+  float* red = reinterpret_cast<float*>(misc + 4);   // per wave: max |x~|^2 [w], max error [8 + w]
+  const uint8_t* __restrict__ Xq = static_cast<const uint8_t*>(A.Xq);
+  constexpr int d = 16 * KS;
+  constexpr int rb = 2 * d;      // bytes per bf16 row
+  constexpr int L = rb >> 4;     // 16-byte chunks per row
+  constexpr int kRowsPerThread = 2;  // leaf rows whose threshold a thread computes
+  const int64_t n = A.n;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int half = lane >> 5;
+  const int qx = blockIdx.x % kPlQueues;
+  const int64_t chunk = (A.range[1] - A.range[0] + kPlQueues - 1) / kPlQueues;
+  const int64_t begin = A.range[0] + qx * chunk;
+  const int64_t end = min(A.range[1], begin + chunk);
+  unsigned* myq = A.queue + qx;
+  if (tid == 0) misc[0] = (int)atomicAdd(myq, 1u);
+  __syncthreads();
+  int64_t item = begin + misc[0];
+  // self bit of the diagonal tile: lane's query row (lane & 31) among its columns
+  const int srel = (lane & 31) - 4 * half;
+  const uint32_t selfbit =
+      (srel >= 0 && (srel & 4) == 0) ? 1u << (((srel >> 3) << 2) | (srel & 3)) : 0u;
+  const double u = 5.9604644775390625e-08;  // 2^-24
+  for (int parity = 1;; parity ^= 1) {
+    if (item >= end) break;
+    if (tid == 0) misc[parity] = (int)atomicAdd(myq, 1u);  // next item, fetched early
+    const LeafItem it = A.items[item];
+    const int m = it.size;
+    if (m < A.size_lo || m > A.size_hi) {  // another launch's leaves
+      __syncthreads();
+      item = begin + misc[parity];
+      continue;
+    }
+    const int64_t lofs = (int64_t)it.tree * n + it.start;
+    const int32_t* pl = A.perm + lofs;
+    // ids, -|x~|^2 / 2, the leaf's largest norm and error; each thread keeps
+    // the norm, error and bound of its rows for their thresholds
+    float mn = 0.0f, me = 0.0f;
+    float rnb[kRowsPerThread], rer[kRowsPerThread], rbd[kRowsPerThread];
+#pragma unroll
+    for (int h = 0; h < kRowsPerThread; ++h) {
+      const int j = tid + h * kPlThreads;
+      rnb[h] = rer[h] = rbd[h] = 0.0f;
+      if (j < m) {
+        rnb[h] = A.lnbT[lofs + j];
+        rer[h] = A.lerrT[lofs + j];
+        rbd[h] = A.boundT[lofs + j];
+        lid[j] = pl[j];
+        chalf[j] = -0.5f * rnb[h];
+        mn = fmaxf(mn, rnb[h]);
+        me = fmaxf(me, rer[h]);
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      mn = fmaxf(mn, __shfl_xor(mn, off));
+      me = fmaxf(me, __shfl_xor(me, off));
+    }
+    if (lane == 0) {
+      red[wave] = mn;
+      red[8 + wave] = me;
+    }
+    if (tid == 0) misc[2] = 0;
+    __syncthreads();  // ids and maxima visible
+    // the leaf's rows: slot s = 16 r + physical chunk, logical chunk = physical ^ (r & 15);
+    // one instruction per wave covers 64 slots (4 rows)
+    for (int blk = wave; blk * 4 < m; blk += kPlWaves) {
+      const int slot = blk * 64 + lane;
+      const int r = slot >> 4, c = (slot & 15) ^ (r & 15);
+      const uint32_t dst = __builtin_amdgcn_readfirstlane(
+          (uint32_t)(uintptr_t)(lds_ptr_t)(rows + blk * 64 * 16));
+      if (r < m && c < L) lds_dma16(Xq + (int64_t)lid[r] * rb + c * 16, dst);
+    }
+    // thresholds while the rows arrive:
+    //   thr = round_down((|x~|^2 - A_cut(B) - eta) / 2)
+    float nmax = red[0], emax = red[8];
+#pragma unroll
+    for (int w = 1; w < kPlWaves; ++w) {
+      nmax = fmaxf(nmax, red[w]);
+      emax = fmaxf(emax, red[8 + w]);
+    }
+    const double nm = sqrt((double)nmax * (1.0 + (d + 4.0) * u));
+#pragma unroll
+    for (int h = 0; h < kRowsPerThread; ++h) {
+      const int j = tid + h * kPlThreads;
+      if (j < m) {
+        const float acut = acut_of(rbd[h], rnb[h], rer[h], nm, emax, d);
+        const double ni = sqrt((double)rnb[h] * (1.0 + (d + 4.0) * u));
+        const double eta = (2.0 * d + 16.0) * u * (ni + nm) * (ni + nm);
+        thrs[j] = __double2float_rd(0.5 * ((double)rnb[h] - (double)acut - eta));
+      }
+    }
+    wait_vmcnt<0>();
+    __syncthreads();  // rows and thresholds visible to every wave
+    // tasks: 64 query rows of the item x a range of 32-column tiles; the split
+    // minimises rounds x (tiles per task + 1) over the block's waves
+    const int row0 = it.row0, row_end = it.row0 + it.rows;
+    const int nq = (it.rows + 63) >> 6;
+    const int ntile = (m + 31) >> 5;
+    int nc = 1, best = INT_MAX;
+    for (int c = 1; c <= ntile && c <= 4 * kPlWaves; ++c) {
+      const int tp = (ntile + c - 1) / c;
+      const int cost = ((nq * c + kPlWaves - 1) / kPlWaves) * (tp + 1);
+      if (cost < best) {
+        best = cost;
+        nc = c;
+      }
+    }
+    const int tper = (ntile + nc - 1) / nc;
+    nc = (ntile + tper - 1) / tper;
+    const int ntask = nq * nc;
+    for (;;) {
+      int task = 0;
+      if (lane == 0) task = atomicAdd(&misc[2], 1);
+      task = __builtin_amdgcn_readfirstlane(__shfl(task, 0));
+      if (task >= ntask) break;
+      const int qi = task / nc, cj = task - qi * nc;
+      const int qb = row0 + 64 * qi;
+      const int t0 = cj * tper, t1 = min(ntile, t0 + tper);
+      uint4 qop[2][KS];
+      float thr[2];
+      int id[2];
+      PosList lst[2];
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        const int r = qb + 32 * g + (lane & 31);
+        const bool valid = r < row_end;
+        const int rr = valid ? r : row0;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) qop[g][ks] = row_chunk(rows, rr, 2 * ks + half);
+        thr[g] = thrs[rr];
+        id[g] = lid[rr];
+        if (!valid) thr[g] = INFINITY;  // and its bits are cleared below
+        lst[g].reset();
+      }
+      const bool valid0 = qb + (lane & 31) < row_end;
+      const bool valid1 = qb + 32 + (lane & 31) < row_end;
+      auto tile = [&](int t, floatx16& a0, floatx16& a1) {
+        const int c0 = 32 * t;
+        const int r = c0 + (lane & 31);
+        uint4 av[KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) av[ks] = row_chunk(rows, r, 2 * ks + half);
+        floatx16 ci;
+        const int cb = c0 + 4 * half;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const float4 v = *reinterpret_cast<const float4*>(&chalf[cb + 8 * g]);
+          ci[4 * g] = v.x;
+          ci[4 * g + 1] = v.y;
+          ci[4 * g + 2] = v.z;
+          ci[4 * g + 3] = v.w;
+        }
+        a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av[0]),
+                                                     __builtin_bit_cast(bf16x8, qop[0][0]), ci,
+                                                     0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av[0]),
+                                                     __builtin_bit_cast(bf16x8, qop[1][0]), ci,
+                                                     0, 0, 0);
+#pragma unroll
+        for (int ks = 1; ks < KS; ++ks) {
+          a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av[ks]),
+                                                       __builtin_bit_cast(bf16x8, qop[0][ks]),
+                                                       a0, 0, 0, 0);
+          a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av[ks]),
+                                                       __builtin_bit_cast(bf16x8, qop[1][ks]),
+                                                       a1, 0, 0, 0);
+        }
+      };
+      auto epilogue = [&](int t, const floatx16& a0, const floatx16& a1) {
+        const int c0 = 32 * t;
+        const int cb = c0 + 4 * half;
+        uint32_t colmask = 0xFFFFu;
+        if (c0 + 32 > m) {  // last tile: columns past the leaf
+          colmask = 0u;
+#pragma unroll
+          for (int e = 0; e < 16; ++e)
+            colmask |= (cb + 8 * (e >> 2) + (e & 3) < m ? 1u : 0u) << e;
+        }
+        uint32_t pm0 = valid0 ? pass_bits(a0, thr[0]) & colmask : 0u;
+        uint32_t pm1 = valid1 ? pass_bits(a1, thr[1]) & colmask : 0u;
+        if (c0 == qb) pm0 &= ~selfbit;
+        if (c0 == qb + 32) pm1 &= ~selfbit;
+        while (pm0 != 0u) {
+          const int e = __builtin_ctz(pm0);
+          pm0 &= pm0 - 1u;
+          lst[0].push((uint32_t)(cb + 8 * (e >> 2) + (e & 3)));
+          if (lst[0].cnt == kPlList) flush_list(A, lid, id[0], lst[0]);
+        }
+        while (pm1 != 0u) {
+          const int e = __builtin_ctz(pm1);
+          pm1 &= pm1 - 1u;
+          lst[1].push((uint32_t)(cb + 8 * (e >> 2) + (e & 3)));
+          if (lst[1].cnt == kPlList) flush_list(A, lid, id[1], lst[1]);
+        }
+      };
+      // two accumulator sets: the next tile's MFMAs are in flight during this
+      // tile's epilogue
+      floatx16 a0, a1, b0, b1;
+      tile(t0, a0, a1);
+      for (int t = t0; t < t1; t += 2) {
+        const bool has1 = t + 1 < t1;
+        if (has1) tile(t + 1, b0, b1);
+        epilogue(t, a0, a1);
+        if (!has1) break;
+        if (t + 2 < t1) tile(t + 2, a0, a1);
+        epilogue(t + 1, b0, b1);
+      }
+      flush_pair(A, lid, id, lst);
+    }
+    __syncthreads();  // every wave done with the leaf
+    item = begin + misc[parity];
+  }
+}
+
+}  // namespace
+
+// Largest leaf (rows, a multiple of 64) the LDS-resident pool pass stages at
+// dimension d on this device, or 0 when it does not apply (BF16 rows of 32 to
+// 256 bytes in 32-byte multiples).
+int pool_leaf_rows(drpt_ctx* ctx, int d) {
+  if (d < 16 || d > 16 * kPlSteps || (d & 15) != 0) return 0;
+  const int64_t room = (int64_t)ctx->max_lds - kPlMisc * 4;
+  const int64_t rows = (room / (kPlPitch + 12)) & ~(int64_t)63;
+  return rows >= 64 ? (int)std::min<int64_t>(rows, 2 * kPlThreads) : 0;
+}
+
+int launch_pool_leaf(drpt_ctx* ctx, const KnnArgs& a0) {
+  const int mmax = pool_leaf_rows(ctx, a0.d);
+  if (mmax <= 0 || a0.size_hi > mmax) {
+    set_error("internal: pool_leaf launched for leaves it cannot stage");
+    return DRPT_ERR_ARG;
+  }
+  const size_t lds = (size_t)mmax * (kPlPitch + 12) + kPlMisc * 4;
+  const int ks = a0.d / 16;
+  const void* fn = nullptr;
+  switch (ks) {
+    case 1: fn = (const void*)k_pool_leaf<1>; break;
+    case 2: fn = (const void*)k_pool_leaf<2>; break;
+    case 3: fn = (const void*)k_pool_leaf<3>; break;
+    case 4: fn = (const void*)k_pool_leaf<4>; break;
+    case 5: fn = (const void*)k_pool_leaf<5>; break;
+    case 6: fn = (const void*)k_pool_leaf<6>; break;
+    case 7: fn = (const void*)k_pool_leaf<7>; break;
+    default: fn = (const void*)k_pool_leaf<8>; break;
+  }
+  if (!(ctx->pool_leaf_lds_set & (1u << ks))) {
+    DRPT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    ctx->pool_leaf_lds_set |= 1u << ks;
+  }
+  KnnArgs a = a0;
+  unsigned* q;
+  DRPT_CHECK(ws(ctx, "knn.queue_pl", kPlQueues, &q));
+  DRPT_CHECK(fill_u32(ctx, q, 0u, kPlQueues));
+  a.queue = q;
+  void* args[] = {(void*)&a, (void*)&mmax};
+  DRPT_HIP(hipLaunchKernel(fn, dim3((unsigned)ctx->num_cus), dim3(kPlThreads), args, lds,
+                           ctx->stream));
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
+}
+
+}  // namespace drpt

@@ -148,7 +148,8 @@ int drpt_build_graph(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t
  * (X and p as for drpt_build_graph), times the pooled filter pass of a build
  * per leaf-size bucket (sizes <= 64, <= 256, <= 1024, larger) over the rows a
  * work item takes (whole leaf, 256, 128, 64) and the Gram pipeline (LDS tile
- * ring or register double buffer), then the whole low-precision stage per
+ * ring, register double buffer, or, for whole leaves whose bf16 rows fit the
+ * LDS, the leaf resident in LDS), then the whole low-precision stage per
  * trees-per-group (ntrees, 8, 6, 5, 4, 3, 2; when p->tree_group is 0), per
  * number of trees whose widened candidates give the first bounds (4, 2, 1) and
  * per number of their best approximate candidates re-ranked exactly for those
@@ -161,7 +162,8 @@ int drpt_build_graph(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t
 int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* plan_out);
 
 /* Sets the work-item plan of a workload shape, in drpt_autotune's plan_out
- * layout (rows per item >= 64, ring flags 0 / 1, trees per group, first trees
+ * layout (rows per item >= 64, pipeline flags 0 (registers) / 1 (LDS tile
+ * ring) / 2 (leaf resident in LDS, pooled filter pass), trees per group, first trees
  * and re-rank depth >= 0 with 0 for the automatic rules; the depth is clamped
  * to [nn, 2 nn]): restores a plan saved from an earlier drpt_autotune. No
  * counterpart in the reference. */

@@ -343,7 +343,7 @@ def test_autotune_and_item_plans(ctx, prec, case, leaf_max, T):
         plan = c.autotune(p, Xd)
         _log(f"  plan {plan}")
         assert all(r in (64, 128, 256, 1 << 30) for r in plan["rows_per_item"])
-        assert all(g in (0, 1) for g in plan["ring"])
+        assert all(g in (0, 1, 2) for g in plan["ring"])
         assert 1 <= plan["tree_group"] <= T
         assert plan["first_trees"] in (1, 2, 4)
         assert 10 <= plan["rerank"] <= 20
@@ -352,7 +352,9 @@ def test_autotune_and_item_plans(ctx, prec, case, leaf_max, T):
                    "rerank": 10},
                   {"rows_per_item": [64, 128, 256, 64], "ring": [1, 0, 1, 0], "tree_group": 3,
                    "first_trees": 2},
-                  {"rows_per_item": [1 << 30] * 4, "ring": [0, 1, 0, 1], "tree_group": 0}]
+                  {"rows_per_item": [1 << 30] * 4, "ring": [0, 1, 0, 1], "tree_group": 0},
+                  {"rows_per_item": [1 << 30, 64, 1 << 30, 128], "ring": [2, 2, 1, 2],
+                   "tree_group": 2}]
         for pl in forced:
             c.set_item_plan(p, pl)
             oi, od, _ = c.build_graph(p, Xd)
