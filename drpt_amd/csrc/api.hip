@@ -789,13 +789,17 @@ static int lowp_pass(drpt_ctx* ctx, LowpRun* r, int mode, const ItemPlan& plan,
   a.range = range;
   // leaves of sizes [lo, hi] with Gram pipeline `ring`: the LDS-resident pool
   // pass (ring 2) takes those it can stage, the streaming kernel the rest
-  const int pl_rows = mode == kModePool ? pool_leaf_rows(ctx, pr->d) : 0;
+  // (candidate pass: BF16 rows and k' in {12, 16, 24} only)
+  const int kp = lowp_slots(pr);
+  const bool pl_ok = mode == kModePool ||
+                     (mode == kModeCand && pr->precision == DRPT_BF16 && (kp == 12 || kp == 16 || kp == 24));
+  const int pl_rows = pl_ok ? pool_leaf_rows(ctx, pr->d) : 0;
   auto run = [&](int lo, int hi, int ring) -> int {
     if (ring == 2 && pl_rows > 0 && lo <= pl_rows) {
       KnnArgs b = a;
       b.size_lo = lo;
       b.size_hi = std::min(hi, pl_rows);
-      DRPT_CHECK(launch_pool_leaf(ctx, b));
+      DRPT_CHECK(launch_pool_leaf(ctx, mode, kp, b));
       if (hi <= pl_rows) return DRPT_OK;
       lo = pl_rows + 1;
     }

@@ -54,7 +54,7 @@ struct drpt_ctx {
   // environment DRPT_ROW_EXACT_BATCH=r: at most r rows per batch of the exact
   // row path (its scratch is otherwise capped by bytes)
   int64_t row_exact_batch = 0;
-  unsigned pool_leaf_lds_set = 0;  // LDS-resident pool pass instances cleared for their dynamic LDS
+  unsigned long long pool_leaf_lds_set = 0;  // LDS-resident pass instances cleared for their dynamic LDS
   // work-item plans found by drpt_autotune, by workload shape (ItemPlan)
   std::map<std::string, std::vector<int>> plans;
 };
@@ -228,11 +228,13 @@ struct KnnArgs {
 // against an external bound, survivors appended to per-row pools
 int launch_lowp_knn(drpt_ctx* ctx, const drpt_params* p, int mode, KnnArgs a);
 int lowp_slots(const drpt_params* p);  // k' actually used (a multiple of 4)
-// leafpool.hip: the pool pass with each leaf resident in LDS (work-item plan
-// ring flag 2): largest leaf it stages at dimension d (0: not applicable), and
-// the launch over the items of a.range with sizes in [a.size_lo, a.size_hi]
+// leafpool.hip: the pool and candidate passes with each leaf resident in LDS
+// (work-item plan flag 2): largest leaf it stages at dimension d (0: not
+// applicable), and the launch (mode kModePool or kModeCand with kp = 4 TK
+// widened candidates, TK in {3, 4, 6}; BF16 rows) over the items of a.range
+// with sizes in [a.size_lo, a.size_hi]
 int pool_leaf_rows(drpt_ctx* ctx, int d);
-int launch_pool_leaf(drpt_ctx* ctx, const KnnArgs& a);
+int launch_pool_leaf(drpt_ctx* ctx, int mode, int kp, const KnnArgs& a);
 
 struct RowMergeArgs {
   const void* X;

@@ -114,13 +114,19 @@ __device__ __forceinline__ uint32_t pass_bits(const floatx16& acc, float thr) {
   return pm;
 }
 
-template <int KS>
+// MODE kModePool: survivors of the certified filter to the rows' pools.
+// MODE kModeCand: each row's k' = 4 TK widened candidates, the TK smallest
+// approximate distances of each of its four column streams (two column ranges
+// x two lane halves; of every 4 adjacent columns of a stream the nearest one),
+// as keys (approx bits & ~0x7FF) << 32 | id.
+template <int KS, int MODE, int TK>
 __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, int mmax) {
+  static_assert(MODE == kModePool || MODE == kModeCand, "pool or candidate mode");
   extern __shared__ __align__(16) unsigned char smem[];
   unsigned char* rows = smem;
   int32_t* lid = reinterpret_cast<int32_t*>(smem + (size_t)mmax * kPlPitch);
   float* chalf = reinterpret_cast<float*>(lid + mmax);
-  float* thrs = chalf + mmax;                        // per leaf row: filter threshold
+  float* thrs = chalf + mmax;  // per leaf row: filter threshold (pool) or |x~|^2 (candidates)
   int* misc = reinterpret_cast<int*>(thrs + mmax);   // tickets [0, 1], task counter [2]
   float* red = reinterpret_cast<float*>(misc + 4);   // per wave: max |x~|^2 [w], max error [8 + w]
   const uint8_t* __restrict__ Xq = static_cast<const uint8_t*>(A.Xq);
@@ -167,7 +173,7 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
       if (j < m) {
         rnb[h] = A.lnbT[lofs + j];
         rer[h] = A.lerrT[lofs + j];
-        rbd[h] = A.boundT[lofs + j];
+        if constexpr (MODE == kModePool) rbd[h] = A.boundT[lofs + j];
         lid[j] = pl[j];
         chalf[j] = -0.5f * rnb[h];
         mn = fmaxf(mn, rnb[h]);
@@ -206,7 +212,8 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
 #pragma unroll
     for (int h = 0; h < kRowsPerThread; ++h) {
       const int j = tid + h * kPlThreads;
-      if (j < m) {
+      if (MODE == kModeCand && j < m) thrs[j] = rnb[h];
+      if (MODE == kModePool && j < m) {
         const float acut = acut_of(rbd[h], rnb[h], rer[h], nm, emax, d);
         const double ni = sqrt((double)rnb[h] * (1.0 + (d + 4.0) * u));
         const double eta = (2.0 * d + 16.0) * u * (ni + nm) * (ni + nm);
@@ -221,7 +228,9 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
     const int nq = (it.rows + 63) >> 6;
     const int ntile = (m + 31) >> 5;
     int nc = 1, best = INT_MAX;
-    for (int c = 1; c <= ntile && c <= 4 * kPlWaves; ++c) {
+    // candidates: one or two column ranges (each keeps TK keys per lane)
+    const int ncmax = MODE == kModeCand ? 2 : 4 * kPlWaves;
+    for (int c = 1; c <= ntile && c <= ncmax; ++c) {
       const int tp = (ntile + c - 1) / c;
       const int cost = ((nq * c + kPlWaves - 1) / kPlWaves) * (tp + 1);
       if (cost < best) {
@@ -230,7 +239,7 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
       }
     }
     const int tper = (ntile + nc - 1) / nc;
-    nc = (ntile + tper - 1) / tper;
+    nc = (ntile + tper - 1) / tper;  // (candidates: 2 stays 2, as ntile >= 2 there)
     const int ntask = nq * nc;
     for (;;) {
       int task = 0;
@@ -244,6 +253,7 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
       float thr[2];
       int id[2];
       PosList lst[2];
+      uint32_t key[2][MODE == kModeCand ? TK : 1];  // candidates: ascending packed keys
 #pragma unroll
       for (int g = 0; g < 2; ++g) {
         const int r = qb + 32 * g + (lane & 31);
@@ -255,6 +265,8 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
         id[g] = lid[rr];
         if (!valid) thr[g] = INFINITY;  // and its bits are cleared below
         lst[g].reset();
+#pragma unroll
+        for (int s2 = 0; s2 < (MODE == kModeCand ? TK : 1); ++s2) key[g][s2] = 0xFFFFFFFFu;
       }
       const bool valid0 = qb + (lane & 31) < row_end;
       const bool valid1 = qb + 32 + (lane & 31) < row_end;
@@ -300,6 +312,39 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
           for (int e = 0; e < 16; ++e)
             colmask |= (cb + 8 * (e >> 2) + (e & 3) < m ? 1u : 0u) << e;
         }
+        if constexpr (MODE == kModeCand) {
+          // key = approximate distance a = |x~_i|^2 - 2 acc (>= 0) with its low
+          // 11 bits replaced by the leaf position; of each 4 adjacent columns
+          // the smallest key enters the lane's TK smallest (branch-free
+          // insertion: the candidates only seed the bounds, which any distinct
+          // leafmates give)
+#pragma unroll
+          for (int g = 0; g < 2; ++g) {
+            const floatx16& ac = g == 0 ? a0 : a1;
+            uint32_t vm = (g == 0 ? valid0 : valid1) ? colmask : 0u;
+            if (c0 == qb + 32 * g) vm &= ~selfbit;
+#pragma unroll
+            for (int q4 = 0; q4 < 4; ++q4) {
+              uint32_t kk = 0xFFFFFFFFu;
+#pragma unroll
+              for (int u4 = 0; u4 < 4; ++u4) {
+                const int e = 4 * q4 + u4;
+                const float a = fmaxf(fmaf(-2.0f, ac[e], thr[g]), 0.0f);
+                const uint32_t ke = (vm >> e) & 1u
+                                        ? (__float_as_uint(a) & 0xFFFFF800u) | (uint32_t)(cb + 8 * q4 + u4)
+                                        : 0xFFFFFFFFu;
+                kk = min(kk, ke);
+              }
+#pragma unroll
+              for (int s2 = 0; s2 < TK; ++s2) {
+                const uint32_t lo = min(kk, key[g][s2]);
+                kk = max(kk, key[g][s2]);
+                key[g][s2] = lo;
+              }
+            }
+          }
+          return;
+        }
         uint32_t pm0 = valid0 ? pass_bits(a0, thr[0]) & colmask : 0u;
         uint32_t pm1 = valid1 ? pass_bits(a1, thr[1]) & colmask : 0u;
         if (c0 == qb) pm0 &= ~selfbit;
@@ -329,7 +374,27 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
         if (t + 2 < t1) tile(t + 2, a0, a1);
         epilogue(t + 1, b0, b1);
       }
-      flush_pair(A, lid, id, lst);
+      if constexpr (MODE == kModePool) {
+        flush_pair(A, lid, id, lst);
+      } else {
+        // stream (column range cj, half) of a row: key slots (2 cj + half) TK
+        // of its 4 TK; with one column range the other two streams are empty
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+          const int r = qb + 32 * g + (lane & 31);
+          if (r < row_end) {
+            uint64_t* o = A.out_key + ((int64_t)(it.tree - A.tree0) * n + id[g]) * (4 * TK);
+#pragma unroll
+            for (int s2 = 0; s2 < TK; ++s2) {
+              const uint32_t k = key[g][s2];
+              o[(2 * cj + half) * TK + s2] =
+                  k == 0xFFFFFFFFu ? ~0ull
+                                   : ((uint64_t)(k & 0xFFFFF800u) << 32) | (uint32_t)lid[k & 0x7FFu];
+              if (nc == 1) o[(2 + half) * TK + s2] = ~0ull;
+            }
+          }
+        }
+      }
     }
     __syncthreads();  // every wave done with the leaf
     item = begin + misc[parity];
@@ -348,28 +413,45 @@ int pool_leaf_rows(drpt_ctx* ctx, int d) {
   return rows >= 64 ? (int)std::min<int64_t>(rows, 2 * kPlThreads) : 0;
 }
 
-int launch_pool_leaf(drpt_ctx* ctx, const KnnArgs& a0) {
+template <int MODE, int TK>
+const void* pool_leaf_fn(int ks) {
+  switch (ks) {
+    case 1: return (const void*)k_pool_leaf<1, MODE, TK>;
+    case 2: return (const void*)k_pool_leaf<2, MODE, TK>;
+    case 3: return (const void*)k_pool_leaf<3, MODE, TK>;
+    case 4: return (const void*)k_pool_leaf<4, MODE, TK>;
+    case 5: return (const void*)k_pool_leaf<5, MODE, TK>;
+    case 6: return (const void*)k_pool_leaf<6, MODE, TK>;
+    case 7: return (const void*)k_pool_leaf<7, MODE, TK>;
+    default: return (const void*)k_pool_leaf<8, MODE, TK>;
+  }
+}
+
+int launch_pool_leaf(drpt_ctx* ctx, int mode, int kp, const KnnArgs& a0) {
   const int mmax = pool_leaf_rows(ctx, a0.d);
-  if (mmax <= 0 || a0.size_hi > mmax) {
+  if (mmax <= 0 || a0.size_hi > mmax || (mode != kModePool && mode != kModeCand)) {
     set_error("internal: pool_leaf launched for leaves it cannot stage");
     return DRPT_ERR_ARG;
   }
   const size_t lds = (size_t)mmax * (kPlPitch + 12) + kPlMisc * 4;
   const int ks = a0.d / 16;
   const void* fn = nullptr;
-  switch (ks) {
-    case 1: fn = (const void*)k_pool_leaf<1>; break;
-    case 2: fn = (const void*)k_pool_leaf<2>; break;
-    case 3: fn = (const void*)k_pool_leaf<3>; break;
-    case 4: fn = (const void*)k_pool_leaf<4>; break;
-    case 5: fn = (const void*)k_pool_leaf<5>; break;
-    case 6: fn = (const void*)k_pool_leaf<6>; break;
-    case 7: fn = (const void*)k_pool_leaf<7>; break;
-    default: fn = (const void*)k_pool_leaf<8>; break;
+  int slot = ks;
+  if (mode == kModePool) {
+    fn = pool_leaf_fn<kModePool, 1>(ks);
+  } else {
+    const int tk = kp / 4;
+    if (kp != 4 * tk || (tk != 3 && tk != 4 && tk != 6)) {
+      set_error("internal: pool_leaf candidate mode needs k' = 12, 16 or 24");
+      return DRPT_ERR_ARG;
+    }
+    fn = tk == 3 ? pool_leaf_fn<kModeCand, 3>(ks) : tk == 4 ? pool_leaf_fn<kModeCand, 4>(ks)
+                                                            : pool_leaf_fn<kModeCand, 6>(ks);
+    slot = 8 * (tk == 3 ? 1 : tk == 4 ? 2 : 3) + ks;
   }
-  if (!(ctx->pool_leaf_lds_set & (1u << ks))) {
+  if (!(ctx->pool_leaf_lds_set & (1ull << slot))) {
     DRPT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    ctx->pool_leaf_lds_set |= 1u << ks;
+    ctx->pool_leaf_lds_set |= 1ull << slot;
   }
   KnnArgs a = a0;
   unsigned* q;
