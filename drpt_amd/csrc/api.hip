@@ -243,9 +243,10 @@ int drpt_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, cons
     return DRPT_ERR_ARG;
   }
   DRPT_HIP(hipSetDevice(ctx->device));
-  std::vector<int32_t> colptr(L + 1, 0);
+  std::vector<int32_t> colptr(L + 1, 0), ridx32;
   std::vector<uint16_t> rowidx;
   std::vector<float> w;
+  std::vector<double> wd;
   for (int c = 0; c < L; ++c) {
     colptr[c] = (int32_t)rowidx.size();
     for (int j = 0; j < d; ++j) {
@@ -257,13 +258,38 @@ int drpt_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, cons
     }
   }
   colptr[L] = (int32_t)rowidx.size();
-  int32_t* dcp;
+  // the same terms grouped by 4 columns, interleaved, padded per group to a
+  // multiple of 4 terms with (row d: a zero slot, weight 0)
+  const int ngroups = (L + 3) / 4;
+  std::vector<int32_t> gk(ngroups + 1, 0);
+  ridx32.clear();
+  wd.clear();
+  for (int g = 0; g < ngroups; ++g) {
+    int kmax = 0;
+    for (int u = 0; u < 4 && 4 * g + u < L; ++u)
+      kmax = std::max(kmax, colptr[4 * g + u + 1] - colptr[4 * g + u]);
+    kmax = (kmax + 3) & ~3;
+    gk[g + 1] = gk[g] + kmax;
+    for (int k = 0; k < kmax; ++k)
+      for (int u = 0; u < 4; ++u) {
+        const int c = 4 * g + u;
+        const bool has = c < L && k < colptr[c + 1] - colptr[c];
+        ridx32.push_back(has ? (int32_t)rowidx[colptr[c] + k] : d);
+        wd.push_back(has ? (double)w[colptr[c] + k] : 0.0);
+      }
+  }
+  int32_t *dcp, *dri32, *dgk;
   uint16_t* dri;
   float* dw;
+  double* dwd;
   DRPT_CHECK(upload(ctx, "proj.colptr", colptr, &dcp));
   DRPT_CHECK(upload(ctx, "proj.rowidx", rowidx, &dri));
   DRPT_CHECK(upload(ctx, "proj.w", w, &dw));
-  return launch_project(ctx, X, dtype, n, d, dcp, dri, dw, L, colptr[L], P, colmax);
+  DRPT_CHECK(upload(ctx, "proj.gk", gk, &dgk));
+  DRPT_CHECK(upload(ctx, "proj.rowidx32", ridx32, &dri32));
+  DRPT_CHECK(upload(ctx, "proj.wd", wd, &dwd));
+  return launch_project(ctx, X, dtype, n, d, dcp, dri, dw, dgk, dri32, dwd, L, colptr[L], P,
+                        colmax);
 }
 
 int drpt_grow_trees(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t n, int ntrees,

@@ -126,6 +126,108 @@ k_project(const XT* __restrict__ X, int64_t n, int d, int L, const int32_t* __re
     if (lmax[c]) atomicMax(&colmax[c], lmax[c]);
 }
 
+// Wave-uniform columns: the 64 lanes of a wave are the 64 rows of the tile, so
+// each column's compressed entries (row index, weight as double) are read with
+// scalar loads, 16 at a time, and every term is one LDS read of x plus one FMA;
+// four columns run as independent FMA chains. The column maximum of |P| is a DPP wave
+// reduction and one LDS atomic per column and tile.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  // quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror, row_mirror, row_bcast:15,
+  // row_bcast:31: lane 63 ends with the wave's maximum
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false));
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false));
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xF, 0xF, false));
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false));
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x142, 0xA, 0xF, false));
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x143, 0xC, 0xF, false));
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+constexpr int kColsPerPass = 4;   // columns per group (independent FMA chains)
+constexpr int kTermsPerBatch = 4; // terms of each column loaded per batch
+
+// R for this kernel: column groups of kColsPerPass, their terms interleaved
+// (term k of column u of group g at gk[g] * 4 + 4 k + u), each group padded
+// to a multiple of kTermsPerBatch terms with (row d, weight 0): row d of the
+// tile is a zero slot, and fma(0, 0, acc) = acc exactly (acc starts at +0 and
+// so is never -0).
+template <typename XT>
+__global__ void __launch_bounds__(kThreads)
+k_project_rows(const XT* __restrict__ X, int64_t n, int d, int L, const int32_t* __restrict__ gk,
+               const int32_t* __restrict__ ridx, const double* __restrict__ wd,
+               float* __restrict__ P, uint32_t* __restrict__ colmax, int pitch) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint32_t* lmax = reinterpret_cast<uint32_t*>(smem);
+  XT* tile = reinterpret_cast<XT*>(smem + (((size_t)L * 4 + 15) & ~(size_t)15));
+  for (int c = threadIdx.x; c < L; c += kThreads) lmax[c] = 0u;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int ngroups = (L + kColsPerPass - 1) / kColsPerPass;
+  const int64_t ntiles = (n + 63) / 64;
+  for (int64_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
+    const int64_t row0 = tl * 64;
+    const int rows = (int)min((int64_t)64, n - row0);
+    __syncthreads();
+    if (sizeof(XT) == 1 && (d & 3) == 0) {
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(X + row0 * d);
+      uint32_t* dst = reinterpret_cast<uint32_t*>(tile);
+      const int wpr = d >> 2, wpitch = pitch >> 2, total = rows * wpr;
+      for (int e = threadIdx.x; e < total; e += kThreads) {
+        const int rr = e / wpr, c = e - rr * wpr;
+        dst[rr * wpitch + c] = src[e];
+      }
+    } else {
+      const XT* src = X + row0 * d;
+      const int total = rows * d;
+      for (int e = threadIdx.x; e < total; e += kThreads) {
+        const int rr = e / d, c = e - rr * d;
+        tile[rr * pitch + c] = src[e];
+      }
+    }
+    if (threadIdx.x < 64) tile[threadIdx.x * pitch + d] = (XT)0;  // the zero slot
+    __syncthreads();
+    const XT* xr = tile + lane * pitch;
+    const bool valid = lane < rows;
+    for (int g = wave; g < ngroups; g += 4) {
+      const int k0 = gk[g], k1 = gk[g + 1];
+      double acc[kColsPerPass];
+#pragma unroll
+      for (int u = 0; u < kColsPerPass; ++u) acc[u] = 0.0;
+      // x * w is exact in double, so one fused multiply-add per term rounds as
+      // the separate product and sum would; each column's terms in ascending row
+      for (int k = k0; k < k1; k += kTermsPerBatch) {
+        int ri[kTermsPerBatch * kColsPerPass];
+        double wv[kTermsPerBatch * kColsPerPass];
+#pragma unroll
+        for (int i = 0; i < kTermsPerBatch * kColsPerPass; ++i) {
+          ri[i] = ridx[k * kColsPerPass + i];
+          wv[i] = wd[k * kColsPerPass + i];
+        }
+        double xv[kTermsPerBatch * kColsPerPass];
+#pragma unroll
+        for (int i = 0; i < kTermsPerBatch * kColsPerPass; ++i) xv[i] = (double)xr[ri[i]];
+#pragma unroll
+        for (int t = 0; t < kTermsPerBatch; ++t)
+#pragma unroll
+          for (int u = 0; u < kColsPerPass; ++u)
+            acc[u] = __fma_rn(xv[t * kColsPerPass + u], wv[t * kColsPerPass + u], acc[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < kColsPerPass; ++u) {
+        const int c = g * kColsPerPass + u;
+        if (c >= L) break;
+        const float pv = (float)acc[u];
+        if (valid) P[(int64_t)c * n + row0 + lane] = pv;
+        const uint32_t m = wave_max_u32(valid ? (__float_as_uint(pv) & 0x7fffffffu) : 0u);
+        if (lane == 0 && m) atomicMax(&lmax[c], m);
+      }
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < L; c += kThreads)
+    if (lmax[c]) atomicMax(&colmax[c], lmax[c]);
+}
+
 constexpr size_t kLdsBudget = 64 * 1024;
 
 // X tile row pitch in elements (odd word pitch: rows land on distinct banks)
@@ -138,6 +240,10 @@ int pitch_of(int d) {
   }
   return d | 1;
 }
+
+// tile pitch of the wave-uniform kernel: one more element for the zero slot
+template <typename XT>
+int pitch_rows(int d) { return pitch_of<XT>(d + 1); }
 
 template <typename XT, int TR, bool RL>
 int run(drpt_ctx* ctx, const XT* X, int64_t n, int d, const int32_t* colptr, const uint16_t* rowidx,
@@ -189,11 +295,42 @@ int dispatch(drpt_ctx* ctx, const XT* X, int64_t n, int d, const int32_t* colptr
 
 }  // namespace
 
+template <typename XT>
+int run_rows(drpt_ctx* ctx, const XT* X, int64_t n, int d, const int32_t* gk,
+             const int32_t* ridx, const double* wd, int L, float* P, uint32_t* colmax) {
+  const int pitch = pitch_rows<XT>(d);
+  const size_t lds = (((size_t)L * 4 + 15) & ~(size_t)15) + (size_t)64 * pitch * sizeof(XT);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_project_rows<XT>, kThreads, lds) !=
+          hipSuccess ||
+      per_cu < 1) {
+    (void)hipGetLastError();
+    per_cu = 2;
+  }
+  const int64_t blocks = std::min<int64_t>((n + 63) / 64, (int64_t)ctx->num_cus * per_cu);
+  hipLaunchKernelGGL((k_project_rows<XT>), dim3((unsigned)blocks), dim3(kThreads), lds, ctx->stream,
+                     X, n, d, L, gk, ridx, wd, P, colmax, pitch);
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
+}
+
+template <typename XT>
+bool rows_fit(int d, int L) {
+  return (((size_t)L * 4 + 15) & ~(size_t)15) + (size_t)64 * pitch_rows<XT>(d) * sizeof(XT) <=
+         kLdsBudget;
+}
+
 int launch_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, const int32_t* colptr,
-                   const uint16_t* rowidx, const float* w, int L, int nnz, float* P,
-                   uint32_t* colmax) {
+                   const uint16_t* rowidx, const float* w, const int32_t* gk, const int32_t* ridx32,
+                   const double* wd, int L, int nnz, float* P, uint32_t* colmax) {
   DRPT_CHECK(fill_u32(ctx, colmax, 0u, (size_t)L));
   if (n <= 0) return DRPT_OK;
+  if (gk && ridx32 && wd) {  // 64-row tiles with wave-uniform columns where they fit the LDS
+    if (dtype == DRPT_U8 && rows_fit<uint8_t>(d, L))
+      return run_rows<uint8_t>(ctx, (const uint8_t*)X, n, d, gk, ridx32, wd, L, P, colmax);
+    if (dtype != DRPT_U8 && rows_fit<float>(d, L))
+      return run_rows<float>(ctx, (const float*)X, n, d, gk, ridx32, wd, L, P, colmax);
+  }
   if (dtype == DRPT_U8)
     return dispatch<uint8_t>(ctx, (const uint8_t*)X, n, d, colptr, rowidx, w, L, nnz, P, colmax);
   return dispatch<float>(ctx, (const float*)X, n, d, colptr, rowidx, w, L, nnz, P, colmax);
