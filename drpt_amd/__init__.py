@@ -17,7 +17,7 @@ import numpy as np
 from . import _lib
 from ._lib import (DRPT_BF16, DRPT_F32, DRPT_FP32, DRPT_FP8, DRPT_LOCAL_LEAFMAX,
                    DRPT_LOCAL_REFERENCE, DRPT_TREE_FIXED, DRPT_TREE_LEAFMAX, DRPT_U8, DrptError,
-                   Params, Timings)
+                   Params, Phases, Timings)
 
 __all__ = [
     "Context", "Params", "Timings", "DrptError", "derive_params", "build_projection",
@@ -184,6 +184,12 @@ class Context:
         _lib.check(self.lib.drpt_build_graph(self.h, ctypes.byref(params), _ptr(X), _ptr(oi),
                                              _ptr(od), ctypes.byref(tm)))
         return oi, od, tm
+
+    def last_phases(self):
+        """drpt_last_phases: stage times of the last build_graph."""
+        ph = Phases()
+        _lib.check(self.lib.drpt_last_phases(self.h, ctypes.byref(ph)))
+        return ph.as_dict()
 
     def autotune(self, params, X):
         """drpt_autotune: the fastest work-item layout per leaf-size bucket

@@ -27,7 +27,7 @@ EXPORTS = [
     "drpt_last_error", "drpt_version", "drpt_create", "drpt_destroy", "drpt_sync",
     "drpt_get_stream", "drpt_derive_params", "drpt_build_projection", "drpt_project",
     "drpt_grow_trees", "drpt_tree_knn", "drpt_merge", "drpt_build_graph", "drpt_autotune",
-    "drpt_set_item_plan", "drpt_grow_trees_noalloc",
+    "drpt_set_item_plan", "drpt_grow_trees_noalloc", "drpt_last_phases",
 ]
 
 
@@ -59,6 +59,20 @@ class Timings(ctypes.Structure):
         ("gram_flops", ctypes.c_double), ("knn_kernel_ms", ctypes.c_double),
         ("n_leaves", ctypes.c_int64), ("n_fallback_rows", ctypes.c_int64),
         ("max_leaf", ctypes.c_int64), ("reserved", ctypes.c_int64 * 4),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class Phases(ctypes.Structure):
+    _fields_ = [
+        ("group_ms", ctypes.c_double), ("prep_ms", ctypes.c_double), ("cand_ms", ctypes.c_double),
+        ("known_ms", ctypes.c_double), ("pool_ms", ctypes.c_double),
+        ("pool_merge_ms", ctypes.c_double), ("bound_ms", ctypes.c_double),
+        ("exact_ms", ctypes.c_double), ("tree_groups", ctypes.c_int64),
+        ("first_trees", ctypes.c_int64), ("pool_cap", ctypes.c_int64), ("rerank", ctypes.c_int64),
+        ("reserved", ctypes.c_int64 * 4),
     ]
 
     def as_dict(self):
@@ -103,6 +117,7 @@ def load():
     L.drpt_build_graph.argtypes = [vp, ctypes.POINTER(Params), vp, vp, vp, ctypes.POINTER(Timings)]
     L.drpt_autotune.argtypes = [vp, ctypes.POINTER(Params), vp, vp]
     L.drpt_set_item_plan.argtypes = [vp, ctypes.POINTER(Params), vp]
+    L.drpt_last_phases.argtypes = [vp, ctypes.POINTER(Phases)]
     for name in EXPORTS:
         fn = getattr(L, name)
         if fn.restype is ctypes.c_int or name in ("drpt_create", "drpt_destroy", "drpt_sync"):

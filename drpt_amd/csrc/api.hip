@@ -164,6 +164,15 @@ int drpt_create(drpt_ctx** out, int device) {
     set_error("hipStreamCreate failed");
     return DRPT_ERR_HIP;
   }
+  for (auto& e : ctx->evp) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      (void)hipGetLastError();
+      e = nullptr;
+      drpt_destroy(ctx);
+      set_error("hipEventCreate failed");
+      return DRPT_ERR_HIP;
+    }
+  }
   for (auto& e : ctx->ev) {
     if (hipEventCreate(&e) != hipSuccess) {
       (void)hipGetLastError();
@@ -185,6 +194,8 @@ int drpt_destroy(drpt_ctx* ctx) {
     if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
   for (auto& kv : ctx->bufs) (void)hipFree(kv.second.first);
   for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->evp)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->ev_pass) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
@@ -772,6 +783,7 @@ static int lowp_setup(drpt_ctx* ctx, const drpt_params* p, const void* X,
   r->pool_cap = pool_leaf_rows(ctx, p->d) > 0 ? std::max(512, pool_cap_of(r->K)) : pool_cap_of(r->K);
   const int64_t n = r->n;
   DRPT_CHECK(group_impl(ctx, p, X, leaf_heap, plan, r->first, &r->G));
+  DRPT_HIP(hipEventRecord(ctx->evp[0], ctx->stream));
   DRPT_CHECK(prep_lowp(ctx, &r->pp, X, r->G.perm, &r->a));
   if (r->pc.precision != r->pp.precision) {
     DRPT_CHECK(prep_lowp(ctx, &r->pc, X, r->G.perm, &r->ac));
@@ -851,6 +863,7 @@ static int lowp_known(drpt_ctx* ctx, LowpRun* r, const void* X, const ItemPlan& 
   r->ac.out_key = r->s1;
   r->ac.tree0 = 0;
   DRPT_CHECK(lowp_pass(ctx, r, kModeCand, plan, r->G.range + 0));
+  DRPT_HIP(hipEventRecord(ctx->evp[1], ctx->stream));
   RowMergeArgs m{};
   m.X = X;
   m.n = r->n;
@@ -866,7 +879,9 @@ static int lowp_known(drpt_ctx* ctx, LowpRun* r, const void* X, const ItemPlan& 
   m.out_idx = r->ki;
   m.out_dist = r->kd;
   m.order = r->G.perm;  // tree 0's leaf order
-  return launch_row_merge(ctx, r->pp.dtype, m);
+  DRPT_CHECK(launch_row_merge(ctx, r->pp.dtype, m));
+  DRPT_HIP(hipEventRecord(ctx->evp[2], ctx->stream));
+  return DRPT_OK;
 }
 
 static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
@@ -903,7 +918,7 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
     DRPT_CHECK(ws(ctx, "knn.heavy", (size_t)n, &heavy));
     DRPT_CHECK(ws(ctx, "knn.nheavy", 1, &nheavy));
   }
-  while ((int)ctx->ev_pass.size() < 2 * ngroups) {
+  while ((int)ctx->ev_pass.size() < 4 * ngroups) {
     hipEvent_t e;
     DRPT_HIP(hipEventCreate(&e));
     ctx->ev_pass.push_back(e);
@@ -915,13 +930,14 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   for (int g = 0; g < ngroups; ++g) {
     const int t0 = g * gsz, t1 = std::min(T, t0 + gsz);
     const bool last = g + 1 == ngroups;
+    DRPT_HIP(hipEventRecord(ctx->ev_pass[4 * g + 3], s));
     DRPT_CHECK(launch_leaf_gather_val(ctx, G.perm + (int64_t)t0 * n, (int64_t)(t1 - t0) * n, kdn, K,
                                       K - 1, r.boundT + (int64_t)t0 * n));
     if (g > 0) DRPT_CHECK(fill_u32(ctx, (uint32_t*)r.pcnt, 0u, n));
-    DRPT_HIP(hipEventRecord(ctx->ev_pass[2 * g], s));
+    DRPT_HIP(hipEventRecord(ctx->ev_pass[4 * g], s));
     r.a.known = kin;
     DRPT_CHECK(lowp_pass(ctx, &r, kModePool, plan, granges + 2 * g));
-    DRPT_HIP(hipEventRecord(ctx->ev_pass[2 * g + 1], s));
+    DRPT_HIP(hipEventRecord(ctx->ev_pass[4 * g + 1], s));
     RowMergeArgs f{};
     f.X = X;
     f.n = n;
@@ -941,6 +957,7 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
     f.heavy = heavy;
     f.nheavy = nheavy;
     DRPT_CHECK(launch_row_merge(ctx, p->dtype, f));
+    DRPT_HIP(hipEventRecord(ctx->ev_pass[4 * g + 2], s));
     kin = f.out_idx;
     kdn = f.out_dist;
   }
@@ -952,12 +969,26 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
                                 out_idx, out_dist));
   DRPT_HIP(hipEventRecord(ctx->ev[5], s));
   DRPT_CHECK(leaf_stats(ctx, p, G, timings));
+  drpt_phases& ph = ctx->last_phases;
+  ph = drpt_phases{};
+  ph.group_ms = ms_between(ctx->ev[6], ctx->evp[0]);
+  ph.prep_ms = ms_between(ctx->evp[0], ctx->ev[7]);
+  ph.cand_ms = ms_between(ctx->ev[7], ctx->evp[1]);
+  ph.known_ms = ms_between(ctx->evp[1], ctx->evp[2]);
+  for (int g = 0; g < ngroups; ++g) {
+    ph.bound_ms += ms_between(ctx->ev_pass[4 * g + 3], ctx->ev_pass[4 * g]);
+    ph.pool_ms += ms_between(ctx->ev_pass[4 * g], ctx->ev_pass[4 * g + 1]);
+    ph.pool_merge_ms += ms_between(ctx->ev_pass[4 * g + 1], ctx->ev_pass[4 * g + 2]);
+  }
+  ph.exact_ms = ms_between(ctx->ev_pass[4 * (ngroups - 1) + 2], ctx->ev[5]);
+  ph.tree_groups = ngroups;
+  ph.first_trees = r.first;
+  ph.pool_cap = r.pool_cap;
+  ph.rerank = plan.rerank > 0 ? std::max(K, std::min(plan.rerank, 2 * K)) : 2 * K;
   if (timings) {
     timings->group_ms = ms_between(ctx->ev[6], ctx->ev[7]);
     timings->knn_ms = ms_between(ctx->ev[7], ctx->ev[5]);
-    timings->knn_kernel_ms = 0.0;
-    for (int g = 0; g < ngroups; ++g)
-      timings->knn_kernel_ms += ms_between(ctx->ev_pass[2 * g], ctx->ev_pass[2 * g + 1]);
+    timings->knn_kernel_ms = ph.pool_ms;
     timings->n_fallback_rows = nov;
   }
   return DRPT_OK;
@@ -1019,6 +1050,7 @@ int drpt_build_graph(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t
   uint32_t* heap;
   DRPT_CHECK(build_front(ctx, p, X, &heap));
   drpt_timings tk{};
+  bool fp32 = false;
   if (p->precision == DRPT_FP32) {
     // per-tree lists (MDRPT::calculate_nns), then the merge (:180-198)
     int32_t* ci;
@@ -1028,6 +1060,7 @@ int drpt_build_graph(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t
     DRPT_CHECK(tree_knn_impl(ctx, p, X, heap, ci, cd, &tk));
     DRPT_HIP(hipEventRecord(ctx->ev[3], s));
     DRPT_CHECK(launch_merge(ctx, n, T, K, ci, cd, out_idx, out_dist));
+    fp32 = true;
   } else {
     // the merged lists directly (the merge is the last row merge)
     DRPT_CHECK(graph_knn_lowp(ctx, p, X, heap, out_idx, out_dist, &tk));
@@ -1035,6 +1068,14 @@ int drpt_build_graph(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t
   }
   DRPT_HIP(hipEventRecord(ctx->ev[4], s));
   DRPT_HIP(hipStreamSynchronize(s));
+  if (fp32) {
+    drpt_phases& ph = ctx->last_phases;
+    ph = drpt_phases{};
+    ph.group_ms = tk.group_ms;
+    ph.pool_ms = tk.knn_kernel_ms;
+    ph.exact_ms = ms_between(ctx->ev[8], ctx->ev[5]);
+    ph.pool_merge_ms = ms_between(ctx->ev[3], ctx->ev[4]);
+  }
   if (timings) {
     *timings = tk;
     timings->project_ms = ms_between(ctx->ev[0], ctx->ev[1]);
@@ -1190,6 +1231,15 @@ int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* p
   }
   store();
   DRPT_HIP(hipStreamSynchronize(s));
+  return DRPT_OK;
+}
+
+int drpt_last_phases(drpt_ctx* ctx, drpt_phases* out) {
+  if (!ctx || !out) {
+    set_error("last_phases: null pointer");
+    return DRPT_ERR_ARG;
+  }
+  *out = ctx->last_phases;
   return DRPT_OK;
 }
 

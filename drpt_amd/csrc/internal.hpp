@@ -42,7 +42,9 @@ struct drpt_ctx {
   hipStream_t stream = nullptr;
   std::map<std::string, std::pair<void*, size_t>> bufs;
   hipEvent_t ev[12] = {};
+  hipEvent_t evp[4] = {};           // stage boundaries inside the leaf k-NN (drpt_last_phases)
   std::vector<hipEvent_t> ev_pass;  // per-pass timing events (grown on demand)
+  drpt_phases last_phases{};        // stage times of the last drpt_build_graph
   // captured launch sequences (hipGraph), replayed while their key (sizes,
   // parameters and every buffer address they touch) is unchanged
   struct Graph {

@@ -71,6 +71,23 @@ typedef struct {
   int64_t reserved[4];
 } drpt_timings;
 
+/* Stage times of the last drpt_build_graph on a context (HIP events on its
+ * stream). BF16 / FP8 builds: grouping, quantised rows and leaf-ordered norms,
+ * candidate pass, first row merge (known lists), pooled filter passes, the
+ * row merges after them, leaf-ordered bound gathers, exact row path. FP32
+ * builds: grouping, the exact leaf k-NN kernel (pool_ms), the cross-tree merge
+ * (pool_merge_ms) and the fallback rows (exact_ms). No counterpart in the
+ * reference, which times index building and querying as two spans
+ * (main.cpp:196-220). */
+typedef struct {
+  double group_ms, prep_ms, cand_ms, known_ms, pool_ms, pool_merge_ms, bound_ms, exact_ms;
+  int64_t tree_groups;  /* pooled filter passes (trees per pass: ntrees / tree_groups, rounded up) */
+  int64_t first_trees;  /* trees whose widened candidates gave the first bounds */
+  int64_t pool_cap;     /* pool slots per row */
+  int64_t rerank;       /* exact re-rank depth of the first bounds */
+  int64_t reserved[4];
+} drpt_phases;
+
 /* Thread-local message of the last failing call. */
 const char* drpt_last_error(void);
 int drpt_version(void);
@@ -142,6 +159,9 @@ int drpt_merge(drpt_ctx* ctx, int64_t n, int T, int K, const int32_t* cand_idx,
  * FP32 squared distances (-1 / +inf padding). */
 int drpt_build_graph(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* out_idx,
                      float* out_dist, drpt_timings* timings);
+
+/* Copies the stage times of the context's last drpt_build_graph to out. */
+int drpt_last_phases(drpt_ctx* ctx, drpt_phases* out);
 
 /* Work-item autotuner (no counterpart in the reference, whose leaf loop is
  * one OpenMP loop over leaves, mdrpt.cpp:123-212). For a BF16/FP8 workload
