@@ -26,7 +26,7 @@ HIP_FLAGS = [
     "-Wno-unused-result", "-Wno-pass-failed",
 ]
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall"]
-CLI_FLAGS = ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
+CLI_FLAGS = ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-pthread"]
 
 HIP_SOURCES = ["api.hip", "util.hip", "project.hip", "split.hip", "group.hip", "leafknn.hip",
                "leafpool.hip", "merge.hip"]
@@ -66,7 +66,7 @@ def build(verbose=False):
     _run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-o", tmp, *lib_objs])
     os.replace(tmp, LIB)
     cli_objs = [_obj(s) for s in CLI_SOURCES]
-    _run(["g++", "-o", CLI, *cli_objs, f"-L{HERE}", "-ldrpt_hip", "-Wl,-rpath,$ORIGIN",
+    _run(["g++", "-pthread", "-o", CLI, *cli_objs, f"-L{HERE}", "-ldrpt_hip", "-Wl,-rpath,$ORIGIN",
           "-L/opt/rocm/lib", "-lamdhip64"])
     return LIB
 

@@ -92,3 +92,44 @@ def test_cli_fbin_bf16_autotune_matches_oracle(tmp_path, orc):
                                   density=p.density)
     got = open(out + "results.txt").read()
     assert got == expected_edges(ri, 9)
+
+
+@pytest.mark.gpu
+def test_cli_full_mnist_like_matches_oracle(tmp_path, orc):
+    """The reference configuration at full size: 60000 x 784 uint8 idx3, T = 8,
+    nn = 10 (the reference's tree depth and local stage): results.txt equals
+    the oracle's lists in the reference's edge-list format, byte for byte."""
+    from drpt_amd import datagen
+    X = datagen.mnist_like(60000)
+    path = tmp_path / "mnist.idx3-ubyte"
+    datagen.write_idx3_ubyte(str(path), X, 28, 28)
+    out = str(tmp_path) + "/"
+    print("[cli] mnist 60000 full", flush=True)
+    r = _run(["-input", str(path), "-output", out, "-data-set-size", "60000", "-dimension", "784",
+              "-ntrees", "8", "-nn", "10", "-seed", "3"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    print(r.stdout.strip().splitlines()[-1], flush=True)
+    D = orc.global_depth(orc.tree_depth(60000))
+    ri, _, _, _ = orc.build_graph(X, 8, D, 10, local_mode=0, seed=3)
+    got = open(out + "results.txt").read()
+    assert got == expected_edges(ri, 9)
+
+
+@pytest.mark.gpu
+def test_cli_integer_density_real_local_tree(tmp_path, orc):
+    """-density 1 (parsed as an integer, as main.cpp:69-72 does): dense R and a
+    dense local matrix, the real-key local tree; results.txt equals the
+    oracle's lists."""
+    from drpt_amd import datagen
+    X = datagen.mnist_like(3000)
+    path = tmp_path / "mnist.idx3-ubyte"
+    datagen.write_idx3_ubyte(str(path), X, 28, 28)
+    out = str(tmp_path) + "/"
+    print("[cli] mnist 3000 -density 1", flush=True)
+    r = _run(["-input", str(path), "-output", out, "-data-set-size", "3000", "-dimension", "784",
+              "-ntrees", "5", "-nn", "10", "-seed", "23", "-density", "1"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    D = orc.global_depth(orc.tree_depth(3000))
+    ri, _, _, _ = orc.build_graph(X, 5, D, 10, local_mode=0, seed=23, density=1.0)
+    got = open(out + "results.txt").read()
+    assert got == expected_edges(ri, 9)
