@@ -97,12 +97,14 @@ def _exact_ok_fp32(X_dev):
     return float((Xf.double() ** 2).sum(1).max()) < 2.0 ** 23
 
 
-def exact_knn_kth(X_dev, queries, K, chunk=2048):
+def exact_knn_kth(X_dev, queries, K, budget_bytes=4 << 30):
     """The exact K-th smallest non-self squared distance of every query row
-    (FP32 products when exact for this data, FP64 otherwise)."""
+    (FP32 products when exact for this data, FP64 otherwise), queries in
+    chunks whose distance block stays within budget_bytes."""
     import torch
     exact32 = _exact_ok_fp32(X_dev)
     dt = torch.float32 if exact32 else torch.float64
+    chunk = max(16, min(2048, budget_bytes // (X_dev.shape[0] * (4 if exact32 else 8))))
     Xc = X_dev.to(dt)
     x2 = (Xc * Xc).sum(1)
     out = torch.empty(len(queries), dtype=torch.float64, device=X_dev.device)
