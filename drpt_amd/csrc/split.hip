@@ -11,13 +11,16 @@
 // histogram, median) lives in small [T][2^l] arrays.
 //
 // Per level, all trees in one launch each:
-//   k_sum1   count + sum of quantised values per node      (per point)
+//   k_sum1   count + sum of quantised values per node (root level only)
 //   k_mu     split decision + mean                          (per node)
 //   k_sum2   sum of quantised fl(v-mu)^2 per node            (per point)
 //   k_edges  variance, sigma, edge table                     (per node)
-//   k_hist   16-slot histogram (wave-ballot when a wave shares a node)
+//   k_hist   16-slot histogram (wave-ballot when a wave shares a node; the
+//            tree's edge tables staged in LDS where they fit)
 //   k_median cumulative search + interpolation               (per node)
-//   k_part   v <= median -> 2h, else 2h+1                     (per point)
+//   k_part_sum1  v <= median -> 2h, else 2h+1, and the child's count and
+//            sum of its next-level values (k_sum1 of level l + 1); k_part
+//            alone at the last split level                  (per point)
 //
 // Arithmetic (bit-exact with the oracle's canonical statistics):
 //   q(v) = rint(v * 2^-kq1) as int64; S = sum q (order-independent);
@@ -28,6 +31,7 @@
 //
 // Roofline: HBM-bound; algorithmic bytes 12 per point per tree per level
 // (4 B value + 4 B node read + 4 B node write).
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 
@@ -39,10 +43,11 @@ namespace {
 
 constexpr int kThreads = 256;
 // per-node LDS arrays of the LDS variants (bytes): sums 12 (sum1: count +
-// int64 sum; sum2: 8), histograms 64 (16 slots); used while they fit
-// kSumLdsMax / kHistLdsMax and the device's per-workgroup LDS
+// int64 sum; sum2: 8), histograms 64 (16 slots), plus for the histogram the
+// edge table (64) and split flag (4) when they fit as well; used while they
+// fit kSumLdsMax / kHistLdsMax and the device's per-workgroup LDS
 constexpr size_t kSumLdsMax = 96 * 1024;
-constexpr size_t kHistLdsMax = 128 * 1024;
+constexpr size_t kHistLdsMax = 136 * 1024;
 
 __device__ __forceinline__ void col_exponents(uint32_t cm, int bitlen, int& kq1, int& kq2) {
   int E = (int)((cm >> 23) & 0xffu);
@@ -294,35 +299,49 @@ __global__ void k_edges(int T, int l, int D, int bitlen, const uint32_t* __restr
   dst[3] = make_float4(e[12], e[13], e[14], e[15]);
 }
 
-// Slot of v: math_operations.cpp:292-308 (no slot matched -> slot 0).
+// Slot of v: math_operations.cpp:292-308 -- the j in 1..15 with
+// E[j-1] < v <= E[j], else 0. E[0..13] is non-decreasing (the steps sigma/30,
+// sigma/6, sigma/2 are monotone and rounding keeps their order) and
+// E[14] = E[15] = 0, so with c = #{j <= 13 : E[j] < v}: c in 1..13 is the slot,
+// c = 0 gives 0, c = 14 gives 14 when v <= E[14] (= 0) and 0 otherwise (slot 15
+// is never matched); NaN values or edges compare false and give 0, as there.
 __device__ __forceinline__ int slot_of(float v, const float* e) {
-  int b = 0;
+  int c = 0;
 #pragma unroll
-  for (int j = 1; j < 16; ++j) {
-    bool m = (e[j - 1] < v) && (e[j] >= v);
-    b = m ? j : b;
-  }
-  return b;
+  for (int j = 0; j < 14; ++j) c += e[j] < v ? 1 : 0;
+  if (c == 14) c = v <= e[14] ? 14 : 0;
+  return c;
 }
 
-template <bool kLds>
+template <bool kLds, bool kEL>
 __global__ void __launch_bounds__(kThreads)
 k_hist(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n, int D, int l,
        const float* __restrict__ E, const uint32_t* __restrict__ split, uint32_t* __restrict__ H,
        int64_t ppb) {
+  // kLds: histograms [NN][16] in LDS; kEL: also the tree's edge tables
+  // [NN][16] and split flags [NN] (staged once per block, read per point)
   extern __shared__ __align__(16) unsigned char smem[];
   const int t = blockIdx.y;
   const int NN = 1 << l;
   uint32_t* lH = reinterpret_cast<uint32_t*>(smem);
+  float* lE = reinterpret_cast<float*>(lH + (kLds ? NN * 16 : 0));
+  uint32_t* lS = reinterpret_cast<uint32_t*>(lE + (kEL ? NN * 16 : 0));
+  const float* Eg = E + (int64_t)t * NN * 16;
+  const uint32_t* Sg = split + (int64_t)t * NN;
   if (kLds) {
     for (int k = threadIdx.x; k < NN * 16; k += kThreads) lH[k] = 0;
+    if (kEL) {
+      for (int k = threadIdx.x; k < NN * 4; k += kThreads)
+        reinterpret_cast<float4*>(lE)[k] = reinterpret_cast<const float4*>(Eg)[k];
+      for (int k = threadIdx.x; k < NN; k += kThreads) lS[k] = Sg[k];
+    }
     __syncthreads();
   }
   const int64_t col = (int64_t)t * D + l;
   const float* Pc = P + col * n;
   const uint32_t* nt = node + (int64_t)t * n;
-  const float* Et = E + (int64_t)t * NN * 16;
-  const uint32_t* st = split + (int64_t)t * NN;
+  const float* Et = kEL ? lE : Eg;
+  const uint32_t* st = kEL ? lS : Sg;
   uint32_t* gH = H + (int64_t)t * NN * 16;
   int64_t b0, b1;
   block_range(n, ppb, b0, b1);
@@ -442,6 +461,128 @@ k_part(const float* __restrict__ P, uint32_t* __restrict__ node, int64_t n, int 
   }
 }
 
+// k_part of level l fused with k_sum1 of level l + 1: each point of a split
+// node moves to its child (v <= median -> 2h, else 2h + 1) and, at once, adds
+// its count and quantised level-(l+1) value to the child's sums (points of
+// unsplit nodes are leaves and take no further part). kLds: the child sums
+// in LDS ([2^(l+1)] int64 sums + counts).
+template <bool kLds>
+__global__ void __launch_bounds__(kThreads)
+k_part_sum1(const float* __restrict__ P, uint32_t* __restrict__ node, int64_t n, int D, int l,
+            const float* __restrict__ med, const uint32_t* __restrict__ split,
+            const uint32_t* __restrict__ colmax, int bitlen, uint32_t* __restrict__ cnt,
+            unsigned long long* __restrict__ S1, int64_t ppb) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int t = blockIdx.y;
+  const int NN = 1 << l, NC = 2 << l;
+  unsigned long long* lS = reinterpret_cast<unsigned long long*>(smem);
+  uint32_t* lc = reinterpret_cast<uint32_t*>(lS + (kLds ? NC : 0));
+  if (kLds) {
+    for (int k = threadIdx.x; k < NC; k += kThreads) { lS[k] = 0; lc[k] = 0; }
+    __syncthreads();
+  }
+  const float* Pc = P + ((int64_t)t * D + l) * n;
+  const float* Pn = Pc + n;  // level l + 1 of the same tree
+  int kq1, kq2;
+  col_exponents(colmax[(int64_t)t * D + l + 1], bitlen, kq1, kq2);
+  uint32_t* nt = node + (int64_t)t * n;
+  const float* mt = med + (int64_t)t * NN;
+  const uint32_t* st = split + (int64_t)t * NN;
+  uint32_t* gc = cnt + (int64_t)t * NC;
+  unsigned long long* gS = S1 + (int64_t)t * NC;
+  int64_t b0, b1;
+  block_range(n, ppb, b0, b1);
+  const int lane = threadIdx.x & 63;
+  for (int64_t base = b0; base < b1; base += kThreads * kUnroll) {
+    uint32_t hu[kUnroll];
+    float pu[kUnroll];
+    load_points(nt, Pc, base, b1, hu, pu);
+    float pn[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int64_t i = base + u * kThreads + threadIdx.x;
+      pn[u] = (i < b1 && (hu[u] >> l) == 1u) ? Pn[i] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int64_t i = base + u * kThreads + threadIdx.x;
+      bool act = false;
+      int nd = 0;
+      long long q = 0;
+      if (i < b1) {
+        const uint32_t h = hu[u];
+        if ((h >> l) == 1u) {
+          const int np = (int)(h - (uint32_t)NN);
+          if (st[np]) {
+            const uint32_t hc = 2u * h + ((pu[u] <= mt[np]) ? 0u : 1u);
+            nt[i] = hc;
+            act = true;
+            nd = (int)(hc - (uint32_t)NC);
+            q = quant(pn[u], kq1);
+          }
+        }
+      }
+      unsigned long long am = __ballot(act);
+      if (am == 0) continue;
+      int leader = __ffsll((long long)am) - 1;
+      int ndl = __shfl(nd, leader);
+      unsigned long long same = __ballot(act && nd == ndl);
+      if (same == am) {
+        long long s = wave_sum_i64(q);
+        if (lane == leader) {
+          if (kLds) {
+            atomicAdd(&lc[ndl], (uint32_t)__popcll(am));
+            atomicAdd(&lS[ndl], (unsigned long long)s);
+          } else {
+            atomicAdd(&gc[ndl], (uint32_t)__popcll(am));
+            atomicAdd(&gS[ndl], (unsigned long long)s);
+          }
+        }
+      } else if (act) {
+        if (kLds) {
+          atomicAdd(&lc[nd], 1u);
+          atomicAdd(&lS[nd], (unsigned long long)q);
+        } else {
+          atomicAdd(&gc[nd], 1u);
+          atomicAdd(&gS[nd], (unsigned long long)q);
+        }
+      }
+    }
+  }
+  if (kLds) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < NC; k += kThreads) {
+      if (lc[k]) {
+        atomicAdd(&gc[k], lc[k]);
+        atomicAdd(&gS[k], lS[k]);
+      }
+    }
+  }
+}
+
+// up to four fills of 32-bit words in one launch (capture-safe)
+__global__ void k_fill4(uint32_t* __restrict__ p0, size_t c0, uint32_t* __restrict__ p1, size_t c1,
+                        uint32_t* __restrict__ p2, size_t c2, uint32_t* __restrict__ p3, size_t c3) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < c0 + c1 + c2 + c3; i += stride) {
+    if (i < c0) p0[i] = 0u;
+    else if (i < c0 + c1) p1[i - c0] = 0u;
+    else if (i < c0 + c1 + c2) p2[i - c0 - c1] = 0u;
+    else p3[i - c0 - c1 - c2] = 0u;
+  }
+}
+
+int zero4(drpt_ctx* ctx, uint32_t* p0, size_t c0, uint32_t* p1, size_t c1, uint32_t* p2 = nullptr,
+          size_t c2 = 0, uint32_t* p3 = nullptr, size_t c3 = 0) {
+  const size_t total = c0 + c1 + c2 + c3;
+  if (total == 0) return DRPT_OK;
+  const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill4, dim3(blocks), dim3(256), 0, ctx->stream, p0, c0, p1, c1, p2, c2, p3,
+                     c3);
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
+}
+
 int split_levels(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t n, int T, int D,
                  int mode, int leaf_max, int bitlen, uint32_t* node, uint32_t* cnt,
                  uint32_t* split, uint32_t* H, unsigned long long* S1, unsigned long long* S2,
@@ -465,8 +606,12 @@ static void raise_split_lds(drpt_ctx* ctx) {
                             bytes) == hipSuccess &&
         hipFuncSetAttribute((const void*)k_sum2<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             bytes) == hipSuccess &&
-        hipFuncSetAttribute((const void*)k_hist<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            hbytes) == hipSuccess)
+        hipFuncSetAttribute((const void*)k_part_sum1<true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess &&
+        hipFuncSetAttribute((const void*)k_hist<true, false>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, hbytes) == hipSuccess &&
+        hipFuncSetAttribute((const void*)k_hist<true, true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, hbytes) == hipSuccess)
       ctx->split_lds_big = true;
     else
       (void)hipGetLastError();
@@ -519,36 +664,37 @@ int split_levels(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t 
                  uint32_t* split, uint32_t* H, unsigned long long* S1, unsigned long long* S2,
                  float* mu, float* E, float* med) {
   hipStream_t s = ctx->stream;
+  const size_t lds_max = std::min<size_t>((size_t)ctx->max_lds,
+                                          ctx->split_lds_big ? kSumLdsMax : 64 * 1024);
+  const size_t hist_max = ctx->split_lds_big ? std::min<size_t>((size_t)ctx->max_lds, kHistLdsMax)
+                                             : std::min<size_t>((size_t)ctx->max_lds, 64 * 1024);
+  // points per block: with per-node LDS arrays, enough points to amortise their
+  // flush; with global atomics, small blocks so every CU has work
+  auto sum_ppb = [&](int64_t NN) -> int64_t {
+    return (size_t)NN * 12 <= lds_max ? std::max<int64_t>(8192, NN * 8) : 8192;
+  };
   DRPT_CHECK(fill_u32(ctx, node, 1u, (size_t)T * n));
+  // sums of the root (later levels get theirs from the fused partition)
+  DRPT_CHECK(zero4(ctx, cnt, (size_t)T, (uint32_t*)S1, 2 * (size_t)T));
+  {
+    const int64_t ppb = sum_ppb(1);
+    const dim3 g((unsigned)((n + ppb - 1) / ppb), (unsigned)T);
+    hipLaunchKernelGGL(k_sum1<true>, g, dim3(kThreads), 12, s, P, node, n, D, 0, colmax, bitlen, cnt,
+                       S1, ppb);
+    DRPT_HIP(hipGetLastError());
+  }
   for (int l = 0; l <= D - 2; ++l) {
     const int64_t NN = (int64_t)1 << l;
     const int64_t TN = (int64_t)T * NN;
-    DRPT_CHECK(fill_u32(ctx, cnt, 0u, TN));
-    DRPT_CHECK(fill_u32(ctx, (uint32_t*)S1, 0u, 2 * TN));
-    DRPT_CHECK(fill_u32(ctx, (uint32_t*)S2, 0u, 2 * TN));
-    DRPT_CHECK(fill_u32(ctx, H, 0u, 16 * TN));
+    DRPT_CHECK(zero4(ctx, (uint32_t*)S2, 2 * (size_t)TN, H, 16 * (size_t)TN));
     unsigned ngrid = (unsigned)((TN + 255) / 256);
-    const size_t lds_max = std::min<size_t>((size_t)ctx->max_lds,
-                                            ctx->split_lds_big ? kSumLdsMax : 64 * 1024);
-    bool lds_sum = (size_t)NN * 12 <= lds_max;
-    const size_t hist_max = ctx->split_lds_big ? std::min<size_t>((size_t)ctx->max_lds, kHistLdsMax)
-                                               : std::min<size_t>((size_t)ctx->max_lds, 64 * 1024);
-    bool lds_hist = (size_t)NN * 64 <= hist_max;
-    // points per block: with per-node LDS arrays, enough points to amortise their
-    // flush; with global atomics, small blocks so every CU has work
-    const int64_t ppb_sum = lds_sum ? std::max<int64_t>(8192, NN * 8) : 8192;
+    const bool lds_sum = (size_t)NN * 12 <= lds_max;
+    const bool lds_hist = (size_t)NN * 64 <= hist_max;
+    const bool lds_edges = lds_hist && (size_t)NN * (64 + 64 + 4) <= hist_max;
+    const int64_t ppb = sum_ppb(NN);
     const int64_t ppb_hist = lds_hist ? std::max<int64_t>(8192, NN * 32) : 8192;
-    const dim3 sgrid((unsigned)((n + ppb_sum - 1) / ppb_sum), (unsigned)T);
+    const dim3 pgrid((unsigned)((n + ppb - 1) / ppb), (unsigned)T);
     const dim3 hgrid((unsigned)((n + ppb_hist - 1) / ppb_hist), (unsigned)T);
-    const int64_t ppb = ppb_sum;
-    const dim3 pgrid = sgrid;
-    if (lds_sum)
-      hipLaunchKernelGGL(k_sum1<true>, pgrid, dim3(kThreads), NN * 12, s, P, node, n, D, l, colmax,
-                         bitlen, cnt, S1, ppb);
-    else
-      hipLaunchKernelGGL(k_sum1<false>, pgrid, dim3(kThreads), 0, s, P, node, n, D, l, colmax,
-                         bitlen, cnt, S1, ppb);
-    DRPT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_mu, dim3(ngrid), dim3(256), 0, s, T, l, D, mode, leaf_max, bitlen, colmax,
                        cnt, S1, mu, split);
     DRPT_HIP(hipGetLastError());
@@ -562,19 +708,38 @@ int split_levels(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t 
     hipLaunchKernelGGL(k_edges, dim3(ngrid), dim3(256), 0, s, T, l, D, bitlen, colmax, cnt, S2, mu,
                        split, E);
     DRPT_HIP(hipGetLastError());
-    if (lds_hist)
-      hipLaunchKernelGGL(k_hist<true>, hgrid, dim3(kThreads), NN * 64, s, P, node, n, D, l, E,
-                         split, H, ppb_hist);
+    if (lds_edges)
+      hipLaunchKernelGGL((k_hist<true, true>), hgrid, dim3(kThreads), NN * (64 + 64 + 4), s, P,
+                         node, n, D, l, E, split, H, ppb_hist);
+    else if (lds_hist)
+      hipLaunchKernelGGL((k_hist<true, false>), hgrid, dim3(kThreads), NN * 64, s, P, node, n, D, l,
+                         E, split, H, ppb_hist);
     else
-      hipLaunchKernelGGL(k_hist<false>, hgrid, dim3(kThreads), 0, s, P, node, n, D, l, E, split, H,
-                         ppb_hist);
+      hipLaunchKernelGGL((k_hist<false, false>), hgrid, dim3(kThreads), 0, s, P, node, n, D, l, E,
+                         split, H, ppb_hist);
     DRPT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_median, dim3(ngrid), dim3(256), 0, s, T, l, cnt, H, E, split, med);
     DRPT_HIP(hipGetLastError());
-    int64_t pbx = std::min<int64_t>((n + kThreads * 4 - 1) / (kThreads * 4), 4096);
-    hipLaunchKernelGGL(k_part, dim3((unsigned)pbx, (unsigned)T), dim3(kThreads), 0, s, P, node, n,
-                       D, l, med, split);
-    DRPT_HIP(hipGetLastError());
+    if (l < D - 2) {
+      // partition fused with the children's sums (their counters cleared
+      // first: the medians above were the last reads of this level's counts)
+      const int64_t NC = 2 * NN;
+      DRPT_CHECK(zero4(ctx, cnt, (size_t)T * NC, (uint32_t*)S1, 2 * (size_t)T * NC));
+      const int64_t ppc = sum_ppb(NC);
+      const dim3 cgrid((unsigned)((n + ppc - 1) / ppc), (unsigned)T);
+      if ((size_t)NC * 12 <= lds_max)
+        hipLaunchKernelGGL(k_part_sum1<true>, cgrid, dim3(kThreads), NC * 12, s, P, node, n, D, l,
+                           med, split, colmax, bitlen, cnt, S1, ppc);
+      else
+        hipLaunchKernelGGL(k_part_sum1<false>, cgrid, dim3(kThreads), 0, s, P, node, n, D, l, med,
+                           split, colmax, bitlen, cnt, S1, ppc);
+      DRPT_HIP(hipGetLastError());
+    } else {
+      int64_t pbx = std::min<int64_t>((n + kThreads * 4 - 1) / (kThreads * 4), 4096);
+      hipLaunchKernelGGL(k_part, dim3((unsigned)pbx, (unsigned)T), dim3(kThreads), 0, s, P, node,
+                         n, D, l, med, split);
+      DRPT_HIP(hipGetLastError());
+    }
   }
   return DRPT_OK;
 }
