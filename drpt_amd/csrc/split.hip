@@ -12,9 +12,10 @@
 //
 // Per level, all trees in one launch each:
 //   k_sum1   count + sum of quantised values per node (root level only)
-//   k_mu     split decision + mean                          (per node)
+//   k_mu     split decision + mean; clears S2 and H           (per node)
 //   k_sum2   sum of quantised fl(v-mu)^2 per node            (per point)
-//   k_edges  variance, sigma, edge table                     (per node)
+//   k_edges  variance, sigma, edge table; clears the next
+//            level's counts and sums                         (per node)
 //   k_hist   16-slot histogram (wave-ballot when a wave shares a node; the
 //            tree's edge tables staged in LDS where they fit)
 //   k_median cumulative search + interpolation               (per node)
@@ -41,13 +42,17 @@ namespace drpt {
 
 namespace {
 
-constexpr int kThreads = 256;
-// per-node LDS arrays of the LDS variants (bytes): sums 12 (sum1: count +
-// int64 sum; sum2: 8), histograms 64 (16 slots), plus for the histogram the
-// edge table (64) and split flag (4) when they fit as well; used while they
-// fit kSumLdsMax / kHistLdsMax and the device's per-workgroup LDS
-constexpr size_t kSumLdsMax = 96 * 1024;
-constexpr size_t kHistLdsMax = 136 * 1024;
+// Threads per block of the per-point kernels: 16 waves share one copy of a
+// block's LDS arrays, so a CU that holds a single block (large arrays) still
+// has 16 waves of loads in flight
+constexpr int kThreads = 1024;
+// per-node LDS arrays of the LDS variants (bytes): k_sum2 16 (int64 sum, mean,
+// split flag), k_part_sum1 32 (two children's count + int64 sum, median,
+// split flag), k_hist 68 (16 slots + pad) plus the 14 edges (60 with pad) and
+// split flag (4) when they fit as well; used while they fit kSumLdsMax /
+// kHistLdsMax and the device's per-workgroup LDS
+constexpr size_t kSumLdsMax = 128 * 1024;
+constexpr size_t kHistLdsMax = 144 * 1024;
 
 __device__ __forceinline__ void col_exponents(uint32_t cm, int bitlen, int& kq1, int& kq2) {
   int E = (int)((cm >> 23) & 0xffu);
@@ -61,15 +66,42 @@ __device__ __forceinline__ long long quant(float v, int kq) {
   return (long long)rint(ldexp((double)v, -kq));
 }
 
+// v + (v of the lane that DPP control kCtrl selects), in the rows of
+// kRowMask (v + 0 in the others); the 64 bits move as two 32-bit halves
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ long long dpp_add_i64(long long v) {
+  const int lo = (int)(uint32_t)(unsigned long long)v;
+  const int hi = (int)(uint32_t)((unsigned long long)v >> 32);
+  const uint32_t plo = (uint32_t)__builtin_amdgcn_update_dpp(0, lo, kCtrl, kRowMask, 0xF, false);
+  const uint32_t phi = (uint32_t)__builtin_amdgcn_update_dpp(0, hi, kCtrl, kRowMask, 0xF, false);
+  return (long long)((unsigned long long)v + (((unsigned long long)phi << 32) | plo));
+}
+
+// Sum over the wave's 64 lanes (modulo 2^64), the same value in every lane:
+// quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror and row_mirror leave
+// each row of 16 lanes with its row's sum; row_bcast:15 (rows 1, 3) and
+// row_bcast:31 (rows 2, 3) carry the sums forward, so lane 63 holds the total.
 __device__ __forceinline__ long long wave_sum_i64(long long v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
+  v = dpp_add_i64<0xB1, 0xF>(v);
+  v = dpp_add_i64<0x4E, 0xF>(v);
+  v = dpp_add_i64<0x141, 0xF>(v);
+  v = dpp_add_i64<0x140, 0xF>(v);
+  v = dpp_add_i64<0x142, 0xA>(v);
+  v = dpp_add_i64<0x143, 0xC>(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(unsigned long long)v, 63);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((unsigned long long)v >> 32), 63);
+  return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
 // Points per thread whose node and value words are loaded before any is used
 // (independent loads in flight; sums and counts do not depend on the order)
 constexpr int kUnroll = 4;
+
+// Split flags: split[node] = 0 for a node that does not split, else
+// kSplitBit | count (the count k_mu read, for k_edges and k_median, so the
+// count array is free for the next level once k_mu has run).
+constexpr uint32_t kSplitBit = 0x80000000u;
 
 __device__ __forceinline__ void load_points(const uint32_t* __restrict__ nt,
                                             const float* __restrict__ Pc, int64_t base,
@@ -89,88 +121,100 @@ __device__ __forceinline__ void block_range(int64_t n, int64_t ppb, int64_t& b0,
   b1 = min(n, b0 + ppb);
 }
 
-template <bool kLds>
-__global__ void __launch_bounds__(kThreads)
-k_sum1(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n, int D, int l,
-       const uint32_t* __restrict__ colmax, int bitlen, uint32_t* __restrict__ cnt,
-       unsigned long long* __restrict__ S1, int64_t ppb) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int t = blockIdx.y;
-  const int NN = 1 << l;
-  unsigned long long* lS = reinterpret_cast<unsigned long long*>(smem);
-  uint32_t* lc = reinterpret_cast<uint32_t*>(lS + (kLds ? NN : 0));
-  if (kLds) {
-    for (int k = threadIdx.x; k < NN; k += kThreads) { lS[k] = 0; lc[k] = 0; }
-    __syncthreads();
+// Accumulation of per-node int64 sums (and counts when kCnt) into S / C
+// (LDS or global arrays). A batch whose active lanes all hold one node adds
+// into per-lane registers while the wave stays on that node, with one wave
+// reduction and one atomic per run of such batches; a batch with several
+// nodes adds per lane. Every lane of the wave calls add() and flush().
+template <bool kCnt>
+struct NodeAcc {
+  unsigned long long* S;
+  uint32_t* C;
+  int cur;        // node of the pending run (the same in every lane), -1: none
+  long long acc;  // this lane's part of the run's sum
+  uint32_t c;     // points of the run
+  __device__ __forceinline__ NodeAcc(unsigned long long* S_, uint32_t* C_)
+      : S(S_), C(C_), cur(-1), acc(0), c(0) {}
+  __device__ __forceinline__ void flush() {
+    if (cur >= 0) {
+      const long long s = wave_sum_i64(acc);
+      if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&S[cur], (unsigned long long)s);
+        if (kCnt) atomicAdd(&C[cur], c);
+      }
+      cur = -1;
+      acc = 0;
+      c = 0;
+    }
   }
-  const int64_t col = (int64_t)t * D + l;
+  __device__ __forceinline__ void add(bool act, int nd, long long q) {
+    const unsigned long long am = __ballot(act);
+    if (am == 0) return;
+    const int leader = __ffsll((long long)am) - 1;
+    const int ndl = __builtin_amdgcn_readlane(nd, leader);
+    if (__ballot(act && nd == ndl) == am) {
+      if (ndl != cur) {
+        flush();
+        cur = ndl;
+      }
+      acc += q;  // 0 in inactive lanes
+      c += (uint32_t)__popcll(am);
+    } else if (act) {
+      atomicAdd(&S[nd], (unsigned long long)q);
+      if (kCnt) atomicAdd(&C[nd], 1u);
+    }
+  }
+};
+
+// Count and sum of quantised values of the root (level 0): every point of a
+// tree is in node 1, so each wave adds one run.
+__global__ void __launch_bounds__(kThreads)
+k_sum1(const float* __restrict__ P, int64_t n, int D, const uint32_t* __restrict__ colmax,
+       int bitlen, uint32_t* __restrict__ cnt, unsigned long long* __restrict__ S1, int64_t ppb) {
+  const int t = blockIdx.y;
+  const int64_t col = (int64_t)t * D;
   int kq1, kq2;
   col_exponents(colmax[col], bitlen, kq1, kq2);
   const float* Pc = P + col * n;
-  const uint32_t* nt = node + (int64_t)t * n;
-  uint32_t* gc = cnt + (int64_t)t * NN;
-  unsigned long long* gS = S1 + (int64_t)t * NN;
+  __shared__ unsigned long long bS;
+  __shared__ uint32_t bC;
+  if (threadIdx.x == 0) {
+    bS = 0;
+    bC = 0;
+  }
+  __syncthreads();
+  NodeAcc<true> acc(&bS, &bC);
   int64_t b0, b1;
   block_range(n, ppb, b0, b1);
-  const int lane = threadIdx.x & 63;
   for (int64_t base = b0; base < b1; base += kThreads * kUnroll) {
-    uint32_t hu[kUnroll];
     float pu[kUnroll];
-    load_points(nt, Pc, base, b1, hu, pu);
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const int64_t i = base + u * kThreads + threadIdx.x;
-      bool act = false;
-      int nd = 0;
-      long long q = 0;
-      if (i < b1) {
-        uint32_t h = hu[u];
-        act = (h >> l) == 1u;
-        nd = (int)(h - (uint32_t)NN);
-        if (act) q = quant(pu[u], kq1);
-      }
-      unsigned long long am = __ballot(act);
-      if (am == 0) continue;
-      int leader = __ffsll((long long)am) - 1;
-      int ndl = __shfl(nd, leader);
-      unsigned long long same = __ballot(act && nd == ndl);
-      if (same == am) {
-        long long s = wave_sum_i64(q);
-        if (lane == leader) {
-          if (kLds) {
-            atomicAdd(&lc[ndl], (uint32_t)__popcll(am));
-            atomicAdd(&lS[ndl], (unsigned long long)s);
-          } else {
-            atomicAdd(&gc[ndl], (uint32_t)__popcll(am));
-            atomicAdd(&gS[ndl], (unsigned long long)s);
-          }
-        }
-      } else if (act) {
-        if (kLds) {
-          atomicAdd(&lc[nd], 1u);
-          atomicAdd(&lS[nd], (unsigned long long)q);
-        } else {
-          atomicAdd(&gc[nd], 1u);
-          atomicAdd(&gS[nd], (unsigned long long)q);
-        }
-      }
+      pu[u] = i < b1 ? Pc[i] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int64_t i = base + u * kThreads + threadIdx.x;
+      acc.add(i < b1, 0, i < b1 ? quant(pu[u], kq1) : 0);
     }
   }
-  if (kLds) {
-    __syncthreads();
-    for (int k = threadIdx.x; k < NN; k += kThreads) {
-      if (lc[k]) {
-        atomicAdd(&gc[k], lc[k]);
-        atomicAdd(&gS[k], lS[k]);
-      }
-    }
+  acc.flush();
+  __syncthreads();
+  if (threadIdx.x == 0 && bC) {
+    atomicAdd(&cnt[t], bC);
+    atomicAdd(&S1[t], bS);
   }
 }
 
+// Split decision and mean per node (drpt_global.cpp:242-260 with the
+// canonical sums); clears this level's S2 and histogram entries for k_sum2
+// and k_hist.
 __global__ void k_mu(int T, int l, int D, int mode, int leaf_max, int bitlen,
                      const uint32_t* __restrict__ colmax, const uint32_t* __restrict__ cnt,
                      const unsigned long long* __restrict__ S1, float* __restrict__ mu,
-                     uint32_t* __restrict__ split) {
+                     uint32_t* __restrict__ split, unsigned long long* __restrict__ S2,
+                     uint32_t* __restrict__ H) {
   const int NN = 1 << l;
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)T * NN) return;
@@ -178,11 +222,21 @@ __global__ void k_mu(int T, int l, int D, int mode, int leaf_max, int bitlen,
   int kq1, kq2;
   col_exponents(colmax[(int64_t)t * D + l], bitlen, kq1, kq2);
   uint32_t c = cnt[idx];
-  split[idx] = (mode == DRPT_TREE_FIXED) ? 1u : (c > (uint32_t)leaf_max ? 1u : 0u);
+  const bool sp = (mode == DRPT_TREE_FIXED) || c > (uint32_t)leaf_max;
+  split[idx] = sp ? (kSplitBit | c) : 0u;
   double s = ldexp((double)(long long)S1[idx], kq1);
   mu[idx] = (float)(s / (double)c);
+  S2[idx] = 0ull;
+  uint4* h = reinterpret_cast<uint4*>(H + idx * 16);
+  const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+  h[0] = z;
+  h[1] = z;
+  h[2] = z;
+  h[3] = z;
 }
 
+// Sum of quantised fl(v - mu)^2 per split node. kLds: the tree's sums, means
+// and split flags in LDS ([NN] int64 + [NN] float + [NN] u32).
 template <bool kLds>
 __global__ void __launch_bounds__(kThreads)
 k_sum2(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n, int D, int l,
@@ -191,22 +245,30 @@ k_sum2(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n
   extern __shared__ __align__(16) unsigned char smem[];
   const int t = blockIdx.y;
   const int NN = 1 << l;
+  const float* mg = mu + (int64_t)t * NN;
+  const uint32_t* sg = split + (int64_t)t * NN;
+  unsigned long long* gS = S2 + (int64_t)t * NN;
   unsigned long long* lS = reinterpret_cast<unsigned long long*>(smem);
+  float* lm = reinterpret_cast<float*>(lS + (kLds ? NN : 0));
+  uint32_t* ls = reinterpret_cast<uint32_t*>(lm + (kLds ? NN : 0));
   if (kLds) {
-    for (int k = threadIdx.x; k < NN; k += kThreads) lS[k] = 0;
+    for (int k = threadIdx.x; k < NN; k += kThreads) {
+      lS[k] = 0;
+      lm[k] = mg[k];
+      ls[k] = sg[k];
+    }
     __syncthreads();
   }
+  const float* mt = kLds ? lm : mg;
+  const uint32_t* st = kLds ? ls : sg;
   const int64_t col = (int64_t)t * D + l;
   int kq1, kq2;
   col_exponents(colmax[col], bitlen, kq1, kq2);
   const float* Pc = P + col * n;
   const uint32_t* nt = node + (int64_t)t * n;
-  const float* mt = mu + (int64_t)t * NN;
-  const uint32_t* st = split + (int64_t)t * NN;
-  unsigned long long* gS = S2 + (int64_t)t * NN;
+  NodeAcc<false> acc(kLds ? lS : gS, nullptr);
   int64_t b0, b1;
   block_range(n, ppb, b0, b1);
-  const int lane = threadIdx.x & 63;
   for (int64_t base = b0; base < b1; base += kThreads * kUnroll) {
     uint32_t hu[kUnroll];
     float pu[kUnroll];
@@ -228,23 +290,10 @@ k_sum2(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n
           q = quant(w2, kq2);
         }
       }
-      unsigned long long am = __ballot(act);
-      if (am == 0) continue;
-      int leader = __ffsll((long long)am) - 1;
-      int ndl = __shfl(nd, leader);
-      unsigned long long same = __ballot(act && nd == ndl);
-      if (same == am) {
-        long long s = wave_sum_i64(q);
-        if (lane == leader) {
-          if (kLds) atomicAdd(&lS[ndl], (unsigned long long)s);
-          else atomicAdd(&gS[ndl], (unsigned long long)s);
-        }
-      } else if (act) {
-        if (kLds) atomicAdd(&lS[nd], (unsigned long long)q);
-        else atomicAdd(&gS[nd], (unsigned long long)q);
-      }
+      acc.add(act, nd, q);
     }
   }
+  acc.flush();
   if (kLds) {
     __syncthreads();
     for (int k = threadIdx.x; k < NN; k += kThreads)
@@ -252,19 +301,30 @@ k_sum2(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n
   }
 }
 
-// Edge table: math_operations.cpp:250-288 (factor 1, no_of_bins 7).
+// Edge table: math_operations.cpp:250-288 (factor 1, no_of_bins 7). With
+// zero_next, also clears the next level's count and sum entries of this
+// node's two children (k_mu of this level was the last reader of both).
 __global__ void k_edges(int T, int l, int D, int bitlen, const uint32_t* __restrict__ colmax,
-                        const uint32_t* __restrict__ cnt, const unsigned long long* __restrict__ S2,
+                        const unsigned long long* __restrict__ S2,
                         const float* __restrict__ mu, const uint32_t* __restrict__ split,
-                        float* __restrict__ E) {
+                        float* __restrict__ E, int zero_next, uint32_t* __restrict__ cnt,
+                        unsigned long long* __restrict__ S1) {
   const int NN = 1 << l;
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)T * NN) return;
-  if (!split[idx]) return;
+  if (zero_next) {
+    // children of (t, nd) at level l + 1: t * 2NN + 2nd + {0, 1} = 2 idx + {0, 1}
+    cnt[2 * idx] = 0u;
+    cnt[2 * idx + 1] = 0u;
+    S1[2 * idx] = 0ull;
+    S1[2 * idx + 1] = 0ull;
+  }
+  const uint32_t sp = split[idx];
+  if (!sp) return;
   int t = (int)(idx / NN);
   int kq1, kq2;
   col_exponents(colmax[(int64_t)t * D + l], bitlen, kq1, kq2);
-  long long c = (long long)cnt[idx];
+  long long c = (long long)(sp & ~kSplitBit);
   double s2 = ldexp((double)(long long)S2[idx], kq2);
   float var = (float)(s2 / (double)(c - 1));
   float m = mu[idx];
@@ -313,26 +373,43 @@ __device__ __forceinline__ int slot_of(float v, const float* e) {
   return c;
 }
 
+// Slot from the 14 edges alone (e[14] = 0 is a constant): the same count c,
+// by binary search over the non-decreasing e[0..13] (4 dependent reads
+// instead of 14; all-NaN edge tables still give 0).
+__device__ __forceinline__ int slot_bsearch(float v, const float* e) {
+  int c = e[7] < v ? 8 : 0;
+  c += e[c + 3] < v ? 4 : 0;
+  c += e[c + 1] < v ? 2 : 0;
+  if (c < 14) c += e[c] < v ? 1 : 0;
+  if (c == 14) c = v <= 0.0f ? 14 : 0;
+  return c;
+}
+
+// LDS row strides (words) of the histogram and edge tables: odd, so lanes at
+// random nodes spread over the 32 banks
+constexpr int kHS = 17;
+constexpr int kES = 15;
+
 template <bool kLds, bool kEL>
 __global__ void __launch_bounds__(kThreads)
 k_hist(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n, int D, int l,
        const float* __restrict__ E, const uint32_t* __restrict__ split, uint32_t* __restrict__ H,
        int64_t ppb) {
-  // kLds: histograms [NN][16] in LDS; kEL: also the tree's edge tables
-  // [NN][16] and split flags [NN] (staged once per block, read per point)
+  // kLds: histograms [NN][kHS] in LDS; kEL: also the tree's edges [NN][kES]
+  // and split flags [NN] (staged once per block, read per point)
   extern __shared__ __align__(16) unsigned char smem[];
   const int t = blockIdx.y;
   const int NN = 1 << l;
   uint32_t* lH = reinterpret_cast<uint32_t*>(smem);
-  float* lE = reinterpret_cast<float*>(lH + (kLds ? NN * 16 : 0));
-  uint32_t* lS = reinterpret_cast<uint32_t*>(lE + (kEL ? NN * 16 : 0));
+  float* lE = reinterpret_cast<float*>(lH + (kLds ? NN * kHS : 0));
+  uint32_t* lS = reinterpret_cast<uint32_t*>(lE + (kEL ? NN * kES : 0));
   const float* Eg = E + (int64_t)t * NN * 16;
   const uint32_t* Sg = split + (int64_t)t * NN;
   if (kLds) {
-    for (int k = threadIdx.x; k < NN * 16; k += kThreads) lH[k] = 0;
+    for (int k = threadIdx.x; k < NN * kHS; k += kThreads) lH[k] = 0;
     if (kEL) {
-      for (int k = threadIdx.x; k < NN * 4; k += kThreads)
-        reinterpret_cast<float4*>(lE)[k] = reinterpret_cast<const float4*>(Eg)[k];
+      for (int k = threadIdx.x; k < NN * 16; k += kThreads)
+        if ((k & 15) < 14) lE[(k >> 4) * kES + (k & 15)] = Eg[k];
       for (int k = threadIdx.x; k < NN; k += kThreads) lS[k] = Sg[k];
     }
     __syncthreads();
@@ -340,7 +417,6 @@ k_hist(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n
   const int64_t col = (int64_t)t * D + l;
   const float* Pc = P + col * n;
   const uint32_t* nt = node + (int64_t)t * n;
-  const float* Et = kEL ? lE : Eg;
   const uint32_t* st = kEL ? lS : Sg;
   uint32_t* gH = H + (int64_t)t * NN * 16;
   int64_t b0, b1;
@@ -361,20 +437,24 @@ k_hist(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n
         nd = (int)(h - (uint32_t)NN);
         if (act) act = st[nd] != 0u;
         if (act) {
-          float e[16];
-          const float4* src = reinterpret_cast<const float4*>(Et + (int64_t)nd * 16);
-          float4 a0 = src[0], a1 = src[1], a2 = src[2], a3 = src[3];
-          e[0] = a0.x; e[1] = a0.y; e[2] = a0.z; e[3] = a0.w;
-          e[4] = a1.x; e[5] = a1.y; e[6] = a1.z; e[7] = a1.w;
-          e[8] = a2.x; e[9] = a2.y; e[10] = a2.z; e[11] = a2.w;
-          e[12] = a3.x; e[13] = a3.y; e[14] = a3.z; e[15] = a3.w;
-          b = slot_of(pu[u], e);
+          if (kEL) {
+            b = slot_bsearch(pu[u], lE + nd * kES);
+          } else {
+            float e[16];
+            const float4* src = reinterpret_cast<const float4*>(Eg + (int64_t)nd * 16);
+            float4 a0 = src[0], a1 = src[1], a2 = src[2], a3 = src[3];
+            e[0] = a0.x; e[1] = a0.y; e[2] = a0.z; e[3] = a0.w;
+            e[4] = a1.x; e[5] = a1.y; e[6] = a1.z; e[7] = a1.w;
+            e[8] = a2.x; e[9] = a2.y; e[10] = a2.z; e[11] = a2.w;
+            e[12] = a3.x; e[13] = a3.y; e[14] = a3.z; e[15] = a3.w;
+            b = slot_of(pu[u], e);
+          }
         }
       }
       unsigned long long am = __ballot(act);
       if (am == 0) continue;
       int leader = __ffsll((long long)am) - 1;
-      int ndl = __shfl(nd, leader);
+      int ndl = __builtin_amdgcn_readlane(nd, leader);
       unsigned long long same = __ballot(act && nd == ndl);
       if (same == am) {
         // wave-ballot histogram: lane k < 16 adds the population of slot k
@@ -385,31 +465,34 @@ k_hist(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n
           mine = (lane == k) ? c : mine;
         }
         if (lane < 16 && mine) {
-          if (kLds) atomicAdd(&lH[ndl * 16 + lane], mine);
+          if (kLds) atomicAdd(&lH[ndl * kHS + lane], mine);
           else atomicAdd(&gH[(int64_t)ndl * 16 + lane], mine);
         }
       } else if (act) {
-        if (kLds) atomicAdd(&lH[nd * 16 + b], 1u);
+        if (kLds) atomicAdd(&lH[nd * kHS + b], 1u);
         else atomicAdd(&gH[(int64_t)nd * 16 + b], 1u);
       }
     }
   }
   if (kLds) {
     __syncthreads();
-    for (int k = threadIdx.x; k < NN * 16; k += kThreads)
-      if (lH[k]) atomicAdd(&gH[k], lH[k]);
+    for (int k = threadIdx.x; k < NN * 16; k += kThreads) {
+      const uint32_t v = lH[(k >> 4) * kHS + (k & 15)];
+      if (v) atomicAdd(&gH[k], v);
+    }
   }
 }
 
 // Cumulative search + interpolation: math_operations.cpp:333-377, E[-1] = 0.
-__global__ void k_median(int T, int l, const uint32_t* __restrict__ cnt,
-                         const uint32_t* __restrict__ H, const float* __restrict__ E,
-                         const uint32_t* __restrict__ split, float* __restrict__ med) {
+__global__ void k_median(int T, int l, const uint32_t* __restrict__ H,
+                         const float* __restrict__ E, const uint32_t* __restrict__ split,
+                         float* __restrict__ med) {
   const int NN = 1 << l;
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)T * NN) return;
-  if (!split[idx]) return;
-  long long c = (long long)cnt[idx];
+  const uint32_t sp = split[idx];
+  if (!sp) return;
+  long long c = (long long)(sp & ~kSplitBit);
   const uint32_t* h = H + idx * 16;
   const float* e = E + idx * 16;
   float val = (float)((50 * c) / 100);
@@ -465,7 +548,8 @@ k_part(const float* __restrict__ P, uint32_t* __restrict__ node, int64_t n, int 
 // node moves to its child (v <= median -> 2h, else 2h + 1) and, at once, adds
 // its count and quantised level-(l+1) value to the child's sums (points of
 // unsplit nodes are leaves and take no further part). kLds: the child sums
-// in LDS ([2^(l+1)] int64 sums + counts).
+// ([2NN] int64 sums + [2NN] counts) and the tree's medians and split flags
+// ([NN] float + [NN] u32) in LDS.
 template <bool kLds>
 __global__ void __launch_bounds__(kThreads)
 k_part_sum1(const float* __restrict__ P, uint32_t* __restrict__ node, int64_t n, int D, int l,
@@ -475,33 +559,44 @@ k_part_sum1(const float* __restrict__ P, uint32_t* __restrict__ node, int64_t n,
   extern __shared__ __align__(16) unsigned char smem[];
   const int t = blockIdx.y;
   const int NN = 1 << l, NC = 2 << l;
+  const float* mg = med + (int64_t)t * NN;
+  const uint32_t* sg = split + (int64_t)t * NN;
+  uint32_t* gc = cnt + (int64_t)t * NC;
+  unsigned long long* gS = S1 + (int64_t)t * NC;
   unsigned long long* lS = reinterpret_cast<unsigned long long*>(smem);
   uint32_t* lc = reinterpret_cast<uint32_t*>(lS + (kLds ? NC : 0));
+  float* lm = reinterpret_cast<float*>(lc + (kLds ? NC : 0));
+  uint32_t* ls = reinterpret_cast<uint32_t*>(lm + (kLds ? NN : 0));
   if (kLds) {
-    for (int k = threadIdx.x; k < NC; k += kThreads) { lS[k] = 0; lc[k] = 0; }
+    for (int k = threadIdx.x; k < NC; k += kThreads) {
+      lS[k] = 0;
+      lc[k] = 0;
+    }
+    for (int k = threadIdx.x; k < NN; k += kThreads) {
+      lm[k] = mg[k];
+      ls[k] = sg[k];
+    }
     __syncthreads();
   }
+  const float* mt = kLds ? lm : mg;
+  const uint32_t* st = kLds ? ls : sg;
   const float* Pc = P + ((int64_t)t * D + l) * n;
   const float* Pn = Pc + n;  // level l + 1 of the same tree
   int kq1, kq2;
   col_exponents(colmax[(int64_t)t * D + l + 1], bitlen, kq1, kq2);
   uint32_t* nt = node + (int64_t)t * n;
-  const float* mt = med + (int64_t)t * NN;
-  const uint32_t* st = split + (int64_t)t * NN;
-  uint32_t* gc = cnt + (int64_t)t * NC;
-  unsigned long long* gS = S1 + (int64_t)t * NC;
+  NodeAcc<true> acc(kLds ? lS : gS, kLds ? lc : gc);
   int64_t b0, b1;
   block_range(n, ppb, b0, b1);
-  const int lane = threadIdx.x & 63;
   for (int64_t base = b0; base < b1; base += kThreads * kUnroll) {
     uint32_t hu[kUnroll];
-    float pu[kUnroll];
-    load_points(nt, Pc, base, b1, hu, pu);
-    float pn[kUnroll];
+    float pu[kUnroll], pn[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const int64_t i = base + u * kThreads + threadIdx.x;
-      pn[u] = (i < b1 && (hu[u] >> l) == 1u) ? Pn[i] : 0.0f;
+      hu[u] = i < b1 ? nt[i] : 0u;
+      pu[u] = i < b1 ? Pc[i] : 0.0f;
+      pn[u] = i < b1 ? Pn[i] : 0.0f;
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
@@ -522,33 +617,10 @@ k_part_sum1(const float* __restrict__ P, uint32_t* __restrict__ node, int64_t n,
           }
         }
       }
-      unsigned long long am = __ballot(act);
-      if (am == 0) continue;
-      int leader = __ffsll((long long)am) - 1;
-      int ndl = __shfl(nd, leader);
-      unsigned long long same = __ballot(act && nd == ndl);
-      if (same == am) {
-        long long s = wave_sum_i64(q);
-        if (lane == leader) {
-          if (kLds) {
-            atomicAdd(&lc[ndl], (uint32_t)__popcll(am));
-            atomicAdd(&lS[ndl], (unsigned long long)s);
-          } else {
-            atomicAdd(&gc[ndl], (uint32_t)__popcll(am));
-            atomicAdd(&gS[ndl], (unsigned long long)s);
-          }
-        }
-      } else if (act) {
-        if (kLds) {
-          atomicAdd(&lc[nd], 1u);
-          atomicAdd(&lS[nd], (unsigned long long)q);
-        } else {
-          atomicAdd(&gc[nd], 1u);
-          atomicAdd(&gS[nd], (unsigned long long)q);
-        }
-      }
+      acc.add(act, nd, q);
     }
   }
+  acc.flush();
   if (kLds) {
     __syncthreads();
     for (int k = threadIdx.x; k < NC; k += kThreads) {
@@ -602,9 +674,7 @@ static void raise_split_lds(drpt_ctx* ctx) {
   if (!ctx->split_lds_big && ctx->max_lds > 64 * 1024) {
     const int bytes = (int)std::min<size_t>((size_t)ctx->max_lds, kSumLdsMax);
     const int hbytes = (int)std::min<size_t>((size_t)ctx->max_lds, kHistLdsMax);
-    if (hipFuncSetAttribute((const void*)k_sum1<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            bytes) == hipSuccess &&
-        hipFuncSetAttribute((const void*)k_sum2<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    if (hipFuncSetAttribute((const void*)k_sum2<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             bytes) == hipSuccess &&
         hipFuncSetAttribute((const void*)k_part_sum1<true>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess &&
@@ -668,71 +738,67 @@ int split_levels(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t 
                                           ctx->split_lds_big ? kSumLdsMax : 64 * 1024);
   const size_t hist_max = ctx->split_lds_big ? std::min<size_t>((size_t)ctx->max_lds, kHistLdsMax)
                                              : std::min<size_t>((size_t)ctx->max_lds, 64 * 1024);
-  // points per block: with per-node LDS arrays, enough points to amortise their
-  // flush; with global atomics, small blocks so every CU has work
-  auto sum_ppb = [&](int64_t NN) -> int64_t {
-    return (size_t)NN * 12 <= lds_max ? std::max<int64_t>(8192, NN * 8) : 8192;
-  };
+  // points per block: with per-node LDS arrays, enough points to amortise
+  // staging and flushing them; with global atomics, small blocks so every CU
+  // has work
+  auto grid_of = [&](int64_t ppb) { return dim3((unsigned)((n + ppb - 1) / ppb), (unsigned)T); };
   DRPT_CHECK(fill_u32(ctx, node, 1u, (size_t)T * n));
-  // sums of the root (later levels get theirs from the fused partition)
+  // the root's count and sum (later levels get theirs from the fused partition)
   DRPT_CHECK(zero4(ctx, cnt, (size_t)T, (uint32_t*)S1, 2 * (size_t)T));
-  {
-    const int64_t ppb = sum_ppb(1);
-    const dim3 g((unsigned)((n + ppb - 1) / ppb), (unsigned)T);
-    hipLaunchKernelGGL(k_sum1<true>, g, dim3(kThreads), 12, s, P, node, n, D, 0, colmax, bitlen, cnt,
-                       S1, ppb);
-    DRPT_HIP(hipGetLastError());
-  }
+  hipLaunchKernelGGL(k_sum1, grid_of(16384), dim3(kThreads), 0, s, P, n, D, colmax, bitlen, cnt, S1,
+                     (int64_t)16384);
+  DRPT_HIP(hipGetLastError());
   for (int l = 0; l <= D - 2; ++l) {
     const int64_t NN = (int64_t)1 << l;
     const int64_t TN = (int64_t)T * NN;
-    DRPT_CHECK(zero4(ctx, (uint32_t*)S2, 2 * (size_t)TN, H, 16 * (size_t)TN));
-    unsigned ngrid = (unsigned)((TN + 255) / 256);
-    const bool lds_sum = (size_t)NN * 12 <= lds_max;
-    const bool lds_hist = (size_t)NN * 64 <= hist_max;
-    const bool lds_edges = lds_hist && (size_t)NN * (64 + 64 + 4) <= hist_max;
-    const int64_t ppb = sum_ppb(NN);
-    const int64_t ppb_hist = lds_hist ? std::max<int64_t>(8192, NN * 32) : 8192;
-    const dim3 pgrid((unsigned)((n + ppb - 1) / ppb), (unsigned)T);
-    const dim3 hgrid((unsigned)((n + ppb_hist - 1) / ppb_hist), (unsigned)T);
+    const unsigned ngrid = (unsigned)((TN + 255) / 256);
+    const bool last = l == D - 2;
+    // k_mu also clears this level's S2 and histograms
     hipLaunchKernelGGL(k_mu, dim3(ngrid), dim3(256), 0, s, T, l, D, mode, leaf_max, bitlen, colmax,
-                       cnt, S1, mu, split);
+                       cnt, S1, mu, split, S2, H);
     DRPT_HIP(hipGetLastError());
-    if (lds_sum)
-      hipLaunchKernelGGL(k_sum2<true>, pgrid, dim3(kThreads), NN * 8, s, P, node, n, D, l, colmax,
-                         bitlen, mu, split, S2, ppb);
-    else
-      hipLaunchKernelGGL(k_sum2<false>, pgrid, dim3(kThreads), 0, s, P, node, n, D, l, colmax,
-                         bitlen, mu, split, S2, ppb);
+    const size_t sum2_lds = (size_t)NN * 16;
+    if (sum2_lds <= lds_max) {
+      const int64_t ppb = std::max<int64_t>(8192, NN * 8);
+      hipLaunchKernelGGL(k_sum2<true>, grid_of(ppb), dim3(kThreads), sum2_lds, s, P, node, n, D, l,
+                         colmax, bitlen, mu, split, S2, ppb);
+    } else {
+      hipLaunchKernelGGL(k_sum2<false>, grid_of(8192), dim3(kThreads), 0, s, P, node, n, D, l,
+                         colmax, bitlen, mu, split, S2, (int64_t)8192);
+    }
     DRPT_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_edges, dim3(ngrid), dim3(256), 0, s, T, l, D, bitlen, colmax, cnt, S2, mu,
-                       split, E);
+    // k_edges also clears the next level's counts and sums
+    hipLaunchKernelGGL(k_edges, dim3(ngrid), dim3(256), 0, s, T, l, D, bitlen, colmax, S2, mu,
+                       split, E, last ? 0 : 1, cnt, S1);
     DRPT_HIP(hipGetLastError());
+    const size_t hist_lds = (size_t)NN * kHS * 4, edge_lds = (size_t)NN * (kES * 4 + 4);
+    const bool lds_hist = hist_lds <= hist_max;
+    const bool lds_edges = lds_hist && hist_lds + edge_lds <= hist_max;
+    const int64_t ppb_hist = lds_hist ? std::max<int64_t>(8192, NN * 32) : 8192;
     if (lds_edges)
-      hipLaunchKernelGGL((k_hist<true, true>), hgrid, dim3(kThreads), NN * (64 + 64 + 4), s, P,
-                         node, n, D, l, E, split, H, ppb_hist);
+      hipLaunchKernelGGL((k_hist<true, true>), grid_of(ppb_hist), dim3(kThreads),
+                         hist_lds + edge_lds, s, P, node, n, D, l, E, split, H, ppb_hist);
     else if (lds_hist)
-      hipLaunchKernelGGL((k_hist<true, false>), hgrid, dim3(kThreads), NN * 64, s, P, node, n, D, l,
-                         E, split, H, ppb_hist);
+      hipLaunchKernelGGL((k_hist<true, false>), grid_of(ppb_hist), dim3(kThreads), hist_lds, s, P,
+                         node, n, D, l, E, split, H, ppb_hist);
     else
-      hipLaunchKernelGGL((k_hist<false, false>), hgrid, dim3(kThreads), 0, s, P, node, n, D, l, E,
-                         split, H, ppb_hist);
+      hipLaunchKernelGGL((k_hist<false, false>), grid_of(ppb_hist), dim3(kThreads), 0, s, P, node,
+                         n, D, l, E, split, H, ppb_hist);
     DRPT_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_median, dim3(ngrid), dim3(256), 0, s, T, l, cnt, H, E, split, med);
+    hipLaunchKernelGGL(k_median, dim3(ngrid), dim3(256), 0, s, T, l, H, E, split, med);
     DRPT_HIP(hipGetLastError());
-    if (l < D - 2) {
-      // partition fused with the children's sums (their counters cleared
-      // first: the medians above were the last reads of this level's counts)
+    if (!last) {
+      // partition fused with the children's counts and sums
       const int64_t NC = 2 * NN;
-      DRPT_CHECK(zero4(ctx, cnt, (size_t)T * NC, (uint32_t*)S1, 2 * (size_t)T * NC));
-      const int64_t ppc = sum_ppb(NC);
-      const dim3 cgrid((unsigned)((n + ppc - 1) / ppc), (unsigned)T);
-      if ((size_t)NC * 12 <= lds_max)
-        hipLaunchKernelGGL(k_part_sum1<true>, cgrid, dim3(kThreads), NC * 12, s, P, node, n, D, l,
-                           med, split, colmax, bitlen, cnt, S1, ppc);
-      else
-        hipLaunchKernelGGL(k_part_sum1<false>, cgrid, dim3(kThreads), 0, s, P, node, n, D, l, med,
-                           split, colmax, bitlen, cnt, S1, ppc);
+      const size_t part_lds = (size_t)NC * 12 + (size_t)NN * 8;
+      if (part_lds <= lds_max) {
+        const int64_t ppc = std::max<int64_t>(8192, NC * 8);
+        hipLaunchKernelGGL(k_part_sum1<true>, grid_of(ppc), dim3(kThreads), part_lds, s, P, node, n,
+                           D, l, med, split, colmax, bitlen, cnt, S1, ppc);
+      } else {
+        hipLaunchKernelGGL(k_part_sum1<false>, grid_of(8192), dim3(kThreads), 0, s, P, node, n, D,
+                           l, med, split, colmax, bitlen, cnt, S1, (int64_t)8192);
+      }
       DRPT_HIP(hipGetLastError());
     } else {
       int64_t pbx = std::min<int64_t>((n + kThreads * 4 - 1) / (kThreads * 4), 4096);
