@@ -326,6 +326,7 @@ struct Grouping {
   LeafItem* items = nullptr;
   int64_t* range = nullptr;  // device {0, first, first, all, 0, all}
   const int64_t* base = nullptr;  // device [T + 1]: first work item of each tree
+  int64_t item_cap = 0;           // capacity of items
   unsigned long long* stats = nullptr;
   double host_gram = -1.0;
   int64_t host_leaves = 0, host_max = 0;
@@ -549,6 +550,7 @@ static int group_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
 
   G->perm = perm_final;
   G->items = items;
+  G->item_cap = item_cap;
   G->range = range;
   G->base = base;
   G->stats = stats;
@@ -584,18 +586,17 @@ static int prep_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X, const i
   const bool fp8 = p->precision == DRPT_FP8;
   const std::string tag = fp8 ? "lowp8." : "lowp.";  // both may be live in one build
   uint8_t* xq;
-  float *nb, *er, *lnbT, *lerrT;
+  float2* nbe;
+  float *lnbT, *lerrT;
   int8_t *pexp = nullptr, *lexpT = nullptr;
   DRPT_CHECK(ws(ctx, (tag + "x").c_str(), (size_t)n * p->d * (fp8 ? 1 : 2), &xq));
-  DRPT_CHECK(ws(ctx, (tag + "nrm2").c_str(), n, &nb));
-  DRPT_CHECK(ws(ctx, (tag + "err").c_str(), n, &er));
+  DRPT_CHECK(ws(ctx, (tag + "nbe").c_str(), n, &nbe));
   if (fp8) DRPT_CHECK(ws(ctx, (tag + "pexp").c_str(), n, &pexp));
-  DRPT_CHECK(launch_prep_lowp(ctx, X, p->dtype, n, p->d, p->precision, xq, nb, er, pexp));
+  DRPT_CHECK(launch_prep_lowp(ctx, X, p->dtype, n, p->d, p->precision, xq, nbe, pexp));
   DRPT_CHECK(ws(ctx, (tag + "lnbT").c_str(), (size_t)p->ntrees * n, &lnbT));
   DRPT_CHECK(ws(ctx, (tag + "lerrT").c_str(), (size_t)p->ntrees * n, &lerrT));
   if (fp8) DRPT_CHECK(ws(ctx, (tag + "lexpT").c_str(), (size_t)p->ntrees * n, &lexpT));
-  DRPT_CHECK(launch_leaf_gather(ctx, perm, (int64_t)p->ntrees * n, nb, er, pexp, lnbT, lerrT,
-                                lexpT));
+  DRPT_CHECK(launch_leaf_gather(ctx, perm, (int64_t)p->ntrees * n, nbe, pexp, lnbT, lerrT, lexpT));
   a->X = X;
   a->Xq = xq;
   a->precision = p->precision;
@@ -728,6 +729,7 @@ struct LowpRun {
   int32_t *ki = nullptr, *pool = nullptr, *pcnt = nullptr, *oflag = nullptr, *olist = nullptr,
           *novf = nullptr;
   float *kd = nullptr, *boundT = nullptr;
+  float* bnd = nullptr;  // [n] K-th distance of the current known lists (the bound)
 };
 
 static std::string plan_key(const drpt_params* p) {
@@ -795,6 +797,7 @@ static int lowp_setup(drpt_ctx* ctx, const drpt_params* p, const void* X,
   DRPT_CHECK(ws(ctx, "knn.known_idx", (size_t)n * r->K, &r->ki));
   DRPT_CHECK(ws(ctx, "knn.known_dist", (size_t)n * r->K, &r->kd));
   DRPT_CHECK(ws(ctx, "knn.boundT", (size_t)r->T * n, &r->boundT));
+  DRPT_CHECK(ws(ctx, "knn.bound", (size_t)n, &r->bnd));
   DRPT_CHECK(ws(ctx, "knn.pool", (size_t)n * r->pool_cap, &r->pool));
   DRPT_CHECK(ws(ctx, "knn.pool_cnt", n, &r->pcnt));
   DRPT_CHECK(ws(ctx, "knn.ovf_flag", n, &r->oflag));
@@ -833,18 +836,27 @@ static int lowp_pass(drpt_ctx* ctx, LowpRun* r, int mode, const ItemPlan& plan,
                      (mode == kModeCand && pr->precision == DRPT_BF16 && (kp == 12 || kp == 16 || kp == 24));
   const int pl_rows = pl_ok ? pool_leaf_rows(ctx, pr->d) : 0;
   auto run = [&](int lo, int hi, int ring) -> int {
+    KnnArgs b = a;
     if (ring == 2 && pl_rows > 0 && lo <= pl_rows) {
-      KnnArgs b = a;
       b.size_lo = lo;
       b.size_hi = std::min(hi, pl_rows);
       DRPT_CHECK(launch_pool_leaf(ctx, mode, kp, b));
       if (hi <= pl_rows) return DRPT_OK;
       lo = pl_rows + 1;
+      // the few larger leaves: their items gathered first, so that the
+      // streaming kernel does not walk (and skip) every other item
+      LeafItem* big;
+      int64_t* big_range;
+      DRPT_CHECK(ws(ctx, "knn.items_big", (size_t)r->G.item_cap, &big));
+      DRPT_CHECK(ws(ctx, "knn.range_big", 2, &big_range));
+      DRPT_CHECK(launch_select_items(ctx, a.items, a.range, lo, hi, r->G.item_cap, big, big_range));
+      b.items = big;
+      b.range = big_range;
     }
-    a.size_lo = lo;
-    a.size_hi = hi;
-    a.ring = ring == 0 ? 0 : 1;
-    return launch_lowp_knn(ctx, pr, mode, a);
+    b.size_lo = lo;
+    b.size_hi = hi;
+    b.ring = ring == 0 ? 0 : 1;
+    return launch_lowp_knn(ctx, pr, mode, b);
   };
   bool uniform = true;
   for (int b = 1; b < kSizeBuckets; ++b) uniform = uniform && plan.ring[b] == plan.ring[0];
@@ -878,6 +890,7 @@ static int lowp_known(drpt_ctx* ctx, LowpRun* r, const void* X, const ItemPlan& 
   m.rerank = std::min(rr, r->first * r->KP);
   m.out_idx = r->ki;
   m.out_dist = r->kd;
+  m.out_bound = r->bnd;
   m.order = r->G.perm;  // tree 0's leaf order
   DRPT_CHECK(launch_row_merge(ctx, r->pp.dtype, m));
   DRPT_HIP(hipEventRecord(ctx->evp[2], ctx->stream));
@@ -931,8 +944,8 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
     const int t0 = g * gsz, t1 = std::min(T, t0 + gsz);
     const bool last = g + 1 == ngroups;
     DRPT_HIP(hipEventRecord(ctx->ev_pass[4 * g + 3], s));
-    DRPT_CHECK(launch_leaf_gather_val(ctx, G.perm + (int64_t)t0 * n, (int64_t)(t1 - t0) * n, kdn, K,
-                                      K - 1, r.boundT + (int64_t)t0 * n));
+    DRPT_CHECK(launch_leaf_gather_val(ctx, G.perm + (int64_t)t0 * n, (int64_t)(t1 - t0) * n, r.bnd,
+                                      1, 0, r.boundT + (int64_t)t0 * n));
     if (g > 0) DRPT_CHECK(fill_u32(ctx, (uint32_t*)r.pcnt, 0u, n));
     DRPT_HIP(hipEventRecord(ctx->ev_pass[4 * g], s));
     r.a.known = kin;
@@ -953,6 +966,7 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
     f.skip_flag = r.oflag;
     f.out_idx = last ? out_idx : (kin == r.ki ? ki2 : r.ki);
     f.out_dist = last ? out_dist : (kdn == r.kd ? kd2 : r.kd);
+    f.out_bound = last ? nullptr : r.bnd;
     f.order = G.perm;
     f.heavy = heavy;
     f.nheavy = nheavy;
@@ -1106,8 +1120,7 @@ int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* p
   LowpRun r;
   DRPT_CHECK(lowp_setup(ctx, p, X, heap, best, &r));
   DRPT_CHECK(lowp_known(ctx, &r, X, best));
-  DRPT_CHECK(launch_leaf_gather_val(ctx, r.G.perm, (int64_t)r.T * r.n, r.kd, r.K, r.K - 1,
-                                    r.boundT));
+  DRPT_CHECK(launch_leaf_gather_val(ctx, r.G.perm, (int64_t)r.T * r.n, r.bnd, 1, 0, r.boundT));
   r.a.known = r.ki;
   const int rb = p->d * 2;  // the pooled passes run in BF16
   const bool ring_ok = rb % 16 == 0 && rb <= 256;

@@ -352,6 +352,34 @@ k_emit_items(const uint32_t* __restrict__ hcount, const int64_t* __restrict__ ho
   }
 }
 
+// The items among [range[0], range[1]) whose leaf size lies in [lo, hi],
+// copied to out (one reservation per wave, so their order is unspecified);
+// out_range = {0, count} with its count word zeroed by the caller.
+__global__ void __launch_bounds__(256)
+k_select_items(const LeafItem* __restrict__ items, const int64_t* __restrict__ range, int lo,
+               int hi, LeafItem* __restrict__ out, int64_t* __restrict__ out_range) {
+  const int64_t b = range[0], e = range[1];
+  const int lane = threadIdx.x & 63;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i0 = b + (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); i0 < e;
+       i0 += stride) {
+    const int64_t i = i0 + lane;
+    LeafItem it{};
+    bool keep = false;
+    if (i < e) {
+      it = items[i];
+      keep = it.size >= lo && it.size <= hi;
+    }
+    const uint64_t bal = __ballot(keep);
+    if (bal == 0) continue;
+    unsigned base = 0;
+    if (lane == 0)
+      base = atomicAdd(reinterpret_cast<unsigned*>(&out_range[1]), (unsigned)__popcll(bal));
+    base = (unsigned)__shfl((int)base, 0);
+    if (keep) out[base + __popcll(bal & ((1ull << lane) - 1))] = it;
+  }
+}
+
 unsigned grid_for(int64_t n) {
   int64_t b = (n + 1023) / 1024;
   if (b > 2048) b = 2048;
@@ -496,6 +524,21 @@ int launch_tree_range(drpt_ctx* ctx, const int64_t* base, int t0, int t1, int64_
     return DRPT_ERR_ARG;
   }
   hipLaunchKernelGGL(k_tree_range, dim3(1), dim3(64), 0, ctx->stream, base, t0, t1, range);
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
+}
+
+int launch_select_items(drpt_ctx* ctx, const LeafItem* items, const int64_t* range, int lo, int hi,
+                        int64_t max_items, LeafItem* out, int64_t* out_range) {
+  if (max_items >= ((int64_t)1 << 32)) {
+    set_error("select_items: too many work items");
+    return DRPT_ERR_ARG;
+  }
+  DRPT_CHECK(fill_u32(ctx, reinterpret_cast<uint32_t*>(out_range), 0u, 4));
+  const unsigned grid = (unsigned)std::max<int64_t>(
+      1, std::min<int64_t>((max_items + 255) / 256, (int64_t)ctx->num_cus * 4));
+  hipLaunchKernelGGL(k_select_items, dim3(grid), dim3(256), 0, ctx->stream, items, range, lo, hi,
+                     out, out_range);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }

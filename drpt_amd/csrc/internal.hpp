@@ -166,6 +166,10 @@ int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, con
                         int64_t* range, LeafItem* items, unsigned long long* stats);
 // range[0] = base[t0], range[1] = base[t1]: the work items of trees [t0, t1)
 int launch_tree_range(drpt_ctx* ctx, const int64_t* base, int t0, int t1, int64_t* range);
+// out[0, out_range[1]) = the items of [range[0], range[1]) with leaf sizes in
+// [lo, hi] (out_range[0] = 0); max_items bounds the input range and out's size
+int launch_select_items(drpt_ctx* ctx, const LeafItem* items, const int64_t* range, int lo, int hi,
+                        int64_t max_items, LeafItem* out, int64_t* out_range);
 int launch_sort_segments(drpt_ctx* ctx, int32_t* perm, int64_t n, const int64_t* seg_start,
                          const int32_t* seg_size, const int32_t* seg_tree, int nseg, int max_seg);
 // sorts each segment of perm by (LP[seg_tree * ld + level][point], point):
@@ -178,14 +182,13 @@ int launch_apply_local(drpt_ctx* ctx, const int32_t* perm_in, int32_t* perm_out,
                        const int32_t* seg_size, int nseg, const int32_t* tables);
 
 // quantised rows (bf16: n x d halves; fp8: n x d bytes with a power-of-two
-// scale 2^pexp[i] per row), |x~|^2 and |x~ - x| of the dequantised rows
+// scale 2^pexp[i] per row), nbe[i] = (|x~|^2, |x~ - x|) of the dequantised rows
 int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, int precision,
-                     void* Xq, float* nrm2, float* err, int8_t* pexp);
-// leaf-ordered |x~|^2, |x~ - x| and (fp8) scale exponents: lnbT[i] =
-// nrm2[perm[i]], lerrT[i] = err[perm[i]], lexpT[i] = pexp[perm[i]] (pexp may be null)
-int launch_leaf_gather(drpt_ctx* ctx, const int32_t* perm, int64_t total, const float* nrm2,
-                       const float* err, const int8_t* pexp, float* lnbT, float* lerrT,
-                       int8_t* lexpT);
+                     void* Xq, float2* nbe, int8_t* pexp);
+// leaf-ordered |x~|^2, |x~ - x| and (fp8) scale exponents: (lnbT[i], lerrT[i])
+// = nbe[perm[i]], lexpT[i] = pexp[perm[i]] (pexp may be null)
+int launch_leaf_gather(drpt_ctx* ctx, const int32_t* perm, int64_t total, const float2* nbe,
+                       const int8_t* pexp, float* lnbT, float* lerrT, int8_t* lexpT);
 // FP32 exact per-tree lists for items[range[0] .. range[1]) (range in device memory)
 int launch_leaf_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
                     const LeafItem* items, const int64_t* range, int32_t* cand_idx,
@@ -261,6 +264,7 @@ struct RowMergeArgs {
   const int32_t* order;      // optional permutation of the rows (processing order)
   int32_t* out_idx;          // [n][K]
   float* out_dist;
+  float* out_bound;          // optional [n]: out_dist[row][K - 1] (the row's bound)
   // optional, one source with more than 256 slots: rows with at most 256
   // candidates are merged by a light pass (fewer registers per wave, more waves
   // in flight), the others listed in heavy[0 .. *nheavy) for the full pass

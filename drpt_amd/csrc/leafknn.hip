@@ -1279,14 +1279,15 @@ __global__ void __launch_bounds__(4 * lowp_rows<MODE>(), lowp_rows<MODE>() == 64
 
 // Leaf-ordered copies of |x~|^2 and |x~ - x| (position j of tree t's leaf order).
 __global__ void k_leaf_gather(const int32_t* __restrict__ perm, int64_t total,
-                              const float* __restrict__ nb, const float* __restrict__ err,
-                              const int8_t* __restrict__ pexp, float* __restrict__ lnbT,
-                              float* __restrict__ lerrT, int8_t* __restrict__ lexpT) {
+                              const float2* __restrict__ nbe, const int8_t* __restrict__ pexp,
+                              float* __restrict__ lnbT, float* __restrict__ lerrT,
+                              int8_t* __restrict__ lexpT) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int id = perm[i];
-    lnbT[i] = nb[id];
-    lerrT[i] = err[id];
+    const float2 v = nbe[id];  // one 8-byte gather per point
+    lnbT[i] = v.x;
+    lerrT[i] = v.y;
     if (pexp) lexpT[i] = pexp[id];
   }
 }
@@ -1441,6 +1442,8 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
         A.out_idx[row * K + lane] = A.known_idx[row * K + lane];
         A.out_dist[row * K + lane] = A.known_dist[row * K + lane];
       }
+      if (A.out_bound != nullptr && lane == K - 1)
+        A.out_bound[row] = A.known_idx != nullptr ? A.known_dist[row * K + lane] : INFINITY;
       continue;
     }
     // known list (sorted, -1 padded): lanes < K
@@ -1460,6 +1463,7 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
         A.out_idx[row * K + lane] = kid;
         A.out_dist[row * K + lane] = kd;
       }
+      if (A.out_bound != nullptr && lane == K - 1) A.out_bound[row] = kd;
       continue;
     }
     // known ids first, then candidates: the first insertion of an id wins
@@ -1600,6 +1604,8 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
         oi[lane] = v == ~0ull ? -1 : (int32_t)(uint32_t)v;
         od[lane] = v == ~0ull ? INFINITY : __uint_as_float((uint32_t)(v >> 32));
       }
+      if (A.out_bound != nullptr && lane == K - 1)
+        A.out_bound[row] = v == ~0ull ? INFINITY : __uint_as_float((uint32_t)(v >> 32));
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
       continue;
@@ -1616,15 +1622,18 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
     if (lane < K && nk != ~0ull && rn < K) {
       oi[rn] = (int32_t)(uint32_t)nk;
       od[rn] = __uint_as_float((uint32_t)(nk >> 32));
+      if (A.out_bound != nullptr && rn == K - 1) A.out_bound[row] = __uint_as_float((uint32_t)(nk >> 32));
     }
     if (lane < K && kk != ~0ull && rk < K) {
       oi[rk] = kid;
       od[rk] = kd;
+      if (A.out_bound != nullptr && rk == K - 1) A.out_bound[row] = kd;
     }
     const int nv = min(neval, K) + nknown;
     if (lane < K && lane >= nv) {
       oi[lane] = -1;
       od[lane] = INFINITY;
+      if (A.out_bound != nullptr && lane == K - 1) A.out_bound[row] = INFINITY;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1811,7 +1820,7 @@ k_knn_fallback(const XT* __restrict__ X, int d, const int32_t* __restrict__ perm
 // where d FP32 additions would loosen the filter bound.
 template <typename XT>
 __global__ void k_prep_bf16(const XT* __restrict__ X, int64_t n, int d, uint16_t* __restrict__ Xb,
-                            float* __restrict__ nb, float* __restrict__ err) {
+                            float2* __restrict__ nbe) {
   const int lane = threadIdx.x & 63;
   const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (row >= n) return;
@@ -1837,10 +1846,9 @@ __global__ void k_prep_bf16(const XT* __restrict__ X, int64_t n, int d, uint16_t
     e2 += __shfl_xor(e2, off);
   }
   if (lane == 0) {
-    nb[row] = wide ? (float)s64 : s;
     float ef = (float)sqrt(e2);
     if (e2 > 0.0) ef = ef * (1.0f + 1.0e-6f) + 1.0e-30f;
-    err[row] = ef;
+    nbe[row] = make_float2(wide ? (float)s64 : s, ef);
   }
 }
 
@@ -1851,8 +1859,7 @@ __global__ void k_prep_bf16(const XT* __restrict__ X, int64_t n, int d, uint16_t
 // |x~|^2 and |x~ - x| (FP64, rounded up) are reduced as for bf16.
 template <typename XT>
 __global__ void k_prep_fp8(const XT* __restrict__ X, int64_t n, int d, uint8_t* __restrict__ Xq,
-                           float* __restrict__ nb, float* __restrict__ err,
-                           int8_t* __restrict__ pexp) {
+                           float2* __restrict__ nbe, int8_t* __restrict__ pexp) {
   const int lane = threadIdx.x & 63;
   const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (row >= n) return;
@@ -1901,10 +1908,9 @@ __global__ void k_prep_fp8(const XT* __restrict__ X, int64_t n, int d, uint8_t* 
     e2 += __shfl_xor(e2, off);
   }
   if (lane == 0) {
-    nb[row] = wide ? (float)s64 : s;
     float ef = (float)sqrt(e2);
     if (e2 > 0.0) ef = ef * (1.0f + 1.0e-6f) + 1.0e-30f;
-    err[row] = ef;
+    nbe[row] = make_float2(wide ? (float)s64 : s, ef);
     pexp[row] = (int8_t)e;
   }
 }
@@ -1982,7 +1988,7 @@ int dispatch_exact(drpt_ctx* ctx, const drpt_params* p, const XT* X, const int32
 }  // namespace
 
 int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, int precision,
-                     void* Xq, float* nrm2, float* err, int8_t* pexp) {
+                     void* Xq, float2* nbe, int8_t* pexp) {
   if (precision != DRPT_BF16 && precision != DRPT_FP8) {
     set_error("prep_lowp: precision not supported");
     return DRPT_ERR_UNSUPPORTED;
@@ -1992,26 +1998,25 @@ int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, 
   if (precision == DRPT_FP8) {
     if (dtype == DRPT_U8)
       hipLaunchKernelGGL(k_prep_fp8<uint8_t>, dim3(blocks), dim3(256), 0, ctx->stream,
-                         (const uint8_t*)X, n, d, (uint8_t*)Xq, nrm2, err, pexp);
+                         (const uint8_t*)X, n, d, (uint8_t*)Xq, nbe, pexp);
     else
       hipLaunchKernelGGL(k_prep_fp8<float>, dim3(blocks), dim3(256), 0, ctx->stream,
-                         (const float*)X, n, d, (uint8_t*)Xq, nrm2, err, pexp);
+                         (const float*)X, n, d, (uint8_t*)Xq, nbe, pexp);
   } else if (dtype == DRPT_U8)
     hipLaunchKernelGGL(k_prep_bf16<uint8_t>, dim3(blocks), dim3(256), 0, ctx->stream,
-                       (const uint8_t*)X, n, d, (uint16_t*)Xq, nrm2, err);
+                       (const uint8_t*)X, n, d, (uint16_t*)Xq, nbe);
   else
     hipLaunchKernelGGL(k_prep_bf16<float>, dim3(blocks), dim3(256), 0, ctx->stream, (const float*)X,
-                       n, d, (uint16_t*)Xq, nrm2, err);
+                       n, d, (uint16_t*)Xq, nbe);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }
 
-int launch_leaf_gather(drpt_ctx* ctx, const int32_t* perm, int64_t total, const float* nrm2,
-                       const float* err, const int8_t* pexp, float* lnbT, float* lerrT,
-                       int8_t* lexpT) {
+int launch_leaf_gather(drpt_ctx* ctx, const int32_t* perm, int64_t total, const float2* nbe,
+                       const int8_t* pexp, float* lnbT, float* lerrT, int8_t* lexpT) {
   const int64_t blocks = std::min<int64_t>((total + 255) / 256, (int64_t)ctx->num_cus * 16);
   hipLaunchKernelGGL(k_leaf_gather, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0,
-                     ctx->stream, perm, total, nrm2, err, pexp, lnbT, lerrT, lexpT);
+                     ctx->stream, perm, total, nbe, pexp, lnbT, lerrT, lexpT);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }

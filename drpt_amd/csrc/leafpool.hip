@@ -403,13 +403,13 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
 
 }  // namespace
 
-// Largest leaf (rows, a multiple of 64) the LDS-resident pool pass stages at
-// dimension d on this device, or 0 when it does not apply (BF16 rows of 32 to
-// 256 bytes in 32-byte multiples).
+// Largest leaf (rows, a multiple of 32: the tiles read whole 32-row groups)
+// the LDS-resident pool pass stages at dimension d on this device, or 0 when
+// it does not apply (BF16 rows of 32 to 256 bytes in 32-byte multiples).
 int pool_leaf_rows(drpt_ctx* ctx, int d) {
   if (d < 16 || d > 16 * kPlSteps || (d & 15) != 0) return 0;
   const int64_t room = (int64_t)ctx->max_lds - kPlMisc * 4;
-  const int64_t rows = (room / (kPlPitch + 12)) & ~(int64_t)63;
+  const int64_t rows = (room / (kPlPitch + 12)) & ~(int64_t)31;
   return rows >= 64 ? (int)std::min<int64_t>(rows, 2 * kPlThreads) : 0;
 }
 

@@ -64,7 +64,8 @@ def test_c1_mnist_like_full_reference_local_stage(ctx, orc):
 def test_c2_int_mixture_tuned_plans(ctx, orc):
     """100000 x 128 integer mixture, T = 16, nn = 10, leaf <= 512, BF16
     k' = 24: the plan the benchmark's autotuner chose at 1M points (whole
-    leaves, LDS ring, one 16-tree group, 4 first trees, re-rank 20), a
+    leaves resident in LDS, one 16-tree group, 4 first trees, re-rank 20),
+    the same with the streaming LDS tile ring, a
     row-block plan with four groups and a shallow re-rank, the autotuner's
     own plan for this input, and the FP32 build."""
     _log("[c2] 100000 x 128 int mixture, T=16, leaf<=512, bf16 k'=24")
@@ -74,6 +75,8 @@ def test_c2_int_mixture_tuned_plans(ctx, orc):
     p = drpt_amd.leafmax_params(n, 128, 16, 10, leaf_max=512, precision="bf16", kprime=24, seed=1)
     ri, rd = _oracle(orc, X, p)
     plans = [
+        {"rows_per_item": [WHOLE] * 4, "ring": [2] * 4, "tree_group": 16, "first_trees": 4,
+         "rerank": 20},
         {"rows_per_item": [WHOLE] * 4, "ring": [1] * 4, "tree_group": 16, "first_trees": 4,
          "rerank": 20},
         {"rows_per_item": [WHOLE, 64, 64, 128], "ring": [1, 0, 1, 1], "tree_group": 4,
@@ -91,6 +94,24 @@ def test_c2_int_mixture_tuned_plans(ctx, orc):
     p32 = drpt_amd.leafmax_params(n, 128, 16, 10, leaf_max=512, precision="fp32", seed=1)
     oi, od, _ = ctx.build_graph(p32, Xd)
     _same(oi, od, ri, rd, "c2 fp32")
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
+def test_leaf_resident_and_streaming_leaves(ctx, orc, prec):
+    """60000 x 128 integer mixture, T = 8, leaves up to 1024: leaves the
+    LDS-resident passes stage (up to 608 rows at d = 128) and larger ones,
+    whose work items are gathered for the streaming kernel, in one build."""
+    _log(f"[leaf-split] 60000 x 128 int mixture, T=8, leaf<=1024, {prec}")
+    n = 60_000
+    X = datagen.int_mixture(n)
+    p = drpt_amd.leafmax_params(n, 128, 8, 10, leaf_max=1024, precision=prec, kprime=24, seed=3)
+    ctx.set_item_plan(p, {"rows_per_item": [WHOLE] * 4, "ring": [2] * 4, "tree_group": 8,
+                          "first_trees": 4, "rerank": 20})
+    oi, od, tm = ctx.build_graph(p, _dev(X))
+    _log(f"  largest leaf {tm.max_leaf}, leaves {tm.n_leaves}")
+    assert tm.max_leaf > 608
+    ri, rd = _oracle(orc, X, p)
+    _same(oi, od, ri, rd, f"leaf-split {prec}")
 
 
 def test_c3_gauss960_leaves_to_2048(ctx, orc):
