@@ -72,7 +72,7 @@ class Phases(ctypes.Structure):
         ("pool_merge_ms", ctypes.c_double), ("bound_ms", ctypes.c_double),
         ("exact_ms", ctypes.c_double), ("tree_groups", ctypes.c_int64),
         ("first_trees", ctypes.c_int64), ("pool_cap", ctypes.c_int64), ("rerank", ctypes.c_int64),
-        ("reserved", ctypes.c_int64 * 4),
+        ("pool_entries", ctypes.c_int64), ("reserved", ctypes.c_int64 * 3),
     ]
 
     def as_dict(self):

@@ -782,7 +782,8 @@ static int lowp_setup(drpt_ctx* ctx, const drpt_params* p, const void* X,
                                                                       : kFirstTrees),
                                   256 / r->KP));
   // the LDS-resident pool pass leaves known ids to the row merge: larger pools
-  r->pool_cap = pool_leaf_rows(ctx, p->d) > 0 ? std::max(512, pool_cap_of(r->K)) : pool_cap_of(r->K);
+  r->pool_cap = pool_leaf_rows(ctx, p->d, kModePool) > 0 ? std::max(512, pool_cap_of(r->K))
+                                                         : pool_cap_of(r->K);
   const int64_t n = r->n;
   DRPT_CHECK(group_impl(ctx, p, X, leaf_heap, plan, r->first, &r->G));
   DRPT_HIP(hipEventRecord(ctx->evp[0], ctx->stream));
@@ -834,7 +835,7 @@ static int lowp_pass(drpt_ctx* ctx, LowpRun* r, int mode, const ItemPlan& plan,
   const int kp = lowp_slots(pr);
   const bool pl_ok = mode == kModePool ||
                      (mode == kModeCand && pr->precision == DRPT_BF16 && (kp == 12 || kp == 16 || kp == 24));
-  const int pl_rows = pl_ok ? pool_leaf_rows(ctx, pr->d) : 0;
+  const int pl_rows = pl_ok ? pool_leaf_rows(ctx, pr->d, mode) : 0;
   auto run = [&](int lo, int hi, int ring) -> int {
     KnnArgs b = a;
     if (ring == 2 && pl_rows > 0 && lo <= pl_rows) {
@@ -938,6 +939,9 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   }
   for (int g = 0; g < ngroups; ++g)
     DRPT_CHECK(launch_tree_range(ctx, G.base, g * gsz, std::min(T, (g + 1) * gsz), granges + 2 * g));
+  unsigned long long* pool_sum;  // entries the pooled passes appended (a statistic)
+  DRPT_CHECK(ws(ctx, "knn.pool_sum", 1, &pool_sum));
+  DRPT_CHECK(fill_u32(ctx, (uint32_t*)pool_sum, 0u, 2));
   int32_t* kin = r.ki;  // known lists in, merged lists out (ping-pong)
   float* kdn = r.kd;
   for (int g = 0; g < ngroups; ++g) {
@@ -951,6 +955,7 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
     r.a.known = kin;
     DRPT_CHECK(lowp_pass(ctx, &r, kModePool, plan, granges + 2 * g));
     DRPT_HIP(hipEventRecord(ctx->ev_pass[4 * g + 1], s));
+    DRPT_CHECK(sum_u32(ctx, (const uint32_t*)r.pcnt, n, pool_sum));
     RowMergeArgs f{};
     f.X = X;
     f.n = n;
@@ -976,7 +981,9 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
     kdn = f.out_dist;
   }
   int32_t nov = 0;
+  unsigned long long npool = 0;
   DRPT_HIP(hipMemcpyAsync(&nov, r.novf, 4, hipMemcpyDeviceToHost, s));
+  DRPT_HIP(hipMemcpyAsync(&npool, pool_sum, 8, hipMemcpyDeviceToHost, s));
   DRPT_HIP(hipStreamSynchronize(s));
   if (nov > 0)
     DRPT_CHECK(launch_row_exact(ctx, &r.pp, X, G.perm, G.items, G.range + 4, r.olist, r.novf, nov,
@@ -999,6 +1006,7 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   ph.first_trees = r.first;
   ph.pool_cap = r.pool_cap;
   ph.rerank = plan.rerank > 0 ? std::max(K, std::min(plan.rerank, 2 * K)) : 2 * K;
+  ph.pool_entries = (int64_t)npool;
   if (timings) {
     timings->group_ms = ms_between(ctx->ev[6], ctx->ev[7]);
     timings->knn_ms = ms_between(ctx->ev[7], ctx->ev[5]);
@@ -1124,7 +1132,7 @@ int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* p
   r.a.known = r.ki;
   const int rb = p->d * 2;  // the pooled passes run in BF16
   const bool ring_ok = rb % 16 == 0 && rb <= 256;
-  const bool leaf_ok = pool_leaf_rows(ctx, p->d) > 0;  // whole leaves resident in LDS
+  const bool leaf_ok = pool_leaf_rows(ctx, p->d, kModePool) > 0;  // whole leaves resident in LDS
   hipEvent_t e0 = ctx->ev[10], e1 = ctx->ev[11];
   // candidates per bucket: rows per item x Gram pipeline; each timed twice
   // after a warm-up, the incumbent kept unless another is 3% faster

@@ -82,6 +82,8 @@ int host_local_depth(int64_t gmin, int nn, int offset);
 // ---- launchers (one per .hip unit) ------------------------------------------
 // util.hip: capture-safe fills (a kernel, not hipMemsetAsync)
 int fill_u32(drpt_ctx* ctx, uint32_t* p, uint32_t v, size_t count);
+// *acc += p[0] + ... + p[count - 1] (stream-ordered)
+int sum_u32(drpt_ctx* ctx, const uint32_t* p, size_t count, unsigned long long* acc);
 // Issues record() on the context's stream: eagerly when graphs are off, else
 // captured into a hipGraph the first time `key` is seen under `name` and
 // replayed from then on. record() may only launch kernels on ctx->stream (no
@@ -237,11 +239,11 @@ struct KnnArgs {
 int launch_lowp_knn(drpt_ctx* ctx, const drpt_params* p, int mode, KnnArgs a);
 int lowp_slots(const drpt_params* p);  // k' actually used (a multiple of 4)
 // leafpool.hip: the pool and candidate passes with each leaf resident in LDS
-// (work-item plan flag 2): largest leaf it stages at dimension d (0: not
-// applicable), and the launch (mode kModePool or kModeCand with kp = 4 TK
+// (work-item plan flag 2): largest leaf it stages at dimension d in a mode
+// (0: not applicable), and the launch (mode kModePool or kModeCand with kp = 4 TK
 // widened candidates, TK in {3, 4, 6}; BF16 rows) over the items of a.range
 // with sizes in [a.size_lo, a.size_hi]
-int pool_leaf_rows(drpt_ctx* ctx, int d);
+int pool_leaf_rows(drpt_ctx* ctx, int d, int mode);
 int launch_pool_leaf(drpt_ctx* ctx, int mode, int kp, const KnnArgs& a);
 
 struct RowMergeArgs {

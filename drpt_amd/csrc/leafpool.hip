@@ -15,7 +15,8 @@
 //   acc_ij = G_ij - |x~_j|^2 / 2 = (|x~_i|^2 - a_ij) / 2,
 // and a_ij <= A_cut(B_i) becomes acc_ij >= thr_i with the per-row threshold
 //   thr_i = round_down((|x~_i|^2 - A_cut(B_i) - eta_i) / 2):
-// one compare per element. The extra eta_i = (2d + 16) 2^-24 (n_i + nm)^2 (the
+// per element the sign bit of fl(acc_ij - thr_i), appended to the lane's mask
+// by one v_alignbit. The extra eta_i = (2d + 16) 2^-24 (n_i + nm)^2 (the
 // bound A_cut already holds once) covers the different rounding of this form
 // (the initialised accumulator, the stored norms and the subtraction, each
 // within (d + 4) 2^-24 (n_i + nm)^2), so every leafmate the streaming pass
@@ -45,7 +46,7 @@ constexpr int kPlPitch = 256;   // LDS bytes per leaf row (16 chunks of 16 bytes
 constexpr int kPlSteps = 8;     // k-steps of 16 elements: rows of at most 128 bf16
 constexpr int kPlList = 8;      // pool entries a lane holds per query row
 constexpr int kPlQueues = 8;    // ticket queues, one per XCD (blocks go round-robin)
-constexpr int kPlMisc = 32;     // ints of bookkeeping after the leaf arrays
+constexpr int kPlMisc = 40;     // ints of bookkeeping after the leaf arrays
 
 __device__ __forceinline__ uint4 row_chunk(const unsigned char* rows, int r, int c) {
   return *reinterpret_cast<const uint4*>(rows + r * kPlPitch + ((c ^ (r & 15)) << 4));
@@ -105,13 +106,50 @@ __device__ __forceinline__ void flush_pair(const KnnArgs& A, const int32_t* lid,
   }
 }
 
-// Filter bits of one lane's 16 tile elements (bit 4 g + u: column
-// 8 g + u of the lane's half) for threshold thr.
+// Filter of one lane's 16 tile elements against threshold thr: bit e (column
+// 8 g + u of the lane's half for e = 4 g + u) set when acc[e] >= thr, from the
+// sign of fl(acc[e] - thr) (one v_sub and one v_alignbit per element: the
+// alignbit shifts the mask left and appends the sign bit). thr is never +0, so
+// a zero difference is +0 and the sign test equals the comparison.
 __device__ __forceinline__ uint32_t pass_bits(const floatx16& acc, float thr) {
-  uint32_t pm = 0;
+  uint32_t fb = 0;
 #pragma unroll
-  for (int e = 15; e >= 0; --e) pm = pm + pm + (acc[e] >= thr ? 1u : 0u);
-  return pm;
+  for (int e = 15; e >= 0; --e)
+    fb = __builtin_amdgcn_alignbit(fb, __float_as_uint(acc[e] - thr), 31);
+  return ~fb & 0xFFFFu;
+}
+
+// LDS layout of k_pool_leaf for mmax leaf rows: the rows, then per row its id,
+// -|x~|^2 / 2 and threshold, then kPlMisc ints of bookkeeping.
+__host__ __device__ constexpr size_t pool_leaf_row_bytes(int) { return kPlPitch + 12; }
+
+// misc (ints): next-item slots [8 s .. 8 s + 6] (a LeafItem and a valid flag),
+// task counter [16], per-wave maxima (floats) [20 .. 35]
+constexpr int kMiscTask = 16, kMiscRed = 20;
+
+// tid 0 only: the next work item of this block's queue whose leaf size lies
+// in [size_lo, size_hi] (items of other launches are passed over), or none,
+// into slot s of misc
+__device__ __forceinline__ void fetch_item(const KnnArgs& A, unsigned* myq, int64_t begin,
+                                           int64_t end, int* misc, int s) {
+  for (;;) {
+    const int64_t item = begin + (int64_t)atomicAdd(myq, 1u);
+    if (item >= end) {
+      misc[8 * s + 6] = 0;
+      return;
+    }
+    static_assert(sizeof(LeafItem) == 24, "LeafItem is three 8-byte words");
+    const int2* src = reinterpret_cast<const int2*>(&A.items[item]);
+    const int2 w0 = src[0], w1 = src[1], w2 = src[2];  // (tree, size), start, (row0, rows)
+    if (w0.y >= A.size_lo && w0.y <= A.size_hi) {
+      int2* dst = reinterpret_cast<int2*>(&misc[8 * s]);
+      dst[0] = w0;
+      dst[1] = w1;
+      dst[2] = w2;
+      misc[8 * s + 6] = 1;
+      return;
+    }
+  }
 }
 
 // MODE kModePool: survivors of the certified filter to the rows' pools.
@@ -119,21 +157,27 @@ __device__ __forceinline__ uint32_t pass_bits(const floatx16& acc, float thr) {
 // approximate distances of each of its four column streams (two column ranges
 // x two lane halves; of every 4 adjacent columns of a stream the nearest one),
 // as keys (approx bits & ~0x7FF) << 32 | id.
+//
+// Per leaf: the next leaf's ids, norms, errors and bounds are loaded into
+// registers, and the item after it is fetched by thread 0, while the leaf's
+// tasks run; after them (one barrier) the next leaf is installed (ids and
+// maxima in LDS, its rows by global_load_lds_dwordx4, thresholds).
 template <int KS, int MODE, int TK>
 __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, int mmax) {
   static_assert(MODE == kModePool || MODE == kModeCand, "pool or candidate mode");
+  constexpr bool kPool = MODE == kModePool;
   extern __shared__ __align__(16) unsigned char smem[];
   unsigned char* rows = smem;
   int32_t* lid = reinterpret_cast<int32_t*>(smem + (size_t)mmax * kPlPitch);
   float* chalf = reinterpret_cast<float*>(lid + mmax);
   float* thrs = chalf + mmax;  // per leaf row: filter threshold (pool) or |x~|^2 (candidates)
-  int* misc = reinterpret_cast<int*>(thrs + mmax);   // tickets [0, 1], task counter [2]
-  float* red = reinterpret_cast<float*>(misc + 4);   // per wave: max |x~|^2 [w], max error [8 + w]
+  int* misc = reinterpret_cast<int*>(thrs + mmax);
+  float* red = reinterpret_cast<float*>(misc + kMiscRed);  // per wave: max |x~|^2 [w], max error [8 + w]
   const uint8_t* __restrict__ Xq = static_cast<const uint8_t*>(A.Xq);
   constexpr int d = 16 * KS;
   constexpr int rb = 2 * d;      // bytes per bf16 row
   constexpr int L = rb >> 4;     // 16-byte chunks per row
-  constexpr int kRowsPerThread = 2;  // leaf rows whose threshold a thread computes
+  constexpr int kRowsPerThread = 2;  // leaf rows whose metadata a thread loads
   const int64_t n = A.n;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int half = lane >> 5;
@@ -142,39 +186,42 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
   const int64_t begin = A.range[0] + qx * chunk;
   const int64_t end = min(A.range[1], begin + chunk);
   unsigned* myq = A.queue + qx;
-  if (tid == 0) misc[0] = (int)atomicAdd(myq, 1u);
-  __syncthreads();
-  int64_t item = begin + misc[0];
   // self bit of the diagonal tile: lane's query row (lane & 31) among its columns
   const int srel = (lane & 31) - 4 * half;
   const uint32_t selfbit =
       (srel >= 0 && (srel & 4) == 0) ? 1u << (((srel >> 3) << 2) | (srel & 3)) : 0u;
   const double u = 5.9604644775390625e-08;  // 2^-24
-  for (int parity = 1;; parity ^= 1) {
-    if (item >= end) break;
-    if (tid == 0) misc[parity] = (int)atomicAdd(myq, 1u);  // next item, fetched early
-    const LeafItem it = A.items[item];
-    const int m = it.size;
-    if (m < A.size_lo || m > A.size_hi) {  // another launch's leaves
-      __syncthreads();
-      item = begin + misc[parity];
-      continue;
-    }
+
+  // metadata of a leaf's rows j = tid + h * kPlThreads, loaded into registers
+  int rid[kRowsPerThread];
+  float rnb[kRowsPerThread], rer[kRowsPerThread], rbd[kRowsPerThread];
+  auto load_meta = [&](const LeafItem& it) {
     const int64_t lofs = (int64_t)it.tree * n + it.start;
-    const int32_t* pl = A.perm + lofs;
-    // ids, -|x~|^2 / 2, the leaf's largest norm and error; each thread keeps
-    // the norm, error and bound of its rows for their thresholds
-    float mn = 0.0f, me = 0.0f;
-    float rnb[kRowsPerThread], rer[kRowsPerThread], rbd[kRowsPerThread];
 #pragma unroll
     for (int h = 0; h < kRowsPerThread; ++h) {
       const int j = tid + h * kPlThreads;
+      rid[h] = 0;
       rnb[h] = rer[h] = rbd[h] = 0.0f;
-      if (j < m) {
+      if (j < it.size) {
+        rid[h] = A.perm[lofs + j];
         rnb[h] = A.lnbT[lofs + j];
         rer[h] = A.lerrT[lofs + j];
-        if constexpr (MODE == kModePool) rbd[h] = A.boundT[lofs + j];
-        lid[j] = pl[j];
+        if constexpr (kPool) rbd[h] = A.boundT[lofs + j];
+      }
+    }
+  };
+  // ids, -|x~|^2 / 2 and the leaf's maxima; then its rows (slot s = 16 r +
+  // physical chunk, logical chunk = physical ^ (r & 15); one instruction per
+  // wave covers 64 slots, 4 rows) and, while they arrive, the thresholds
+  // thr = round_down((|x~|^2 - A_cut(B) - eta) / 2)
+  auto install = [&](const LeafItem& it) {
+    const int m = it.size;
+    float mn = 0.0f, me = 0.0f;
+#pragma unroll
+    for (int h = 0; h < kRowsPerThread; ++h) {
+      const int j = tid + h * kPlThreads;
+      if (j < m) {
+        lid[j] = rid[h];
         chalf[j] = -0.5f * rnb[h];
         mn = fmaxf(mn, rnb[h]);
         me = fmaxf(me, rer[h]);
@@ -189,10 +236,8 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
       red[wave] = mn;
       red[8 + wave] = me;
     }
-    if (tid == 0) misc[2] = 0;
+    if (tid == 0) misc[kMiscTask] = 0;
     __syncthreads();  // ids and maxima visible
-    // the leaf's rows: slot s = 16 r + physical chunk, logical chunk = physical ^ (r & 15);
-    // one instruction per wave covers 64 slots (4 rows)
     for (int blk = wave; blk * 4 < m; blk += kPlWaves) {
       const int slot = blk * 64 + lane;
       const int r = slot >> 4, c = (slot & 15) ^ (r & 15);
@@ -200,8 +245,6 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
           (uint32_t)(uintptr_t)(lds_ptr_t)(rows + blk * 64 * 16));
       if (r < m && c < L) lds_dma16(Xq + (int64_t)lid[r] * rb + c * 16, dst);
     }
-    // thresholds while the rows arrive:
-    //   thr = round_down((|x~|^2 - A_cut(B) - eta) / 2)
     float nmax = red[0], emax = red[8];
 #pragma unroll
     for (int w = 1; w < kPlWaves; ++w) {
@@ -212,16 +255,40 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
 #pragma unroll
     for (int h = 0; h < kRowsPerThread; ++h) {
       const int j = tid + h * kPlThreads;
-      if (MODE == kModeCand && j < m) thrs[j] = rnb[h];
-      if (MODE == kModePool && j < m) {
+      if (!kPool && j < m) thrs[j] = rnb[h];
+      if (kPool && j < m) {
         const float acut = acut_of(rbd[h], rnb[h], rer[h], nm, emax, d);
         const double ni = sqrt((double)rnb[h] * (1.0 + (d + 4.0) * u));
         const double eta = (2.0 * d + 16.0) * u * (ni + nm) * (ni + nm);
-        thrs[j] = __double2float_rd(0.5 * ((double)rnb[h] - (double)acut - eta));
+        const float t = __double2float_rd(0.5 * ((double)rnb[h] - (double)acut - eta));
+        thrs[j] = t == 0.0f ? -0.0f : t;  // a zero difference from it is then +0
       }
     }
     wait_vmcnt<0>();
     __syncthreads();  // rows and thresholds visible to every wave
+  };
+
+  // items fetched one leaf ahead: the next item's metadata loads are in
+  // flight during the current leaf's tasks
+  if (tid == 0) {
+    fetch_item(A, myq, begin, end, misc, 0);
+    fetch_item(A, myq, begin, end, misc, 1);
+  }
+  __syncthreads();
+  if (misc[6] == 0) return;
+  LeafItem it = *reinterpret_cast<const LeafItem*>(&misc[0]);
+  load_meta(it);
+  install(it);
+  for (int s = 0;; s ^= 1) {
+    // slot s ^ 1: the next item; slot s (this leaf's, read) takes the one after
+    // it, fetched by thread 0 while the tasks run (it takes tasks after that)
+    const bool more = misc[8 * (s ^ 1) + 6] != 0;
+    const LeafItem nx = *reinterpret_cast<const LeafItem*>(&misc[8 * (s ^ 1)]);
+    if (more) {
+      load_meta(nx);
+      if (tid == 0) fetch_item(A, myq, begin, end, misc, s);
+    }
+    const int m = it.size;
     // tasks: 64 query rows of the item x a range of 32-column tiles; the split
     // minimises rounds x (tiles per task + 1) over the block's waves
     const int row0 = it.row0, row_end = it.row0 + it.rows;
@@ -243,7 +310,7 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
     const int ntask = nq * nc;
     for (;;) {
       int task = 0;
-      if (lane == 0) task = atomicAdd(&misc[2], 1);
+      if (lane == 0) task = atomicAdd(&misc[kMiscTask], 1);
       task = __builtin_amdgcn_readfirstlane(__shfl(task, 0));
       if (task >= ntask) break;
       const int qi = task / nc, cj = task - qi * nc;
@@ -264,10 +331,11 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
         thr[g] = thrs[rr];
         id[g] = lid[rr];
         if (!valid) thr[g] = INFINITY;  // and its bits are cleared below
-        lst[g].reset();
 #pragma unroll
         for (int s2 = 0; s2 < (MODE == kModeCand ? TK : 1); ++s2) key[g][s2] = 0xFFFFFFFFu;
       }
+      lst[0].reset();
+      lst[1].reset();
       const bool valid0 = qb + (lane & 31) < row_end;
       const bool valid1 = qb + 32 + (lane & 31) < row_end;
       auto tile = [&](int t, floatx16& a0, floatx16& a1) {
@@ -374,7 +442,7 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
         if (t + 2 < t1) tile(t + 2, a0, a1);
         epilogue(t + 1, b0, b1);
       }
-      if constexpr (MODE == kModePool) {
+      if constexpr (kPool) {
         flush_pair(A, lid, id, lst);
       } else {
         // stream (column range cj, half) of a row: key slots (2 cj + half) TK
@@ -396,8 +464,10 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
         }
       }
     }
-    __syncthreads();  // every wave done with the leaf
-    item = begin + misc[parity];
+    __syncthreads();  // every wave done with the leaf's tasks; slot s written
+    if (!more) break;
+    it = nx;
+    install(it);
   }
 }
 
@@ -406,10 +476,10 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
 // Largest leaf (rows, a multiple of 32: the tiles read whole 32-row groups)
 // the LDS-resident pool pass stages at dimension d on this device, or 0 when
 // it does not apply (BF16 rows of 32 to 256 bytes in 32-byte multiples).
-int pool_leaf_rows(drpt_ctx* ctx, int d) {
+int pool_leaf_rows(drpt_ctx* ctx, int d, int mode) {
   if (d < 16 || d > 16 * kPlSteps || (d & 15) != 0) return 0;
   const int64_t room = (int64_t)ctx->max_lds - kPlMisc * 4;
-  const int64_t rows = (room / (kPlPitch + 12)) & ~(int64_t)31;
+  const int64_t rows = (room / (int64_t)pool_leaf_row_bytes(mode)) & ~(int64_t)31;
   return rows >= 64 ? (int)std::min<int64_t>(rows, 2 * kPlThreads) : 0;
 }
 
@@ -428,12 +498,16 @@ const void* pool_leaf_fn(int ks) {
 }
 
 int launch_pool_leaf(drpt_ctx* ctx, int mode, int kp, const KnnArgs& a0) {
-  const int mmax = pool_leaf_rows(ctx, a0.d);
-  if (mmax <= 0 || a0.size_hi > mmax || (mode != kModePool && mode != kModeCand)) {
+  if (mode != kModePool && mode != kModeCand) {
+    set_error("internal: pool_leaf mode");
+    return DRPT_ERR_ARG;
+  }
+  const int mmax = pool_leaf_rows(ctx, a0.d, mode);
+  if (mmax <= 0 || a0.size_hi > mmax) {
     set_error("internal: pool_leaf launched for leaves it cannot stage");
     return DRPT_ERR_ARG;
   }
-  const size_t lds = (size_t)mmax * (kPlPitch + 12) + kPlMisc * 4;
+  const size_t lds = (size_t)mmax * pool_leaf_row_bytes(mode) + kPlMisc * 4;
   const int ks = a0.d / 16;
   const void* fn = nullptr;
   int slot = ks;

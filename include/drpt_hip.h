@@ -85,7 +85,8 @@ typedef struct {
   int64_t first_trees;  /* trees whose widened candidates gave the first bounds */
   int64_t pool_cap;     /* pool slots per row */
   int64_t rerank;       /* exact re-rank depth of the first bounds */
-  int64_t reserved[4];
+  int64_t pool_entries; /* leafmates the pooled filter passes appended, over all rows and passes */
+  int64_t reserved[3];
 } drpt_phases;
 
 /* Thread-local message of the last failing call. */

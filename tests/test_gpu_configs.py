@@ -99,7 +99,8 @@ def test_c2_int_mixture_tuned_plans(ctx, orc):
 @pytest.mark.parametrize("prec", ["bf16", "fp8"])
 def test_leaf_resident_and_streaming_leaves(ctx, orc, prec):
     """60000 x 128 integer mixture, T = 8, leaves up to 1024: leaves the
-    LDS-resident passes stage (up to 608 rows at d = 128) and larger ones,
+    LDS-resident passes stage (at d = 128 up to 608 rows for the candidate
+    pass and 544 for the pool pass) and larger ones,
     whose work items are gathered for the streaming kernel, in one build."""
     _log(f"[leaf-split] 60000 x 128 int mixture, T=8, leaf<=1024, {prec}")
     n = 60_000
