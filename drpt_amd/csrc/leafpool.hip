@@ -308,6 +308,14 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
     const int tper = (ntile + nc - 1) / nc;
     nc = (ntile + tper - 1) / tper;  // (candidates: 2 stays 2, as ntile >= 2 there)
     const int ntask = nq * nc;
+    // pool mode: the filter mask of the leaf's last tile (columns past m
+    // cleared), for a branch-free epilogue
+    uint32_t lastmask = 0u;
+    if constexpr (kPool) {
+      const int cbl = 32 * (ntile - 1) + 4 * half;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) lastmask |= (cbl + 8 * (e >> 2) + (e & 3) < m ? 1u : 0u) << e;
+    }
     for (;;) {
       int task = 0;
       if (lane == 0) task = atomicAdd(&misc[kMiscTask], 1);
@@ -434,13 +442,65 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
       // tile's epilogue
       floatx16 a0, a1, b0, b1;
       tile(t0, a0, a1);
-      for (int t = t0; t < t1; t += 2) {
-        const bool has1 = t + 1 < t1;
-        if (has1) tile(t + 1, b0, b1);
-        epilogue(t, a0, a1);
-        if (!has1) break;
-        if (t + 2 < t1) tile(t + 2, a0, a1);
-        epilogue(t + 1, b0, b1);
+      if constexpr (kPool) {
+        const uint32_t vm0 = valid0 ? 0xFFFFu : 0u, vm1 = valid1 ? 0xFFFFu : 0u;
+        // filter masks of tile t (branch-free, so that they share a basic
+        // block with the next tile's MFMAs and the scheduler interleaves them)
+        auto filter = [&](int t, const floatx16& x0, const floatx16& x1, uint32_t& pm0,
+                          uint32_t& pm1) {
+          const int c0 = 32 * t;
+          const uint32_t cm = c0 + 32 > m ? lastmask : 0xFFFFu;
+          pm0 = pass_bits(x0, thr[0]) & cm & vm0 & (c0 == qb ? ~selfbit : 0xFFFFFFFFu);
+          pm1 = pass_bits(x1, thr[1]) & cm & vm1 & (c0 == qb + 32 ? ~selfbit : 0xFFFFFFFFu);
+        };
+        auto survivors = [&](int t, uint32_t pm0, uint32_t pm1) {
+          const int pcb = 32 * t + 4 * half;
+          while (pm0 != 0u) {
+            const int e = __builtin_ctz(pm0);
+            pm0 &= pm0 - 1u;
+            lst[0].push((uint32_t)(pcb + 8 * (e >> 2) + (e & 3)));
+            if (lst[0].cnt == kPlList) flush_list(A, lid, id[0], lst[0]);
+          }
+          while (pm1 != 0u) {
+            const int e = __builtin_ctz(pm1);
+            pm1 &= pm1 - 1u;
+            lst[1].push((uint32_t)(pcb + 8 * (e >> 2) + (e & 3)));
+            if (lst[1].cnt == kPlList) flush_list(A, lid, id[1], lst[1]);
+          }
+        };
+        // tile pairs while two more tiles follow (the next tile's MFMAs and
+        // this tile's filter in one basic block, so that they interleave),
+        // then the last one or two
+        int t = t0;
+        for (; t + 2 < t1; t += 2) {  // a0/a1: tile t, in flight
+          uint32_t p0, p1;
+          tile(t + 1, b0, b1);
+          filter(t, a0, a1, p0, p1);
+          survivors(t, p0, p1);
+          tile(t + 2, a0, a1);
+          filter(t + 1, b0, b1, p0, p1);
+          survivors(t + 1, p0, p1);
+        }
+        uint32_t p0, p1;
+        if (t + 1 < t1) {
+          tile(t + 1, b0, b1);
+          filter(t, a0, a1, p0, p1);
+          survivors(t, p0, p1);
+          filter(t + 1, b0, b1, p0, p1);
+          survivors(t + 1, p0, p1);
+        } else {
+          filter(t, a0, a1, p0, p1);
+          survivors(t, p0, p1);
+        }
+      } else {
+        for (int t = t0; t < t1; t += 2) {
+          const bool has1 = t + 1 < t1;
+          if (has1) tile(t + 1, b0, b1);
+          epilogue(t, a0, a1);
+          if (!has1) break;
+          if (t + 2 < t1) tile(t + 2, a0, a1);
+          epilogue(t + 1, b0, b1);
+        }
       }
       if constexpr (kPool) {
         flush_pair(A, lid, id, lst);
