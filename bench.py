@@ -297,6 +297,9 @@ def main():
     ap.add_argument("--density", type=float, default=None,
                     help="projection density (an integer >= 1 gives the reference's dense "
                          "local projection, -density on the reference command line)")
+    ap.add_argument("--locality", type=int, default=0,
+                    help="1: the row merges read X in tree 0's leaf order (-locality 1); 0: X "
+                         "itself (default: measured faster)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip drpt_autotune (work-item layout per leaf-size bucket) before warm-up")
     ap.add_argument("--item-plan", default=None, metavar="JSON",
@@ -334,6 +337,7 @@ def main():
     p = make_params(w, n, d, X.dtype == np.uint8, prec, a.seed)
     if a.density is not None:
         p.density = a.density
+    p.locality = a.locality
     out = (torch.empty((n, p.nn), dtype=torch.int32, device="cuda:0"),
            torch.empty((n, p.nn), dtype=torch.float32, device="cuda:0"))
 
@@ -381,7 +385,7 @@ def main():
            "config": {"workload": w["desc"], "n": n, "d": d, "ntrees": p.ntrees, "nn": p.nn,
                       "depth": p.depth, "leaf_max": p.leaf_max, "precision": prec,
                       "kprime": p.kprime if prec != "fp32" else 0,
-                      "local_mode": w["mode"], "density": p.density},
+                      "local_mode": w["mode"], "density": p.density, "locality": p.locality},
            "phases_ms": phases, "stages_ms": stages, "phase_rooflines": rl_phases,
            "sum_phase_fraction": sum_frac,
            "n_leaves": tm_last["n_leaves"], "max_leaf": tm_last["max_leaf"],

@@ -241,9 +241,10 @@ class MDRPT:
         self.ctx = Context(device)
         self._X = None
         self._density = None
+        self._locality = 0
         self._result = None
 
-    def grow_trees(self, original_data, density, use_locality_optimization=True, nn=10):
+    def grow_trees(self, original_data, density, use_locality_optimization=False, nn=10):
         torch = _torch()
         X = original_data
         if not torch.is_tensor(X):
@@ -252,6 +253,7 @@ class MDRPT:
             X = X.to(f"cuda:{self.ctx.device}")
         self._X = X.contiguous()
         self._density = float(density)
+        self._locality = 1 if use_locality_optimization else 0
         self.n, self.d = self._X.shape
 
     def gather_nns(self, nn):
@@ -263,6 +265,7 @@ class MDRPT:
         p.seed = self.seed
         p.precision = PRECISIONS[self.precision]
         p.kprime = self.kprime
+        p.locality = self._locality
         if self.leaf_max:
             p.local_mode = DRPT_LOCAL_LEAFMAX
             p.leaf_max = self.leaf_max

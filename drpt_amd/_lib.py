@@ -44,7 +44,8 @@ class Params(ctypes.Structure):
         ("local_mode", ctypes.c_int32), ("leaf_max", ctypes.c_int32),
         ("local_offset", ctypes.c_int32), ("precision", ctypes.c_int32),
         ("kprime", ctypes.c_int32), ("seed", ctypes.c_uint32), ("density", ctypes.c_float),
-        ("tree_group", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4),
+        ("tree_group", ctypes.c_int32), ("locality", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
     def as_dict(self):

@@ -730,6 +730,10 @@ struct LowpRun {
           *novf = nullptr;
   float *kd = nullptr, *boundT = nullptr;
   float* bnd = nullptr;  // [n] K-th distance of the current known lists (the bound)
+  // rows the merges read: X itself, or (locality) X in tree 0's leaf order
+  // with xpos[i] the position of point i in it
+  const void* Xm = nullptr;
+  const int32_t* xpos = nullptr;
 };
 
 static std::string plan_key(const drpt_params* p) {
@@ -786,6 +790,18 @@ static int lowp_setup(drpt_ctx* ctx, const drpt_params* p, const void* X,
                                                          : pool_cap_of(r->K);
   const int64_t n = r->n;
   DRPT_CHECK(group_impl(ctx, p, X, leaf_heap, plan, r->first, &r->G));
+  r->Xm = X;
+  r->xpos = nullptr;
+  if (p->locality) {
+    const size_t rowbytes = (size_t)p->d * (p->dtype == DRPT_U8 ? 1 : 4);
+    void* Xp;
+    int32_t* pos;
+    DRPT_CHECK(ws_get(ctx, "loc.X", (size_t)n * rowbytes + 16, &Xp));
+    DRPT_CHECK(ws(ctx, "loc.pos", (size_t)n, &pos));
+    DRPT_CHECK(launch_permute_rows(ctx, X, rowbytes, r->G.perm, n, Xp, pos));
+    r->Xm = Xp;
+    r->xpos = pos;
+  }
   DRPT_HIP(hipEventRecord(ctx->evp[0], ctx->stream));
   DRPT_CHECK(prep_lowp(ctx, &r->pp, X, r->G.perm, &r->a));
   if (r->pc.precision != r->pp.precision) {
@@ -878,7 +894,8 @@ static int lowp_known(drpt_ctx* ctx, LowpRun* r, const void* X, const ItemPlan& 
   DRPT_CHECK(lowp_pass(ctx, r, kModeCand, plan, r->G.range + 0));
   DRPT_HIP(hipEventRecord(ctx->evp[1], ctx->stream));
   RowMergeArgs m{};
-  m.X = X;
+  m.X = r->Xm;
+  m.xpos = r->xpos;
   m.n = r->n;
   m.d = r->pp.d;
   m.K = r->K;
@@ -957,7 +974,8 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
     DRPT_HIP(hipEventRecord(ctx->ev_pass[4 * g + 1], s));
     DRPT_CHECK(sum_u32(ctx, (const uint32_t*)r.pcnt, n, pool_sum));
     RowMergeArgs f{};
-    f.X = X;
+    f.X = r.Xm;
+    f.xpos = r.xpos;
     f.n = n;
     f.d = p->d;
     f.K = K;

@@ -12,7 +12,10 @@
 // nodes larger than this instead of the reference local stage), -depth,
 // -device, -tree-group (trees per pooled filter pass of a bf16/fp8 build),
 // -autotune 1 (tune the work-item plan on this input before the build).
-// -locality is accepted and has no effect on one GPU.
+// -locality 0|1 (main.cpp:39; default 0 here): bf16/fp8 builds' row merges
+// read a copy of the input in tree 0's leaf order (measured slower on one GPU:
+// the rows' L2 reuse does not depend on their addresses, and the extra index
+// load lengthens every gather).
 #include <hip/hip_runtime_api.h>
 
 #include <chrono>
@@ -55,6 +58,7 @@ int main(int argc, char* argv[]) {
   uint32_t seed = 1;
   std::string precision = "fp32";
   int kprime = 0, leaf_max = 0, depth_override = 0, device = 0, tree_group = 0, autotune = 0;
+  int locality = 0;
   for (int p = 0; p + 1 < argc; p++) {
     const char* a = argv[p];
     const char* v = argv[p + 1];
@@ -66,7 +70,7 @@ int main(int argc, char* argv[]) {
     else if (!std::strcmp(a, "-tree-depth-ratio")) tree_depth_ratio = std::stof(v);
     else if (!std::strcmp(a, "-density")) density = std::atoi(v);  // integer, as main.cpp:71
     else if (!std::strcmp(a, "-nn")) nn = std::atoi(v);
-    else if (!std::strcmp(a, "-locality")) (void)std::atoi(v);
+    else if (!std::strcmp(a, "-locality")) locality = std::atoi(v);
     else if (!std::strcmp(a, "-local-tree-offset")) local_tree_offset = std::atoi(v);
     else if (!std::strcmp(a, "-data-file-format")) file_format = std::atoi(v);
     else if (!std::strcmp(a, "-bytes-for-data-type")) bytes_for_data_type = std::atoi(v);
@@ -98,6 +102,7 @@ int main(int argc, char* argv[]) {
   p.seed = seed;
   p.kprime = kprime;
   p.tree_group = tree_group;
+  p.locality = locality != 0 ? 1 : 0;
   if (precision == "fp32") p.precision = DRPT_FP32;
   else if (precision == "bf16") p.precision = DRPT_BF16;
   else if (precision == "fp8") p.precision = DRPT_FP8;

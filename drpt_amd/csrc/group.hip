@@ -380,6 +380,20 @@ k_select_items(const LeafItem* __restrict__ items, const int64_t* __restrict__ r
   }
 }
 
+// Xp[i] = X[perm[i]], 16-, 4- or 1-byte words W per row; pos[perm[i]] = i
+template <typename W>
+__global__ void k_permute_rows(const W* __restrict__ X, int64_t words, const int32_t* __restrict__ perm,
+                               int64_t n, W* __restrict__ Xp, int32_t* __restrict__ pos) {
+  const int64_t total = n * words;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = k / words, w = k - i * words;
+    const int32_t src = perm[i];
+    Xp[k] = X[(int64_t)src * words + w];
+    if (w == 0) pos[src] = (int32_t)i;
+  }
+}
+
 unsigned grid_for(int64_t n) {
   int64_t b = (n + 1023) / 1024;
   if (b > 2048) b = 2048;
@@ -524,6 +538,22 @@ int launch_tree_range(drpt_ctx* ctx, const int64_t* base, int t0, int t1, int64_
     return DRPT_ERR_ARG;
   }
   hipLaunchKernelGGL(k_tree_range, dim3(1), dim3(64), 0, ctx->stream, base, t0, t1, range);
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
+}
+
+int launch_permute_rows(drpt_ctx* ctx, const void* X, size_t rowbytes, const int32_t* perm,
+                        int64_t n, void* Xp, int32_t* pos) {
+  const unsigned grid = (unsigned)ctx->num_cus * 8;
+  if (rowbytes % 16 == 0)
+    hipLaunchKernelGGL(k_permute_rows<uint4>, dim3(grid), dim3(256), 0, ctx->stream,
+                       (const uint4*)X, (int64_t)(rowbytes / 16), perm, n, (uint4*)Xp, pos);
+  else if (rowbytes % 4 == 0)
+    hipLaunchKernelGGL(k_permute_rows<uint32_t>, dim3(grid), dim3(256), 0, ctx->stream,
+                       (const uint32_t*)X, (int64_t)(rowbytes / 4), perm, n, (uint32_t*)Xp, pos);
+  else
+    hipLaunchKernelGGL(k_permute_rows<uint8_t>, dim3(grid), dim3(256), 0, ctx->stream,
+                       (const uint8_t*)X, (int64_t)rowbytes, perm, n, (uint8_t*)Xp, pos);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }

@@ -166,6 +166,9 @@ int launch_scatter_leaves(drpt_ctx* ctx, const uint32_t* heap, int64_t n, int T,
 int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, const ItemPlan& plan,
                         int first_trees, int64_t* hoff, int64_t* tree_items, int64_t* base,
                         int64_t* range, LeafItem* items, unsigned long long* stats);
+// Xp[i] = X[perm[i]] (rows of rowbytes bytes) and pos[perm[i]] = i, i < n
+int launch_permute_rows(drpt_ctx* ctx, const void* X, size_t rowbytes, const int32_t* perm,
+                        int64_t n, void* Xp, int32_t* pos);
 // range[0] = base[t0], range[1] = base[t1]: the work items of trees [t0, t1)
 int launch_tree_range(drpt_ctx* ctx, const int64_t* base, int t0, int t1, int64_t* range);
 // out[0, out_range[1]) = the items of [range[0], range[1]) with leaf sizes in
@@ -264,6 +267,7 @@ struct RowMergeArgs {
   const float* known_dist;
   const int32_t* skip_flag;  // optional [n]: rows left to the exact row path
   const int32_t* order;      // optional permutation of the rows (processing order)
+  const int32_t* xpos;       // optional: row i of the data is X[xpos[i]] (X a permuted copy)
   int32_t* out_idx;          // [n][K]
   float* out_dist;
   float* out_bound;          // optional [n]: out_dist[row][K - 1] (the row's bound)

@@ -1539,7 +1539,7 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
       neval = A.rerank;
     }
     // exact D, keys (D bits << 32 | id)
-    const XT* xr = X + row * d;
+    const XT* xr = X + (A.xpos != nullptr ? (int64_t)A.xpos[row] : row) * d;
     if (kRowLds) {
       // the row staged once in LDS as float: every lane reads it by broadcast
       // instead of loading it again beside its candidate (same fma chain)
@@ -1551,7 +1551,7 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
       for (int e = 0; e < E; ++e) {
         if (e * 64 + lane < neval) {
           const uint32_t id = (uint32_t)key[e];
-          const XT* xc = X + (int64_t)id * d;
+          const XT* xc = X + (A.xpos != nullptr ? (int64_t)A.xpos[id] : (int64_t)id) * d;
           float acc = 0.0f;
 #pragma unroll 8
           for (int k = 0; k < d; k += 4) {
@@ -1575,7 +1575,8 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
       for (int e = 0; e < E; ++e) {
         if (e * 64 + lane < neval) {
           const uint32_t id = (uint32_t)key[e];
-          key[e] = ((uint64_t)__float_as_uint(exact_dist8(xr, X + (int64_t)id * d, d)) << 32) | id;
+          const XT* xc = X + (A.xpos != nullptr ? (int64_t)A.xpos[id] : (int64_t)id) * d;
+          key[e] = ((uint64_t)__float_as_uint(exact_dist8(xr, xc, d)) << 32) | id;
         } else {
           key[e] = ~0ull;
         }

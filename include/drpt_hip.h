@@ -58,7 +58,11 @@ typedef struct {
   float density;        /* projection density (main.cpp:129-132 default 1/sqrt(d)) */
   int32_t tree_group;   /* BF16/FP8 builds: trees per pooled filter pass, each followed by a
                            merge that tightens the bound (0 = automatic) */
-  int32_t reserved[4];
+  int32_t locality;     /* -locality (main.cpp:39, 0/1): BF16/FP8 builds' row merges read
+                           their rows from a copy of X in tree 0's leaf order (the one-GPU
+                           analogue of calculate_tree_leaf_correlation, drpt_global.cpp:572-732:
+                           leafmates' rows stored together); results are unchanged */
+  int32_t reserved[3];
 } drpt_params;
 
 typedef struct {

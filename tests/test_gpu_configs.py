@@ -149,3 +149,21 @@ def test_c4_sphere_nn32_fp8(ctx, orc, centres):
     _log(f"  exact rows {tm.n_fallback_rows}")
     ri, rd = _oracle(orc, X, p)
     _same(oi, od, ri, rd, f"c4 ({centres} centres)")
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
+def test_locality_copy_same_lists(ctx, orc, prec):
+    """-locality 1: the row merges read X in tree 0's leaf order (a permuted
+    copy and each point's position in it); the lists equal the oracle's and
+    those of the build without it, for float and uint8 input."""
+    _log(f"[locality] 30000 x 128 int mixture, T=8, leaf<=512, {prec}")
+    n = 30_000
+    X = datagen.int_mixture(n)
+    for u8 in (False, True):
+        Xi = X.astype(np.uint8) if u8 else X
+        p = drpt_amd.leafmax_params(n, 128, 8, 10, leaf_max=512, precision=prec, kprime=24, seed=5)
+        p.dtype = drpt_amd.DRPT_U8 if u8 else drpt_amd.DRPT_F32
+        p.locality = 1
+        oi, od, _ = ctx.build_graph(p, _dev(Xi))
+        ri, rd = _oracle(orc, Xi, p)
+        _same(oi, od, ri, rd, f"locality {prec} u8={u8}")
