@@ -12,7 +12,7 @@
 //
 // Pins (what checks each function independently of the HIP path):
 //   orc_build_R        mt19937 10000th-output known answer; generate_canonical
-//                      and Marsaglia-polar restatement in numpy (tests/test_oracle_rng.py)
+//                      and Marsaglia-polar restatement in Python (tests/refpy.py)
 //   orc_project        closed form X.R in exact rational arithmetic (Python
 //                      fractions) on small inputs; linearity in X
 //   orc_node_split     Appendix-A closed form re-derived in Python on small
@@ -20,9 +20,11 @@
 //                      fixed-point sums); strict mode == sequential float32
 //   orc_grow_tree      invariants (counts, leaf ranges), brute-force re-run
 //                      of orc_node_split per node, monotone-shift invariance
-//   orc_local_*        count_leaf_sizes closed form; permutation property;
-//                      std::nth_element tie behaviour (libstdc++) by a second
-//                      direct call
+//   orc_local_*        count_leaf_sizes closed form; the known answer m = 8,
+//                      ld = 3 -> 6 5 4 7 0 3 2 1; a step-by-step Python
+//                      restatement of libstdc++'s introselect (tests/refpy.py)
+//                      for every leaf size below 400 and several depths, with
+//                      all-equal keys and with real keys (value, index)
 //   orc_leaf_knn       brute force over leafmates in Python (FP64 + exact FP32
 //                      FMA sequence via numpy float32)
 //   orc_merge          set semantics: top-K of the union (Python)
