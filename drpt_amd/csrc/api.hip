@@ -893,6 +893,19 @@ static int lowp_known(drpt_ctx* ctx, LowpRun* r, const void* X, const ItemPlan& 
   r->ac.tree0 = 0;
   DRPT_CHECK(lowp_pass(ctx, r, kModeCand, plan, r->G.range + 0));
   DRPT_HIP(hipEventRecord(ctx->evp[1], ctx->stream));
+  if (plan.rerank < 0) {
+    // bounds from the candidates' approximate distances (no exact distance):
+    // the known lists start empty and the first pool merge evaluates every
+    // distinct pool entry
+    float2* nbe;
+    DRPT_CHECK(ws(ctx, r->pc.precision == DRPT_FP8 ? "lowp8.nbe" : "lowp.nbe", (size_t)r->n, &nbe));
+    DRPT_CHECK(launch_row_bound(ctx, r->s1, r->n * r->KP, r->KP, r->first, r->n, r->pp.d, r->K,
+                                nbe, r->bnd));
+    DRPT_CHECK(fill_u32(ctx, (uint32_t*)r->ki, 0xFFFFFFFFu, (size_t)r->n * r->K));  // id -1
+    DRPT_CHECK(fill_u32(ctx, (uint32_t*)r->kd, 0x7F800000u, (size_t)r->n * r->K));  // +inf
+    DRPT_HIP(hipEventRecord(ctx->evp[2], ctx->stream));
+    return DRPT_OK;
+  }
   RowMergeArgs m{};
   m.X = r->Xm;
   m.xpos = r->xpos;
@@ -1023,7 +1036,7 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   ph.tree_groups = ngroups;
   ph.first_trees = r.first;
   ph.pool_cap = r.pool_cap;
-  ph.rerank = plan.rerank > 0 ? std::max(K, std::min(plan.rerank, 2 * K)) : 2 * K;
+  ph.rerank = plan.rerank < 0 ? -1 : plan.rerank > 0 ? std::max(K, std::min(plan.rerank, 2 * K)) : 2 * K;
   ph.pool_entries = (int64_t)npool;
   if (timings) {
     timings->group_ms = ms_between(ctx->ev[6], ctx->ev[7]);
@@ -1250,13 +1263,13 @@ int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* p
     best.first_trees = best_f;
   }
   // exact re-rank depth of the first bounds: fewer gathers in the first merge,
-  // looser bounds for the pools
+  // looser bounds for the pools; -1: bounds from approximate distances alone
   {
     const int K = p->nn;
-    const int cands[3] = {2 * K, (3 * K + 1) / 2, K};
+    const int cands[4] = {2 * K, (3 * K + 1) / 2, K, -1};
     double best_ms = -1.0;
     int best_r = 0;
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < 4; ++i) {
       if (i > 0 && cands[i] == cands[i - 1]) continue;
       best.rerank = cands[i];
       double ms = 0.0;
@@ -1299,8 +1312,8 @@ int drpt_set_item_plan(drpt_ctx* ctx, const drpt_params* p, const int32_t* plan)
     v[kSizeBuckets + b] = plan[kSizeBuckets + b];
   }
   if (plan[2 * kSizeBuckets] < 0 || plan[2 * kSizeBuckets + 1] < 0 ||
-      plan[2 * kSizeBuckets + 2] < 0) {
-    set_error("set_item_plan: tree_group, first_trees and rerank must be >= 0");
+      plan[2 * kSizeBuckets + 2] < -1) {
+    set_error("set_item_plan: tree_group and first_trees must be >= 0, rerank >= -1");
     return DRPT_ERR_ARG;
   }
   v[2 * kSizeBuckets] = plan[2 * kSizeBuckets];

@@ -124,8 +124,9 @@ struct LeafItem {   // one work item: a block of rows of one leaf (or the whole 
 // pooled filter pass (0: the rule of drpt_params.tree_group = 0); first_trees =
 // trees whose widened candidates give the first bounds (0: 4); rerank = their
 // best approximate candidates whose exact distances give those bounds, in
-// [K, 2K] (0: 2K). Chosen by drpt_autotune; the default is whole leaves, the
-// ring and the rules.
+// [K, 2K] (0: 2K; -1: the bounds come from upper bounds of the approximate
+// distances, no exact distance). Chosen by drpt_autotune; the default is whole
+// leaves, the ring and the rules.
 constexpr int kSizeBuckets = 4;
 constexpr int kPlanInts = 2 * kSizeBuckets + 3;  // rpi, ring, tree_group, first_trees, rerank
 struct ItemPlan {
@@ -279,6 +280,12 @@ struct RowMergeArgs {
   const int32_t* nrows_dev;  // kernel: rows to process read on the device (order = heavy)
 };
 int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a);
+// out_bound[row] = K-th smallest upper bound of the exact distance over the
+// distinct ids among a row's candidate keys (slot s of source t of a row at
+// t * tree_stride + row * S + s), from the keys' approximate distances and
+// nbe = (|x~|^2, |x~ - x|) of the quantisation they were computed in
+int launch_row_bound(drpt_ctx* ctx, const uint64_t* src_key, int64_t tree_stride, int S, int ntsrc,
+                     int64_t n, int d, int K, const float2* nbe, float* out_bound);
 // rows listed in ovf_list: exact K best over every leafmate in every tree
 int launch_row_exact(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
                      const LeafItem* items, const int64_t* range, const int32_t* ovf_list,

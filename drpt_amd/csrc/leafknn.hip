@@ -1641,6 +1641,81 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
   }
 }
 
+// First bounds from approximate distances alone (no row of X is read). Per
+// row, the distinct ids among the candidate keys of the first trees; for each
+// an upper bound of its exact distance D (the FP32 chain) from the key's
+// approximate distance a (dropped low bits set: a_up >= a):
+//   |x_i - x_j| <= |x~_i - x~_j| + e_i + e_j,  |x~_i - x~_j|^2 <= a + eta_ij,
+//   eta_ij = (2d + 16) 2^-24 (n_i + n_j)^2 (the FP32 error of a, norms n
+//   widened by (d + 4) 2^-24 as in the filter), D <= |x_i - x_j|^2 (1 + c),
+//   c = (d + 8) 2^-24,
+// so UB = (sqrt(a_up + eta_ij) + e_i + e_j)^2 (1 + c), rounded up. The row's
+// bound is the K-th smallest UB over distinct leafmates (+inf with fewer):
+// at least K distinct leafmates lie within it, so it bounds the final K-th
+// distance and every final neighbour passes the pooled filter. (n_i^2, e_i)
+// of each point come from nbe, the quantisation the keys were computed in.
+template <int E>
+__global__ void __launch_bounds__(64 * kMergeWaves)
+k_row_bound(const uint64_t* __restrict__ src_key, int64_t tree_stride, int S, int ntsrc, int64_t n,
+            int d, int K, const float2* __restrict__ nbe, float* __restrict__ out_bound) {
+  constexpr int kHashBits = E <= 4 ? 9 : 10;
+  constexpr int kHashSlots = 1 << kHashBits;
+  __shared__ uint32_t tab[kMergeWaves][kHashSlots];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t* tb = tab[w];
+  for (int j = lane; j < kHashSlots; j += 64) tb[j] = 0xFFFFFFFFu;
+  const double u = 5.9604644775390625e-08;  // 2^-24
+  const int nslots = S * ntsrc;
+  const int64_t nwaves = (int64_t)gridDim.x * kMergeWaves;
+  for (int64_t row = (int64_t)blockIdx.x * kMergeWaves + w; row < n; row += nwaves) {
+    const float2 mine = nbe[row];
+    const double ni = sqrt((double)mine.x * (1.0 + (d + 4.0) * u));
+    uint32_t ub[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int j = e * 64 + lane;
+      const int t = j / S, sl = j - t * S;
+      uint64_t kv = ~0ull;
+      if (j < nslots) kv = src_key[t * tree_stride + row * S + sl];
+      const int id = kv != ~0ull ? (int)(uint32_t)kv : -1;
+      bool fresh = false;
+      if (id >= 0 && id != (int)row) {
+        uint32_t h = ((uint32_t)id * 2654435761u) >> (32 - kHashBits);
+        while (true) {
+          const uint32_t old = atomicCAS(&tb[h], 0xFFFFFFFFu, (uint32_t)id);
+          if (old == 0xFFFFFFFFu) { fresh = true; break; }
+          if (old == (uint32_t)id) break;
+          h = (h + 1) & (kHashSlots - 1);
+        }
+      }
+      ub[e] = 0xFFFFFFFFu;
+      if (fresh) {
+        const uint32_t abits = (uint32_t)(kv >> 32) | 0x7FFu;
+        if (abits < 0x7F800000u) {
+          const float2 o = nbe[id];
+          const double nj = sqrt((double)o.x * (1.0 + (d + 4.0) * u));
+          const double eta = (2.0 * d + 16.0) * u * (ni + nj) * (ni + nj);
+          const double s = sqrt((double)__uint_as_float(abits) + eta) + (double)mine.y + (double)o.y;
+          const float b = (float)(s * s * (1.0 + (d + 8.0) * u) * (1.0 + 1e-6) + 1e-30);
+          ub[e] = __float_as_uint(b);  // non-negative: orders like its bits
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int j = lane; j < kHashSlots; j += 64) tb[j] = 0xFFFFFFFFu;
+    bitonic_wave<uint32_t, E>(ub, lane);
+    // element K - 1 of the sorted 64 E values: register (K - 1) / 64, lane (K - 1) % 64
+    uint32_t kth = 0xFFFFFFFFu;
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      if (e == (K - 1) / 64) kth = (uint32_t)__shfl((int)ub[e], (K - 1) % 64);
+    if (lane == 0) out_bound[row] = kth >= 0x7F800000u ? INFINITY : __uint_as_float(kth);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 // Exact K best distinct leafmates over every tree, for rows whose pool
 // overflowed or whose leaf is too large to stage. k_row_exact_tree: one block
 // per (row, tree) pair, the row's exact KP best leafmates in that tree (each
@@ -2085,6 +2160,33 @@ void launch_rm(drpt_ctx* ctx, unsigned grid, const RowMergeArgs& a) {
     }
   }
   hipLaunchKernelGGL((k_row_merge<XT, E, false>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+}
+
+int launch_row_bound(drpt_ctx* ctx, const uint64_t* src_key, int64_t tree_stride, int S, int ntsrc,
+                     int64_t n, int d, int K, const float2* nbe, float* out_bound) {
+  const int slots = S * ntsrc;
+  if (K < 1 || K > 64 || slots > 256 || slots < 1) {
+    set_error("row_bound: unsupported sizes (nn in [1, 64], at most 256 candidate slots)");
+    return DRPT_ERR_UNSUPPORTED;
+  }
+  const unsigned grid = (unsigned)std::min<int64_t>((n + kMergeWaves - 1) / kMergeWaves,
+                                                    (int64_t)ctx->num_cus * 16);
+  const int E = slots <= 64 ? 1 : slots <= 128 ? 2 : 4;
+  if (E < (K + 63) / 64) {
+    set_error("row_bound: fewer candidate slots than nn");
+    return DRPT_ERR_UNSUPPORTED;
+  }
+  if (E == 1)
+    hipLaunchKernelGGL(k_row_bound<1>, dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, src_key,
+                       tree_stride, S, ntsrc, n, d, K, nbe, out_bound);
+  else if (E == 2)
+    hipLaunchKernelGGL(k_row_bound<2>, dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, src_key,
+                       tree_stride, S, ntsrc, n, d, K, nbe, out_bound);
+  else
+    hipLaunchKernelGGL(k_row_bound<4>, dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, src_key,
+                       tree_stride, S, ntsrc, n, d, K, nbe, out_bound);
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
 }
 
 int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a) {

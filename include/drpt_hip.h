@@ -88,7 +88,8 @@ typedef struct {
   int64_t tree_groups;  /* pooled filter passes (trees per pass: ntrees / tree_groups, rounded up) */
   int64_t first_trees;  /* trees whose widened candidates gave the first bounds */
   int64_t pool_cap;     /* pool slots per row */
-  int64_t rerank;       /* exact re-rank depth of the first bounds */
+  int64_t rerank;       /* exact re-rank depth of the first bounds (-1: bounds from the
+                           approximate distances alone) */
   int64_t pool_entries; /* leafmates the pooled filter passes appended, over all rows and passes */
   int64_t reserved[3];
 } drpt_phases;
@@ -178,7 +179,8 @@ int drpt_last_phases(drpt_ctx* ctx, drpt_phases* out);
  * trees-per-group (ntrees, 8, 6, 5, 4, 3, 2; when p->tree_group is 0), per
  * number of trees whose widened candidates give the first bounds (4, 2, 1) and
  * per number of their best approximate candidates re-ranked exactly for those
- * bounds (2 nn, 1.5 nn, nn). Keeps the fastest in the context and writes it to
+ * bounds (2 nn, 1.5 nn, nn, or none: -1, bounds from upper bounds of the
+ * approximate distances). Keeps the fastest in the context and writes it to
  * plan_out[11]: rows per item of buckets 0-3, their ring flags, trees per
  * group, first trees, re-rank depth. Later drpt_build_graph calls
  * with the same n, d, dtype, precision, nn, ntrees, depth, leaf_max,
@@ -188,9 +190,10 @@ int drpt_autotune(drpt_ctx* ctx, const drpt_params* p, const void* X, int32_t* p
 
 /* Sets the work-item plan of a workload shape, in drpt_autotune's plan_out
  * layout (rows per item >= 64, pipeline flags 0 (registers) / 1 (LDS tile
- * ring) / 2 (leaf resident in LDS, pooled filter pass), trees per group, first trees
- * and re-rank depth >= 0 with 0 for the automatic rules; the depth is clamped
- * to [nn, 2 nn]): restores a plan saved from an earlier drpt_autotune. No
+ * ring) / 2 (leaf resident in LDS, pooled filter pass), trees per group and first
+ * trees >= 0 and re-rank depth >= -1, with 0 for the automatic rules; a depth
+ * above 0 is clamped to [nn, 2 nn], -1 takes the first bounds from the
+ * approximate distances): restores a plan saved from an earlier drpt_autotune. No
  * counterpart in the reference. */
 int drpt_set_item_plan(drpt_ctx* ctx, const drpt_params* p, const int32_t* plan);
 

@@ -65,7 +65,8 @@ def test_c2_int_mixture_tuned_plans(ctx, orc):
     """100000 x 128 integer mixture, T = 16, nn = 10, leaf <= 512, BF16
     k' = 24: the plan the benchmark's autotuner chose at 1M points (whole
     leaves resident in LDS, one 16-tree group, 4 first trees, re-rank 20),
-    the same with the streaming LDS tile ring, a
+    the same with first bounds from approximate distances alone (re-rank -1,
+    one and four tree groups), the same with the streaming LDS tile ring, a
     row-block plan with four groups and a shallow re-rank, the autotuner's
     own plan for this input, and the FP32 build."""
     _log("[c2] 100000 x 128 int mixture, T=16, leaf<=512, bf16 k'=24")
@@ -77,6 +78,10 @@ def test_c2_int_mixture_tuned_plans(ctx, orc):
     plans = [
         {"rows_per_item": [WHOLE] * 4, "ring": [2] * 4, "tree_group": 16, "first_trees": 4,
          "rerank": 20},
+        {"rows_per_item": [WHOLE] * 4, "ring": [2] * 4, "tree_group": 16, "first_trees": 4,
+         "rerank": -1},
+        {"rows_per_item": [WHOLE] * 4, "ring": [2] * 4, "tree_group": 4, "first_trees": 2,
+         "rerank": -1},
         {"rows_per_item": [WHOLE] * 4, "ring": [1] * 4, "tree_group": 16, "first_trees": 4,
          "rerank": 20},
         {"rows_per_item": [WHOLE, 64, 64, 128], "ring": [1, 0, 1, 1], "tree_group": 4,
@@ -133,18 +138,18 @@ def test_c3_gauss960_leaves_to_2048(ctx, orc):
     _same(oi, od, ri, rd, "c3")
 
 
-@pytest.mark.parametrize("centres", [4096, 20])
-def test_c4_sphere_nn32_fp8(ctx, orc, centres):
+@pytest.mark.parametrize("centres,rerank", [(4096, 48), (20, 48), (4096, -1), (20, -1)])
+def test_c4_sphere_nn32_fp8(ctx, orc, centres, rerank):
     """50000 x 96 unit-sphere mixture, T = 32, nn = 32, leaf <= 1024, FP8
-    k' = 64, 8-tree groups, re-rank 48; with the workload's 4096 centres, and
-    with 20 (the 10M-point workload's points per centre: dense clusters, many
-    pool survivors per row)."""
-    _log(f"[c4] 50000 x 96 sphere ({centres} centres), T=32, nn=32, fp8 k'=64")
+    k' = 64, 8-tree groups, re-rank 48 or first bounds from the FP8 distances
+    alone (-1); with the workload's 4096 centres, and with 20 (the 10M-point
+    workload's points per centre: dense clusters, many pool survivors per row)."""
+    _log(f"[c4] 50000 x 96 sphere ({centres} centres), T=32, nn=32, fp8 k'=64, rerank {rerank}")
     n = 50_000
     X = datagen.sphere_mixture(n, centres=centres)
     p = drpt_amd.leafmax_params(n, 96, 32, 32, leaf_max=1024, precision="fp8", kprime=64, seed=1)
     ctx.set_item_plan(p, {"rows_per_item": [WHOLE] * 4, "ring": [1] * 4, "tree_group": 8,
-                          "first_trees": 0, "rerank": 48})
+                          "first_trees": 0, "rerank": rerank})
     oi, od, tm = ctx.build_graph(p, _dev(X))
     _log(f"  exact rows {tm.n_fallback_rows}")
     ri, rd = _oracle(orc, X, p)

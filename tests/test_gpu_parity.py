@@ -328,7 +328,8 @@ def test_autotune_and_item_plans(ctx, prec, case, leaf_max, T):
     """drpt_autotune returns a valid plan; builds with the tuned plan and with
     forced plans (leaves cut into 64-row items, register pipeline, mixed
     pipelines across buckets: one launch per bucket; 2 / 3 trees per pooled
-    pass) equal the default build bit for bit."""
+    pass; first bounds from approximate distances) equal the default build
+    bit for bit."""
     _log(f"[autotune] {prec} {case} leaf_max={leaf_max}")
     X = CASES[case]()
     n, d = X.shape
@@ -346,7 +347,7 @@ def test_autotune_and_item_plans(ctx, prec, case, leaf_max, T):
         assert all(g in (0, 1, 2) for g in plan["ring"])
         assert 1 <= plan["tree_group"] <= T
         assert plan["first_trees"] in (1, 2, 4)
-        assert 10 <= plan["rerank"] <= 20
+        assert 10 <= plan["rerank"] <= 20 or plan["rerank"] == -1
         forced = [plan,
                   {"rows_per_item": [64] * 4, "ring": [0] * 4, "tree_group": 2, "first_trees": 1,
                    "rerank": 10},
@@ -354,7 +355,9 @@ def test_autotune_and_item_plans(ctx, prec, case, leaf_max, T):
                    "first_trees": 2},
                   {"rows_per_item": [1 << 30] * 4, "ring": [0, 1, 0, 1], "tree_group": 0},
                   {"rows_per_item": [1 << 30, 64, 1 << 30, 128], "ring": [2, 2, 1, 2],
-                   "tree_group": 2}]
+                   "tree_group": 2},
+                  {"rows_per_item": [1 << 30] * 4, "ring": [2] * 4, "tree_group": 2,
+                   "rerank": -1}]
         for pl in forced:
             c.set_item_plan(p, pl)
             oi, od, _ = c.build_graph(p, Xd)
