@@ -1649,7 +1649,9 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
 //   eta_ij = (2d + 16) 2^-24 (n_i + n_j)^2 (the FP32 error of a, norms n
 //   widened by (d + 4) 2^-24 as in the filter), D <= |x_i - x_j|^2 (1 + c),
 //   c = (d + 8) 2^-24,
-// so UB = (sqrt(a_up + eta_ij) + e_i + e_j)^2 (1 + c), rounded up. The row's
+// so UB = (sqrt(a_up + eta_ij) + e_i + e_j)^2 (1 + c), evaluated in FP32 and
+// widened by (1 + 64 2^-24) (the dozen roundings of the evaluation, sqrt
+// within 2 ulp, stay below 16 2^-24 relative) plus 1e-30. The row's
 // bound is the K-th smallest UB over distinct leafmates (+inf with fewer):
 // at least K distinct leafmates lie within it, so it bounds the final K-th
 // distance and every final neighbour passes the pooled filter. (n_i^2, e_i)
@@ -1664,12 +1666,14 @@ k_row_bound(const uint64_t* __restrict__ src_key, int64_t tree_stride, int S, in
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint32_t* tb = tab[w];
   for (int j = lane; j < kHashSlots; j += 64) tb[j] = 0xFFFFFFFFu;
-  const double u = 5.9604644775390625e-08;  // 2^-24
+  const float u = 5.9604644775390625e-08f;  // 2^-24
+  const float wn = 1.0f + (d + 4.0f) * u, wc = (1.0f + (d + 8.0f) * u) * (1.0f + 64.0f * u);
+  const float ce = (2.0f * d + 16.0f) * u;
   const int nslots = S * ntsrc;
   const int64_t nwaves = (int64_t)gridDim.x * kMergeWaves;
   for (int64_t row = (int64_t)blockIdx.x * kMergeWaves + w; row < n; row += nwaves) {
     const float2 mine = nbe[row];
-    const double ni = sqrt((double)mine.x * (1.0 + (d + 4.0) * u));
+    const float ni = sqrtf(mine.x * wn);
     uint32_t ub[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -1693,11 +1697,11 @@ k_row_bound(const uint64_t* __restrict__ src_key, int64_t tree_stride, int S, in
         const uint32_t abits = (uint32_t)(kv >> 32) | 0x7FFu;
         if (abits < 0x7F800000u) {
           const float2 o = nbe[id];
-          const double nj = sqrt((double)o.x * (1.0 + (d + 4.0) * u));
-          const double eta = (2.0 * d + 16.0) * u * (ni + nj) * (ni + nj);
-          const double s = sqrt((double)__uint_as_float(abits) + eta) + (double)mine.y + (double)o.y;
-          const float b = (float)(s * s * (1.0 + (d + 8.0) * u) * (1.0 + 1e-6) + 1e-30);
-          ub[e] = __float_as_uint(b);  // non-negative: orders like its bits
+          const float nj = sqrtf(o.x * wn);
+          const float eta = ce * (ni + nj) * (ni + nj);
+          const float s = sqrtf(__uint_as_float(abits) + eta) + mine.y + o.y;
+          const float b = s * s * wc + 1e-30f;
+          ub[e] = __float_as_uint(b);  // non-negative: orders like its bits (+inf past the range)
         }
       }
     }
