@@ -273,8 +273,12 @@ def phase_rooflines(p, d, L, xbytes, tm, ph, prec):
             + 12.0 * T * n * (2 if prec == "fp8" else 1), HBM_PEAK, "B")
         add("candidate_pass", ph["cand_ms"], gram * first / T, MFMA_FP8 if prec == "fp8" else MFMA_BF16,
             "flop")
-        add("known_merge", ph["known_ms"], n * (ph["rerank"] * d * xbytes + first * kp * 8.0 + 8.0 * K),
-            HBM_PEAK, "B")
+        if ph["rerank"] < 0:
+            # first bounds from approximate distances: the keys, one (|x~|^2, e) gather per slot
+            add("first_bound", ph["known_ms"], n * first * kp * 16.0, HBM_PEAK, "B")
+        else:
+            add("known_merge", ph["known_ms"],
+                n * (ph["rerank"] * d * xbytes + first * kp * 8.0 + 8.0 * K), HBM_PEAK, "B")
         add("bound_gather", ph["bound_ms"], 12.0 * T * n, HBM_PEAK, "B")
         add("pool_pass", ph["pool_ms"], gram, MFMA_BF16, "flop")
         add("pool_merge", ph["pool_merge_ms"], n * (16.0 * K * T + 4.0 * K), HBM_PEAK, "B")

@@ -1641,6 +1641,174 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
   }
 }
 
+// Pool merge with two rows per wave (lanes 0-31 one row, 32-63 the next), for
+// one source of ids (a pool) and K <= 32. Per row: its known list (sorted,
+// -1 padded) is the running top K; the pool's ids minus known ones and
+// duplicates (open-addressing hash per half wave) are listed in LDS; their
+// exact D are taken 32 at a time, one per lane, each chunk sorted by (D, id)
+// (32-lane bitonic network) and merged into the running list (half-cleaner
+// against the reversed chunk, then 5 stages). Any pool count works, so no
+// light / heavy split is needed, and the exact-distance phase fills the wave
+// with both rows' candidates.
+constexpr int kM2HashBits = 10;      // slots per half wave: more than its insertions (pool + K)
+constexpr int kM2Fresh = 512;        // distinct new ids a row may list (pools of up to 512 slots)
+constexpr int kM2RowLds = 256;       // elements of a row staged in LDS (two rows per wave)
+
+// ascending sort of the 32 values of each half wave
+__device__ __forceinline__ void sort32(uint64_t& v, int hl) {
+#pragma unroll
+  for (int k = 2; k <= 32; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const uint64_t o = xor_lane(v, j);
+      const bool asc = (hl & k) == 0;
+      const bool lower = (hl & j) == 0;
+      const uint64_t mn = v < o ? v : o, mx = v < o ? o : v;
+      v = (asc == lower) ? mn : mx;
+    }
+  }
+}
+
+// sorted half-wave lists r and c (ascending): r = the 32 smallest of both
+__device__ __forceinline__ void merge32(uint64_t& r, uint64_t c, int hl) {
+  const uint64_t rc = (uint64_t)(uint32_t)__shfl((int)(uint32_t)c, 31 - hl, 32) |
+                      ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(c >> 32), 31 - hl, 32) << 32);
+  r = r < rc ? r : rc;  // bitonic: the 32 smallest
+#pragma unroll
+  for (int j = 16; j > 0; j >>= 1) {
+    const uint64_t o = xor_lane(r, j);
+    const bool lower = (hl & j) == 0;
+    const uint64_t mn = r < o ? r : o, mx = r < o ? o : r;
+    r = lower ? mn : mx;
+  }
+}
+
+template <typename XT, bool RL>
+__global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge2(const RowMergeArgs A) {
+  constexpr int kSlots = 1 << kM2HashBits;
+  __shared__ uint32_t tab[kMergeWaves][2][kSlots];
+  __shared__ int32_t fresh[kMergeWaves][2][kM2Fresh];
+  __shared__ __align__(16) float qrow[kMergeWaves][2][RL ? kM2RowLds : 4];
+  const XT* __restrict__ X = static_cast<const XT*>(A.X);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int h = lane >> 5, hl = lane & 31;
+  const int K = A.K, d = A.d;
+  uint32_t* tb = tab[w][h];
+  int32_t* fr = fresh[w][h];
+  for (int j = hl; j < kSlots; j += 32) tb[j] = 0xFFFFFFFFu;
+  const uint64_t hmask = h ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
+  // pairs of rows in A.order, one contiguous chunk per XCD (blocks go round-robin over 8)
+  const int xcd = blockIdx.x & 7;
+  const int64_t nblk = (gridDim.x - xcd + 7) >> 3;
+  const int64_t nr = A.n;
+  const int64_t npairs = (nr + 1) >> 1;
+  const int64_t chunk = (npairs + 7) >> 3;
+  const int64_t lo = xcd * chunk, hi = min(npairs, lo + chunk);
+  for (int64_t pi = lo + (int64_t)(blockIdx.x >> 3) * kMergeWaves + w; pi < hi;
+       pi += nblk * kMergeWaves) {
+    const int64_t ri = 2 * pi + h;
+    const bool has = ri < nr;
+    const int64_t row = has ? (A.order != nullptr ? (int64_t)A.order[ri] : ri) : 0;
+    const bool skip = has && A.skip_flag != nullptr && A.skip_flag[row] != 0;
+    const bool act = has && !skip;
+    // known list: the running top K (keys D bits << 32 | id, ascending, ~0 padded)
+    int kid = -1;
+    float kdv = INFINITY;
+    if (has && A.known_idx != nullptr && hl < K) {
+      kid = A.known_idx[row * K + hl];
+      kdv = A.known_dist[row * K + hl];
+    }
+    if (skip && hl < K) {  // left to the exact row path; the known list passes through
+      A.out_idx[row * K + hl] = kid;
+      A.out_dist[row * K + hl] = kdv;
+    }
+    if (skip && A.out_bound != nullptr && hl == K - 1) A.out_bound[row] = kdv;
+    uint64_t run = (act && kid >= 0) ? (((uint64_t)__float_as_uint(kdv) << 32) | (uint32_t)kid) : ~0ull;
+    if (act && kid >= 0) {
+      uint32_t hh = ((uint32_t)kid * 2654435761u) >> (32 - kM2HashBits);
+      while (true) {
+        const uint32_t old = atomicCAS(&tb[hh], 0xFFFFFFFFu, (uint32_t)kid);
+        if (old == 0xFFFFFFFFu || old == (uint32_t)kid) break;
+        hh = (hh + 1) & (kSlots - 1);
+      }
+    }
+    // distinct new ids of the pool
+    const int cnt = act ? min(A.cnt_tree[row], A.S) : 0;
+    const int cmax = max(cnt, __shfl_xor(cnt, 32));
+    int nf = 0;
+    for (int s0 = 0; s0 < cmax; s0 += 32) {
+      const int sl = s0 + hl;
+      const int id = sl < cnt ? A.src[row * A.S + sl] : -1;
+      bool isnew = false;
+      if (id >= 0 && id != (int)row) {
+        uint32_t hh = ((uint32_t)id * 2654435761u) >> (32 - kM2HashBits);
+        while (true) {
+          const uint32_t old = atomicCAS(&tb[hh], 0xFFFFFFFFu, (uint32_t)id);
+          if (old == 0xFFFFFFFFu) { isnew = true; break; }
+          if (old == (uint32_t)id) break;
+          hh = (hh + 1) & (kSlots - 1);
+        }
+      }
+      const uint64_t bal = __ballot(isnew) & hmask;
+      const uint64_t below = bal & ((1ull << lane) - 1);
+      if (isnew && nf + __popcll(below) < kM2Fresh) fr[nf + __popcll(below)] = id;
+      nf = min(kM2Fresh, nf + __popcll(bal));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int j = hl; j < kSlots; j += 32) tb[j] = 0xFFFFFFFFu;
+    // exact D of the new ids, 32 at a time per row
+    const int64_t qpos = A.xpos != nullptr ? (int64_t)A.xpos[row] : row;
+    const XT* xr = X + qpos * d;
+    if constexpr (RL) {
+      float* qf = qrow[w][h];
+      if (act)
+        for (int k = hl * 4; k < d; k += 128) *reinterpret_cast<float4*>(&qf[k]) = load4f(xr + k);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    const int nfmax = max(nf, __shfl_xor(nf, 32));
+    for (int c0 = 0; c0 < nfmax; c0 += 32) {
+      const int j = c0 + hl;
+      uint64_t key = ~0ull;
+      if (j < nf) {
+        const int id = fr[j];
+        const XT* xc = X + (A.xpos != nullptr ? (int64_t)A.xpos[id] : (int64_t)id) * d;
+        float acc;
+        if constexpr (RL) {
+          const float* qf = qrow[w][h];
+          acc = 0.0f;
+#pragma unroll 8
+          for (int k = 0; k < d; k += 4) {
+            const float4 a = *reinterpret_cast<const float4*>(&qf[k]), b = load4f(xc + k);
+            float t = __fsub_rn(a.x, b.x);
+            acc = fmaf(t, t, acc);
+            t = __fsub_rn(a.y, b.y);
+            acc = fmaf(t, t, acc);
+            t = __fsub_rn(a.z, b.z);
+            acc = fmaf(t, t, acc);
+            t = __fsub_rn(a.w, b.w);
+            acc = fmaf(t, t, acc);
+          }
+        } else {
+          acc = exact_dist8(xr, xc, d);
+        }
+        key = ((uint64_t)__float_as_uint(acc) << 32) | (uint32_t)id;
+      }
+      sort32(key, hl);
+      merge32(run, key, hl);
+    }
+    if (act && hl < K) {
+      A.out_idx[row * K + hl] = run == ~0ull ? -1 : (int32_t)(uint32_t)run;
+      A.out_dist[row * K + hl] = run == ~0ull ? INFINITY : __uint_as_float((uint32_t)(run >> 32));
+    }
+    if (act && A.out_bound != nullptr && hl == K - 1)
+      A.out_bound[row] = run == ~0ull ? INFINITY : __uint_as_float((uint32_t)(run >> 32));
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 // First bounds from approximate distances alone (no row of X is read). Per
 // row, the distinct ids among the candidate keys of the first trees; for each
 // an upper bound of its exact distance D (the FP32 chain) from the key's
@@ -2195,6 +2363,24 @@ int launch_row_bound(drpt_ctx* ctx, const uint64_t* src_key, int64_t tree_stride
 
 int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a) {
   const int slots = a.S * a.ntsrc;
+  if (a.ntsrc == 1 && a.src != nullptr && a.src_key == nullptr && a.cnt_tree != nullptr &&
+      a.K <= 32 && a.rerank == 0 && a.nrows_dev == nullptr && a.S <= kM2Fresh &&
+      a.S + a.K < (1 << kM2HashBits)) {
+    // pools: two rows per wave
+    const int64_t want = std::min<int64_t>((a.n + 2 * kMergeWaves - 1) / (2 * kMergeWaves),
+                                           (int64_t)ctx->num_cus * 16);
+    const unsigned grid = (unsigned)(((std::max<int64_t>(want, 8) + 7) / 8) * 8);
+    const bool rl = (a.d & 3) == 0 && a.d >= 128 && a.d <= kM2RowLds;
+    if (dtype == DRPT_U8) {
+      if (rl) hipLaunchKernelGGL((k_row_merge2<uint8_t, true>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+      else hipLaunchKernelGGL((k_row_merge2<uint8_t, false>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+    } else {
+      if (rl) hipLaunchKernelGGL((k_row_merge2<float, true>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+      else hipLaunchKernelGGL((k_row_merge2<float, false>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
+    }
+    DRPT_HIP(hipGetLastError());
+    return DRPT_OK;
+  }
   if (a.K < 1 || a.K > 64 || slots > 1024 || a.rerank > 64 * 4 || a.ntsrc > 64 ||
       (a.src_key != nullptr && slots > 256)) {
     set_error("row_merge: unsupported sizes (nn in [1, 64], at most 1024 candidate slots, "
