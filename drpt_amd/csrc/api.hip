@@ -734,6 +734,7 @@ struct LowpRun {
   // with xpos[i] the position of point i in it
   const void* Xm = nullptr;
   const int32_t* xpos = nullptr;
+  int32_t* lossy = nullptr;  // device flag: some row differs from its bf16 copy
 };
 
 static std::string plan_key(const drpt_params* p) {
@@ -804,6 +805,13 @@ static int lowp_setup(drpt_ctx* ctx, const drpt_params* p, const void* X,
   }
   DRPT_HIP(hipEventRecord(ctx->evp[0], ctx->stream));
   DRPT_CHECK(prep_lowp(ctx, &r->pp, X, r->G.perm, &r->a));
+  {
+    float2* nbe;
+    DRPT_CHECK(ws(ctx, "lowp.nbe", (size_t)n, &nbe));
+    DRPT_CHECK(ws(ctx, "lowp.lossy", 1, &r->lossy));
+    DRPT_CHECK(fill_u32(ctx, (uint32_t*)r->lossy, 0u, 1));
+    DRPT_CHECK(launch_lossy_flag(ctx, nbe, n, r->lossy));
+  }
   if (r->pc.precision != r->pp.precision) {
     DRPT_CHECK(prep_lowp(ctx, &r->pc, X, r->G.perm, &r->ac));
   } else {
@@ -989,6 +997,8 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
     RowMergeArgs f{};
     f.X = r.Xm;
     f.xpos = r.xpos;
+    f.Xb = static_cast<const uint16_t*>(r.a.Xq);  // the pooled passes' bf16 copy
+    f.lossy = r.lossy;
     f.n = n;
     f.d = p->d;
     f.K = K;

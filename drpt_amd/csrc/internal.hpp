@@ -191,6 +191,8 @@ int launch_apply_local(drpt_ctx* ctx, const int32_t* perm_in, int32_t* perm_out,
 // scale 2^pexp[i] per row), nbe[i] = (|x~|^2, |x~ - x|) of the dequantised rows
 int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, int precision,
                      void* Xq, float2* nbe, int8_t* pexp);
+// *lossy = 1 when some row's quantisation error (nbe[i].y) is not 0 (the caller zeroes it)
+int launch_lossy_flag(drpt_ctx* ctx, const float2* nbe, int64_t n, int32_t* lossy);
 // leaf-ordered |x~|^2, |x~ - x| and (fp8) scale exponents: (lnbT[i], lerrT[i])
 // = nbe[perm[i]], lexpT[i] = pexp[perm[i]] (pexp may be null)
 int launch_leaf_gather(drpt_ctx* ctx, const int32_t* perm, int64_t total, const float2* nbe,
@@ -269,6 +271,11 @@ struct RowMergeArgs {
   const int32_t* skip_flag;  // optional [n]: rows left to the exact row path
   const int32_t* order;      // optional permutation of the rows (processing order)
   const int32_t* xpos;       // optional: row i of the data is X[xpos[i]] (X a permuted copy)
+  // optional bf16 copy of X (row i at Xb + i * d) and a device flag, 0 when
+  // every row of it equals its X row exactly: the exact distances then read
+  // the bf16 rows (same values, half the bytes)
+  const uint16_t* Xb;
+  const int32_t* lossy;
   int32_t* out_idx;          // [n][K]
   float* out_dist;
   float* out_bound;          // optional [n]: out_dist[row][K - 1] (the row's bound)

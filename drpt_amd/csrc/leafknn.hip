@@ -1292,6 +1292,15 @@ __global__ void k_leaf_gather(const int32_t* __restrict__ perm, int64_t total,
   }
 }
 
+// *lossy = 1 when some row's quantisation error is not 0
+__global__ void k_lossy_flag(const float2* __restrict__ nbe, int64_t n, int32_t* __restrict__ lossy) {
+  bool any = false;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    any = any || nbe[i].y != 0.0f;
+  if (__ballot(any) != 0 && (threadIdx.x & 63) == 0) atomicOr(lossy, 1);
+}
+
 // leaf-ordered gather of one value per point: outT[i] = src[perm[i] * stride + offset]
 __global__ void k_leaf_gather_val(const int32_t* __restrict__ perm, int64_t total,
                                   const float* __restrict__ src, int stride, int offset,
@@ -1649,9 +1658,10 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge(const RowMergeAr
 // (32-lane bitonic network) and merged into the running list (half-cleaner
 // against the reversed chunk, then 5 stages). Any pool count works, so no
 // light / heavy split is needed, and the exact-distance phase fills the wave
-// with both rows' candidates.
-constexpr int kM2HashBits = 10;      // slots per half wave: more than its insertions (pool + K)
-constexpr int kM2Fresh = 512;        // distinct new ids a row may list (pools of up to 512 slots)
+// with both rows' candidates. Rows whose pools hold more than kM2Fresh
+// entries are listed (heavy) for the full-width pass.
+constexpr int kM2HashBits = 9;       // slots per half wave: more than its insertions (pool + K)
+constexpr int kM2Fresh = 256;        // pool entries a row may hold here
 constexpr int kM2RowLds = 256;       // elements of a row staged in LDS (two rows per wave)
 
 // ascending sort of the 32 values of each half wave
@@ -1710,7 +1720,10 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge2(const RowMergeA
     const bool has = ri < nr;
     const int64_t row = has ? (A.order != nullptr ? (int64_t)A.order[ri] : ri) : 0;
     const bool skip = has && A.skip_flag != nullptr && A.skip_flag[row] != 0;
-    const bool act = has && !skip;
+    const int cnt0 = has && !skip ? min(A.cnt_tree[row], A.S) : 0;
+    const bool heavy = cnt0 > kM2Fresh;  // to the full-width pass
+    if (heavy && hl == 0) A.heavy[atomicAdd(A.nheavy, 1)] = (int32_t)row;
+    const bool act = has && !skip && !heavy;
     // known list: the running top K (keys D bits << 32 | id, ascending, ~0 padded)
     int kid = -1;
     float kdv = INFINITY;
@@ -1733,7 +1746,7 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge2(const RowMergeA
       }
     }
     // distinct new ids of the pool
-    const int cnt = act ? min(A.cnt_tree[row], A.S) : 0;
+    const int cnt = act ? cnt0 : 0;
     const int cmax = max(cnt, __shfl_xor(cnt, 32));
     int nf = 0;
     for (int s0 = 0; s0 < cmax; s0 += 32) {
@@ -1768,6 +1781,9 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge2(const RowMergeA
       __builtin_amdgcn_wave_barrier();
     }
     const int nfmax = max(nf, __shfl_xor(nf, 32));
+    // rows exactly representable in bf16 (integer data, for one): read the
+    // bf16 copy, the same values in half the bytes
+    const bool b16 = RL && A.Xb != nullptr && (d & 7) == 0 && *A.lossy == 0;
     for (int c0 = 0; c0 < nfmax; c0 += 32) {
       const int j = c0 + hl;
       uint64_t key = ~0ull;
@@ -1775,7 +1791,27 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge2(const RowMergeA
         const int id = fr[j];
         const XT* xc = X + (A.xpos != nullptr ? (int64_t)A.xpos[id] : (int64_t)id) * d;
         float acc;
-        if constexpr (RL) {
+        if (b16) {
+          const float* qf = qrow[w][h];
+          const uint16_t* xb = A.Xb + (int64_t)id * d;
+          acc = 0.0f;
+#pragma unroll 4
+          for (int k = 0; k < d; k += 8) {
+            const uint4 v = *reinterpret_cast<const uint4*>(xb + k);
+            const float4 a0 = *reinterpret_cast<const float4*>(&qf[k]);
+            const float4 a1 = *reinterpret_cast<const float4*>(&qf[k + 4]);
+            const float b[8] = {__uint_as_float(v.x << 16), __uint_as_float(v.x & 0xFFFF0000u),
+                                __uint_as_float(v.y << 16), __uint_as_float(v.y & 0xFFFF0000u),
+                                __uint_as_float(v.z << 16), __uint_as_float(v.z & 0xFFFF0000u),
+                                __uint_as_float(v.w << 16), __uint_as_float(v.w & 0xFFFF0000u)};
+            const float a[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const float t = __fsub_rn(a[u], b[u]);
+              acc = fmaf(t, t, acc);
+            }
+          }
+        } else if constexpr (RL) {
           const float* qf = qrow[w][h];
           acc = 0.0f;
 #pragma unroll 8
@@ -2334,6 +2370,13 @@ void launch_rm(drpt_ctx* ctx, unsigned grid, const RowMergeArgs& a) {
   hipLaunchKernelGGL((k_row_merge<XT, E, false>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
 }
 
+int launch_lossy_flag(drpt_ctx* ctx, const float2* nbe, int64_t n, int32_t* lossy) {
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 1024));
+  hipLaunchKernelGGL(k_lossy_flag, dim3(grid), dim3(256), 0, ctx->stream, nbe, n, lossy);
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
+}
+
 int launch_row_bound(drpt_ctx* ctx, const uint64_t* src_key, int64_t tree_stride, int S, int ntsrc,
                      int64_t n, int d, int K, const float2* nbe, float* out_bound) {
   const int slots = S * ntsrc;
@@ -2364,9 +2407,11 @@ int launch_row_bound(drpt_ctx* ctx, const uint64_t* src_key, int64_t tree_stride
 int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a) {
   const int slots = a.S * a.ntsrc;
   if (a.ntsrc == 1 && a.src != nullptr && a.src_key == nullptr && a.cnt_tree != nullptr &&
-      a.K <= 32 && a.rerank == 0 && a.nrows_dev == nullptr && a.S <= kM2Fresh &&
-      a.S + a.K < (1 << kM2HashBits)) {
-    // pools: two rows per wave
+      a.K <= 32 && a.rerank == 0 && a.nrows_dev == nullptr && kM2Fresh + a.K < (1 << kM2HashBits) &&
+      (a.S <= kM2Fresh || (a.heavy != nullptr && a.nheavy != nullptr))) {
+    // pools: two rows per wave; rows of more than kM2Fresh entries listed and
+    // merged by the full-width pass after it
+    if (a.S > kM2Fresh) DRPT_CHECK(fill_u32(ctx, (uint32_t*)a.nheavy, 0u, 1));
     const int64_t want = std::min<int64_t>((a.n + 2 * kMergeWaves - 1) / (2 * kMergeWaves),
                                            (int64_t)ctx->num_cus * 16);
     const unsigned grid = (unsigned)(((std::max<int64_t>(want, 8) + 7) / 8) * 8);
@@ -2379,10 +2424,13 @@ int launch_row_merge(drpt_ctx* ctx, int dtype, RowMergeArgs a) {
       else hipLaunchKernelGGL((k_row_merge2<float, false>), dim3(grid), dim3(64 * kMergeWaves), 0, ctx->stream, a);
     }
     DRPT_HIP(hipGetLastError());
-    return DRPT_OK;
-  }
-  if (a.K < 1 || a.K > 64 || slots > 1024 || a.rerank > 64 * 4 || a.ntsrc > 64 ||
-      (a.src_key != nullptr && slots > 256)) {
+    if (a.S <= kM2Fresh) return DRPT_OK;
+    a.order = a.heavy;
+    a.nrows_dev = a.nheavy;
+    a.heavy = nullptr;
+    a.nheavy = nullptr;
+  } else if (a.K < 1 || a.K > 64 || slots > 1024 || a.rerank > 64 * 4 || a.ntsrc > 64 ||
+             (a.src_key != nullptr && slots > 256)) {
     set_error("row_merge: unsupported sizes (nn in [1, 64], at most 1024 candidate slots, "
               "256 with approximate keys)");
     return DRPT_ERR_UNSUPPORTED;
