@@ -33,6 +33,10 @@ import time
 # the CPU baseline's OpenMP threads bound to their cores (read when the
 # OpenMP runtime loads, so before torch and the oracle are imported)
 os.environ.setdefault("OMP_PROC_BIND", "true")
+# CPUs this process may run on, read before any OpenMP runtime binds the main
+# thread to a single core (after that, the main thread's mask lists one CPU)
+START_AFFINITY = (len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+                  else (os.cpu_count() or 1))
 
 import numpy as np
 
@@ -216,7 +220,7 @@ def cpu_baseline(w, seed, repeats=3, budget_s=25.0):
     log(f"[bench] cpu baseline input {X.shape} ready")
     # the CPU share of this process: OMP_NUM_THREADS where the host sets it
     # (the affinity mask can list more CPUs than the process may use)
-    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    aff = START_AFFINITY
     threads = int(os.environ.get("OMP_NUM_THREADS") or aff)
     p = make_params(w, n, X.shape[1], X.dtype == np.uint8, "fp32", seed)
     times = []

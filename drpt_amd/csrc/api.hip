@@ -269,27 +269,35 @@ int drpt_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, cons
     }
   }
   colptr[L] = (int32_t)rowidx.size();
-  // the same terms grouped by 4 columns, interleaved, padded per group to a
-  // multiple of 4 terms with (row d: a zero slot, weight 0)
+  // the same terms grouped by 4 columns of similar non-zero counts (columns
+  // in order of count), interleaved, padded per group to a multiple of 4 terms
+  // with (row d: a zero slot, weight 0); gcol[4g + u] = column of slot u
   const int ngroups = (L + 3) / 4;
-  std::vector<int32_t> gk(ngroups + 1, 0);
+  std::vector<int32_t> order(L), gk(ngroups + 1, 0), gcol(4 * (size_t)ngroups, -1);
+  for (int c = 0; c < L; ++c) order[c] = c;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return colptr[a + 1] - colptr[a] > colptr[b + 1] - colptr[b];
+  });
   ridx32.clear();
   wd.clear();
   for (int g = 0; g < ngroups; ++g) {
     int kmax = 0;
-    for (int u = 0; u < 4 && 4 * g + u < L; ++u)
-      kmax = std::max(kmax, colptr[4 * g + u + 1] - colptr[4 * g + u]);
+    for (int u = 0; u < 4 && 4 * g + u < L; ++u) {
+      const int c = order[4 * g + u];
+      gcol[4 * (size_t)g + u] = c;
+      kmax = std::max(kmax, colptr[c + 1] - colptr[c]);
+    }
     kmax = (kmax + 3) & ~3;
     gk[g + 1] = gk[g] + kmax;
     for (int k = 0; k < kmax; ++k)
       for (int u = 0; u < 4; ++u) {
-        const int c = 4 * g + u;
-        const bool has = c < L && k < colptr[c + 1] - colptr[c];
+        const int c = gcol[4 * (size_t)g + u];
+        const bool has = c >= 0 && k < colptr[c + 1] - colptr[c];
         ridx32.push_back(has ? (int32_t)rowidx[colptr[c] + k] : d);
         wd.push_back(has ? (double)w[colptr[c] + k] : 0.0);
       }
   }
-  int32_t *dcp, *dri32, *dgk;
+  int32_t *dcp, *dri32, *dgk, *dgcol;
   uint16_t* dri;
   float* dw;
   double* dwd;
@@ -297,9 +305,10 @@ int drpt_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, cons
   DRPT_CHECK(upload(ctx, "proj.rowidx", rowidx, &dri));
   DRPT_CHECK(upload(ctx, "proj.w", w, &dw));
   DRPT_CHECK(upload(ctx, "proj.gk", gk, &dgk));
+  DRPT_CHECK(upload(ctx, "proj.gcol", gcol, &dgcol));
   DRPT_CHECK(upload(ctx, "proj.rowidx32", ridx32, &dri32));
   DRPT_CHECK(upload(ctx, "proj.wd", wd, &dwd));
-  return launch_project(ctx, X, dtype, n, d, dcp, dri, dw, dgk, dri32, dwd, L, colptr[L], P,
+  return launch_project(ctx, X, dtype, n, d, dcp, dri, dw, dgk, dgcol, dri32, dwd, L, colptr[L], P,
                         colmax);
 }
 

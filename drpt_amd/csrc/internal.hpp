@@ -93,11 +93,13 @@ int run_graph(drpt_ctx* ctx, const char* name, const std::string& key,
 
 // project.hip
 // (rowidx / w: R's compressed columns as uint16 / float; gk / ridx32 / wd the
-// same terms in groups of 4 columns, interleaved and padded to multiples of 4
-// terms with row d and weight 0, for the wave-uniform kernel; may be null)
+// same terms in groups of 4 columns (columns gcol[4g .. 4g+3], -1: none),
+// interleaved and padded to multiples of 4 terms with row d and weight 0, for
+// the wave-uniform kernel; may be null)
 int launch_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, const int32_t* colptr,
-                   const uint16_t* rowidx, const float* w, const int32_t* gk, const int32_t* ridx32,
-                   const double* wd, int L, int nnz, float* P, uint32_t* colmax);
+                   const uint16_t* rowidx, const float* w, const int32_t* gk, const int32_t* gcol,
+                   const int32_t* ridx32, const double* wd, int L, int nnz, float* P,
+                   uint32_t* colmax);
 
 // split.hip
 // device address of the allocation-free entry points' scratch array on

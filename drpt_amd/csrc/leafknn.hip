@@ -2099,15 +2099,14 @@ k_knn_fallback(const XT* __restrict__ X, int d, const int32_t* __restrict__ perm
   }
 }
 
-// bf16 copy (round to nearest even), |x~|^2 and |x~ - x| (rounded up). The
-// squared norm is reduced in FP32, or in FP64 (rounded once) for d >= 512,
-// where d FP32 additions would loosen the filter bound.
+// bf16 copy (round to nearest even), |x~|^2 and |x~ - x| (rounded up) of one
+// row (the whole wave). The squared norm is reduced in FP32, or in FP64
+// (rounded once) for d >= 512, where d FP32 additions would loosen the filter
+// bound.
 template <typename XT>
-__global__ void k_prep_bf16(const XT* __restrict__ X, int64_t n, int d, uint16_t* __restrict__ Xb,
-                            float2* __restrict__ nbe) {
-  const int lane = threadIdx.x & 63;
-  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (row >= n) return;
+__device__ __forceinline__ void prep_bf16_row(const XT* __restrict__ X, int64_t row, int d, int lane,
+                                              uint16_t* __restrict__ Xb,
+                                              float2* __restrict__ nbe) {
   const bool wide = d >= 512;
   float s = 0.0f;
   double s64 = 0.0, e2 = 0.0;
@@ -2123,17 +2122,32 @@ __global__ void k_prep_bf16(const XT* __restrict__ X, int64_t n, int d, uint16_t
     const double df = (double)xb - (double)x;
     e2 += df * df;
   }
+  if (wide) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    s += __shfl_xor(s, off);
-    s64 += __shfl_xor(s64, off);
-    e2 += __shfl_xor(e2, off);
+    for (int off = 32; off > 0; off >>= 1) s64 += __shfl_xor(s64, off);
+  } else {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
   }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off);
   if (lane == 0) {
     float ef = (float)sqrt(e2);
     if (e2 > 0.0) ef = ef * (1.0f + 1.0e-6f) + 1.0e-30f;
     nbe[row] = make_float2(wide ? (float)s64 : s, ef);
   }
+}
+
+// One wave per row, the waves of the grid striding over the rows (a grid of
+// a few thousand blocks instead of one block per four rows).
+template <typename XT>
+__global__ void k_prep_bf16(const XT* __restrict__ X, int64_t n, int d, uint16_t* __restrict__ Xb,
+                            float2* __restrict__ nbe) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wstride = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; row < n;
+       row += wstride)
+    prep_bf16_row(X, row, d, lane, Xb, nbe);
 }
 
 // FP8 quantisation, one wave per row: scale 2^e with e the smallest exponent
@@ -2279,6 +2293,8 @@ int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, 
   }
   const int64_t threads = n * 64;
   const unsigned blocks = (unsigned)((threads + 255) / 256);
+  // k_prep_bf16 strides over the rows: a few resident rounds of blocks
+  const unsigned sblocks = (unsigned)std::min<int64_t>(blocks, (int64_t)ctx->num_cus * 8);
   if (precision == DRPT_FP8) {
     if (dtype == DRPT_U8)
       hipLaunchKernelGGL(k_prep_fp8<uint8_t>, dim3(blocks), dim3(256), 0, ctx->stream,
@@ -2287,11 +2303,11 @@ int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, 
       hipLaunchKernelGGL(k_prep_fp8<float>, dim3(blocks), dim3(256), 0, ctx->stream,
                          (const float*)X, n, d, (uint8_t*)Xq, nbe, pexp);
   } else if (dtype == DRPT_U8)
-    hipLaunchKernelGGL(k_prep_bf16<uint8_t>, dim3(blocks), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_prep_bf16<uint8_t>, dim3(sblocks), dim3(256), 0, ctx->stream,
                        (const uint8_t*)X, n, d, (uint16_t*)Xq, nbe);
   else
-    hipLaunchKernelGGL(k_prep_bf16<float>, dim3(blocks), dim3(256), 0, ctx->stream, (const float*)X,
-                       n, d, (uint16_t*)Xq, nbe);
+    hipLaunchKernelGGL(k_prep_bf16<float>, dim3(sblocks), dim3(256), 0, ctx->stream,
+                       (const float*)X, n, d, (uint16_t*)Xq, nbe);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }

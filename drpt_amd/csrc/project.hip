@@ -147,15 +147,18 @@ constexpr int kColsPerPass = 4;   // columns per group (independent FMA chains)
 constexpr int kTermsPerBatch = 4; // terms of each column loaded per batch
 
 // R for this kernel: column groups of kColsPerPass, their terms interleaved
-// (term k of column u of group g at gk[g] * 4 + 4 k + u), each group padded
+// (term k of slot u of group g at gk[g] * 4 + 4 k + u), each group padded
 // to a multiple of kTermsPerBatch terms with (row d, weight 0): row d of the
 // tile is a zero slot, and fma(0, 0, acc) = acc exactly (acc starts at +0 and
-// so is never -0).
-template <typename XT>
+// so is never -0). Slot u of group g computes column gcol[4 g + u] (-1: none);
+// the columns are grouped in order of their non-zero counts, so a group's
+// padding to its longest column stays small.
+template <typename XT, int kStage>  // kStage: 16-byte chunks of the next X tile per thread
 __global__ void __launch_bounds__(kThreads)
 k_project_rows(const XT* __restrict__ X, int64_t n, int d, int L, const int32_t* __restrict__ gk,
-               const int32_t* __restrict__ ridx, const double* __restrict__ wd,
-               float* __restrict__ P, uint32_t* __restrict__ colmax, int pitch) {
+               const int32_t* __restrict__ gcol, const int32_t* __restrict__ ridx,
+               const double* __restrict__ wd, float* __restrict__ P,
+               uint32_t* __restrict__ colmax, int pitch) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t* lmax = reinterpret_cast<uint32_t*>(smem);
   XT* tile = reinterpret_cast<XT*>(smem + (((size_t)L * 4 + 15) & ~(size_t)15));
@@ -164,11 +167,44 @@ k_project_rows(const XT* __restrict__ X, int64_t n, int d, int L, const int32_t*
   const int lane = threadIdx.x & 63;
   const int ngroups = (L + kColsPerPass - 1) / kColsPerPass;
   const int64_t ntiles = (n + 63) / 64;
+  // Rows of 16-byte multiples up to kStage * 256 bytes: the next tile of X is
+  // loaded into registers (16-byte loads, all in flight at once) while the
+  // current tile is computed, and written to LDS after it.
+  const int rb = d * (int)sizeof(XT);
+  const bool fast = (rb & 15) == 0 && rb <= kStage * 16 * kThreads / 64 && ((uintptr_t)X & 15) == 0;
+  const int cpr = rb >> 4;  // 16-byte chunks per row
+  uint4 pre[kStage];
+  auto load_tile = [&](int64_t t) {
+    const int64_t r0 = t * 64;
+    const int nq = (int)min((int64_t)64, n - r0) * cpr;
+    const uint4* src = reinterpret_cast<const uint4*>(X + r0 * d);
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      const int q = threadIdx.x + u * kThreads;
+      pre[u] = q < nq ? src[q] : make_uint4(0u, 0u, 0u, 0u);
+    }
+  };
+  if (fast && (int64_t)blockIdx.x < ntiles) load_tile(blockIdx.x);
   for (int64_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
     const int64_t row0 = tl * 64;
     const int rows = (int)min((int64_t)64, n - row0);
     __syncthreads();
-    if (sizeof(XT) == 1 && (d & 3) == 0) {
+    if (fast) {
+      unsigned char* tb = reinterpret_cast<unsigned char*>(tile);
+      const int pb = pitch * (int)sizeof(XT);  // row pitch in bytes (a multiple of 4)
+#pragma unroll
+      for (int u = 0; u < kStage; ++u) {
+        const int q = threadIdx.x + u * kThreads;
+        if (q < rows * cpr) {
+          const int rr = q / cpr, c = q - rr * cpr;
+          uint32_t* dst = reinterpret_cast<uint32_t*>(tb + rr * pb + c * 16);
+          dst[0] = pre[u].x;
+          dst[1] = pre[u].y;
+          dst[2] = pre[u].z;
+          dst[3] = pre[u].w;
+        }
+      }
+    } else if (sizeof(XT) == 1 && (d & 3) == 0) {
       const uint32_t* src = reinterpret_cast<const uint32_t*>(X + row0 * d);
       uint32_t* dst = reinterpret_cast<uint32_t*>(tile);
       const int wpr = d >> 2, wpitch = pitch >> 2, total = rows * wpr;
@@ -186,6 +222,7 @@ k_project_rows(const XT* __restrict__ X, int64_t n, int d, int L, const int32_t*
     }
     if (threadIdx.x < 64) tile[threadIdx.x * pitch + d] = (XT)0;  // the zero slot
     __syncthreads();
+    if (fast && tl + gridDim.x < ntiles) load_tile(tl + gridDim.x);
     const XT* xr = tile + lane * pitch;
     const bool valid = lane < rows;
     for (int g = wave; g < ngroups; g += 4) {
@@ -214,8 +251,8 @@ k_project_rows(const XT* __restrict__ X, int64_t n, int d, int L, const int32_t*
       }
 #pragma unroll
       for (int u = 0; u < kColsPerPass; ++u) {
-        const int c = g * kColsPerPass + u;
-        if (c >= L) break;
+        const int c = gcol[g * kColsPerPass + u];
+        if (c < 0) continue;
         const float pv = (float)acc[u];
         if (valid) P[(int64_t)c * n + row0 + lane] = pv;
         const uint32_t m = wave_max_u32(valid ? (__float_as_uint(pv) & 0x7fffffffu) : 0u);
@@ -296,20 +333,27 @@ int dispatch(drpt_ctx* ctx, const XT* X, int64_t n, int d, const int32_t* colptr
 }  // namespace
 
 template <typename XT>
-int run_rows(drpt_ctx* ctx, const XT* X, int64_t n, int d, const int32_t* gk,
+int run_rows(drpt_ctx* ctx, const XT* X, int64_t n, int d, const int32_t* gk, const int32_t* gcol,
              const int32_t* ridx, const double* wd, int L, float* P, uint32_t* colmax) {
   const int pitch = pitch_rows<XT>(d);
   const size_t lds = (((size_t)L * 4 + 15) & ~(size_t)15) + (size_t)64 * pitch * sizeof(XT);
+  // register staging of the next tile: 8 chunks per thread for rows up to 512
+  // bytes, else 16
+  const bool small = (size_t)d * sizeof(XT) <= 8 * 16 * kThreads / 64;
+  const void* f = small ? (const void*)k_project_rows<XT, 8> : (const void*)k_project_rows<XT, 16>;
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_project_rows<XT>, kThreads, lds) !=
-          hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kThreads, lds) != hipSuccess ||
       per_cu < 1) {
     (void)hipGetLastError();
     per_cu = 2;
   }
   const int64_t blocks = std::min<int64_t>((n + 63) / 64, (int64_t)ctx->num_cus * per_cu);
-  hipLaunchKernelGGL((k_project_rows<XT>), dim3((unsigned)blocks), dim3(kThreads), lds, ctx->stream,
-                     X, n, d, L, gk, ridx, wd, P, colmax, pitch);
+  if (small)
+    hipLaunchKernelGGL((k_project_rows<XT, 8>), dim3((unsigned)blocks), dim3(kThreads), lds,
+                       ctx->stream, X, n, d, L, gk, gcol, ridx, wd, P, colmax, pitch);
+  else
+    hipLaunchKernelGGL((k_project_rows<XT, 16>), dim3((unsigned)blocks), dim3(kThreads), lds,
+                       ctx->stream, X, n, d, L, gk, gcol, ridx, wd, P, colmax, pitch);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }
@@ -321,15 +365,16 @@ bool rows_fit(int d, int L) {
 }
 
 int launch_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, const int32_t* colptr,
-                   const uint16_t* rowidx, const float* w, const int32_t* gk, const int32_t* ridx32,
-                   const double* wd, int L, int nnz, float* P, uint32_t* colmax) {
+                   const uint16_t* rowidx, const float* w, const int32_t* gk, const int32_t* gcol,
+                   const int32_t* ridx32, const double* wd, int L, int nnz, float* P,
+                   uint32_t* colmax) {
   DRPT_CHECK(fill_u32(ctx, colmax, 0u, (size_t)L));
   if (n <= 0) return DRPT_OK;
-  if (gk && ridx32 && wd) {  // 64-row tiles with wave-uniform columns where they fit the LDS
+  if (gk && gcol && ridx32 && wd) {  // 64-row tiles with wave-uniform columns where they fit
     if (dtype == DRPT_U8 && rows_fit<uint8_t>(d, L))
-      return run_rows<uint8_t>(ctx, (const uint8_t*)X, n, d, gk, ridx32, wd, L, P, colmax);
+      return run_rows<uint8_t>(ctx, (const uint8_t*)X, n, d, gk, gcol, ridx32, wd, L, P, colmax);
     if (dtype != DRPT_U8 && rows_fit<float>(d, L))
-      return run_rows<float>(ctx, (const float*)X, n, d, gk, ridx32, wd, L, P, colmax);
+      return run_rows<float>(ctx, (const float*)X, n, d, gk, gcol, ridx32, wd, L, P, colmax);
   }
   if (dtype == DRPT_U8)
     return dispatch<uint8_t>(ctx, (const uint8_t*)X, n, d, colptr, rowidx, w, L, nnz, P, colmax);

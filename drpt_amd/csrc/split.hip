@@ -166,6 +166,98 @@ struct NodeAcc {
   }
 };
 
+// Split decision and mean of one node from its count c and fixed-point sum s1
+// (drpt_global.cpp:242-260 with the canonical sums).
+__device__ __forceinline__ void node_mu(int mode, int leaf_max, int kq1, uint32_t c,
+                                        unsigned long long s1, uint32_t& sp, float& mu) {
+  const bool split = (mode == DRPT_TREE_FIXED) || c > (uint32_t)leaf_max;
+  sp = split ? (kSplitBit | c) : 0u;
+  const double s = ldexp((double)(long long)s1, kq1);
+  mu = (float)(s / (double)c);
+}
+
+// Edge table of a split node (math_operations.cpp:250-288, factor 1,
+// no_of_bins 7): variance from the fixed-point sum s2, then e[0..13] around
+// the mean m; e[14] = e[15] = 0.
+__device__ __forceinline__ void node_edges(uint32_t sp, int kq2, unsigned long long s2q, float m,
+                                           float (&e)[16]) {
+  const long long c = (long long)(sp & ~kSplitBit);
+  const double s2 = ldexp((double)(long long)s2q, kq2);
+  const float var = (float)(s2 / (double)(c - 1));
+  const float sigma = sqrtf(var);
+  const float s1 = (float)((double)sigma / 30.0);
+  const float s2f = (float)((double)sigma / 6.0);
+  const float s3 = (float)((double)sigma / 2.0);
+#pragma unroll
+  for (int j = 1; j <= 4; ++j) {
+    const float r = __fmul_rn((float)j, s1);
+    e[6 + j] = __fadd_rn(m, r);
+    e[7 - j] = __fsub_rn(m, r);
+  }
+#pragma unroll
+  for (int j = 1; j <= 2; ++j) {
+    const float r = __fadd_rn(sigma, __fmul_rn((float)j, s2f));
+    e[10 + j] = __fadd_rn(m, r);
+    e[3 - j] = __fsub_rn(m, r);
+  }
+  {
+    const float r = __fadd_rn(__fmul_rn(2.0f, sigma), s3);
+    e[13] = __fadd_rn(m, r);
+    e[0] = __fsub_rn(m, r);
+  }
+  e[14] = 0.0f;
+  e[15] = 0.0f;
+}
+
+// Median of a split node: cumulative search over its 16-slot histogram h and
+// interpolation in the edge table (edge(j) = E[j]), math_operations.cpp:333-377
+// with E[-1] = 0.
+template <class EdgeAt>
+__device__ __forceinline__ float node_median(uint32_t sp, const uint32_t* h, EdgeAt edge) {
+  const long long c = (long long)(sp & ~kSplitBit);
+  const float val = (float)((50 * c) / 100);
+  float cf = 0.0f;
+  int sel = -1;
+  for (int k = 0; k < 16; ++k) {
+    cf = (float)__dadd_rn((double)cf, (double)h[k]);
+    if (cf >= val) {
+      sel = k;
+      break;
+    }
+  }
+  if (sel < 0) sel = 15;
+  const double count = (double)h[sel];
+  const float lo = (sel == 0) ? 0.0f : edge(sel - 1);
+  const float width = __fsub_rn(edge(sel), lo);
+  const double num = __dsub_rn((double)(c / 2), __dsub_rn((double)cf, count));
+  const double frac = num / count;
+  const double m = __dadd_rn((double)lo, __dmul_rn(frac, (double)width));
+  return (float)m;
+}
+
+// Adds one batch of slots b (lanes where act) to the histograms Hb[nd * stride
+// + slot]: when every active lane holds one node, lane k < 16 adds the
+// population of slot k (wave ballots); otherwise one atomic per lane. Every
+// lane of the wave calls it.
+__device__ __forceinline__ void hist_add(bool act, int nd, int b, uint32_t* Hb, int stride,
+                                         int lane) {
+  const unsigned long long am = __ballot(act);
+  if (am == 0) return;
+  const int leader = __ffsll((long long)am) - 1;
+  const int ndl = __builtin_amdgcn_readlane(nd, leader);
+  if (__ballot(act && nd == ndl) == am) {
+    uint32_t mine = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t c = (uint32_t)__popcll(__ballot(act && b == k));
+      mine = (lane == k) ? c : mine;
+    }
+    if (lane < 16 && mine) atomicAdd(&Hb[(int64_t)ndl * stride + lane], mine);
+  } else if (act) {
+    atomicAdd(&Hb[(int64_t)nd * stride + b], 1u);
+  }
+}
+
 // Count and sum of quantised values of the root (level 0): every point of a
 // tree is in node 1, so each wave adds one run.
 __global__ void __launch_bounds__(kThreads)
@@ -221,11 +313,11 @@ __global__ void k_mu(int T, int l, int D, int mode, int leaf_max, int bitlen,
   int t = (int)(idx / NN);
   int kq1, kq2;
   col_exponents(colmax[(int64_t)t * D + l], bitlen, kq1, kq2);
-  uint32_t c = cnt[idx];
-  const bool sp = (mode == DRPT_TREE_FIXED) || c > (uint32_t)leaf_max;
-  split[idx] = sp ? (kSplitBit | c) : 0u;
-  double s = ldexp((double)(long long)S1[idx], kq1);
-  mu[idx] = (float)(s / (double)c);
+  uint32_t sp;
+  float m;
+  node_mu(mode, leaf_max, kq1, cnt[idx], S1[idx], sp, m);
+  split[idx] = sp;
+  mu[idx] = m;
   S2[idx] = 0ull;
   uint4* h = reinterpret_cast<uint4*>(H + idx * 16);
   const uint4 z = make_uint4(0u, 0u, 0u, 0u);
@@ -324,34 +416,8 @@ __global__ void k_edges(int T, int l, int D, int bitlen, const uint32_t* __restr
   int t = (int)(idx / NN);
   int kq1, kq2;
   col_exponents(colmax[(int64_t)t * D + l], bitlen, kq1, kq2);
-  long long c = (long long)(sp & ~kSplitBit);
-  double s2 = ldexp((double)(long long)S2[idx], kq2);
-  float var = (float)(s2 / (double)(c - 1));
-  float m = mu[idx];
-  float sigma = sqrtf(var);
-  float s1 = (float)((double)sigma / 30.0);
-  float s2f = (float)((double)sigma / 6.0);
-  float s3 = (float)((double)sigma / 2.0);
   float e[16];
-#pragma unroll
-  for (int j = 1; j <= 4; ++j) {
-    float r = __fmul_rn((float)j, s1);
-    e[6 + j] = __fadd_rn(m, r);
-    e[7 - j] = __fsub_rn(m, r);
-  }
-#pragma unroll
-  for (int j = 1; j <= 2; ++j) {
-    float r = __fadd_rn(sigma, __fmul_rn((float)j, s2f));
-    e[10 + j] = __fadd_rn(m, r);
-    e[3 - j] = __fsub_rn(m, r);
-  }
-  {
-    float r = __fadd_rn(__fmul_rn(2.0f, sigma), s3);
-    e[13] = __fadd_rn(m, r);
-    e[0] = __fsub_rn(m, r);
-  }
-  e[14] = 0.0f;
-  e[15] = 0.0f;
+  node_edges(sp, kq2, S2[idx], mu[idx], e);
   float4* dst = reinterpret_cast<float4*>(E + idx * 16);
   dst[0] = make_float4(e[0], e[1], e[2], e[3]);
   dst[1] = make_float4(e[4], e[5], e[6], e[7]);
@@ -451,27 +517,8 @@ k_hist(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n
           }
         }
       }
-      unsigned long long am = __ballot(act);
-      if (am == 0) continue;
-      int leader = __ffsll((long long)am) - 1;
-      int ndl = __builtin_amdgcn_readlane(nd, leader);
-      unsigned long long same = __ballot(act && nd == ndl);
-      if (same == am) {
-        // wave-ballot histogram: lane k < 16 adds the population of slot k
-        uint32_t mine = 0;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-          uint32_t c = (uint32_t)__popcll(__ballot(act && b == k));
-          mine = (lane == k) ? c : mine;
-        }
-        if (lane < 16 && mine) {
-          if (kLds) atomicAdd(&lH[ndl * kHS + lane], mine);
-          else atomicAdd(&gH[(int64_t)ndl * 16 + lane], mine);
-        }
-      } else if (act) {
-        if (kLds) atomicAdd(&lH[nd * kHS + b], 1u);
-        else atomicAdd(&gH[(int64_t)nd * 16 + b], 1u);
-      }
+      if (kLds) hist_add(act, nd, b, lH, kHS, lane);
+      else hist_add(act, nd, b, gH, 16, lane);
     }
   }
   if (kLds) {
@@ -492,27 +539,8 @@ __global__ void k_median(int T, int l, const uint32_t* __restrict__ H,
   if (idx >= (int64_t)T * NN) return;
   const uint32_t sp = split[idx];
   if (!sp) return;
-  long long c = (long long)(sp & ~kSplitBit);
-  const uint32_t* h = H + idx * 16;
   const float* e = E + idx * 16;
-  float val = (float)((50 * c) / 100);
-  float cf = 0.0f;
-  int sel = -1;
-  for (int k = 0; k < 16; ++k) {
-    cf = (float)__dadd_rn((double)cf, (double)h[k]);
-    if (cf >= val) {
-      sel = k;
-      break;
-    }
-  }
-  if (sel < 0) sel = 15;
-  double count = (double)h[sel];
-  float lo = (sel == 0) ? 0.0f : e[sel - 1];
-  float width = __fsub_rn(e[sel], lo);
-  double num = __dsub_rn((double)(c / 2), __dsub_rn((double)cf, count));
-  double frac = num / count;
-  double m = __dadd_rn((double)lo, __dmul_rn(frac, (double)width));
-  med[idx] = (float)m;
+  med[idx] = node_median(sp, H + idx * 16, [&](int j) { return e[j]; });
 }
 
 __global__ void __launch_bounds__(kThreads)
