@@ -79,10 +79,11 @@ def test_grow_trees_bitexact(ctx, orc, case, mode, leaf_max):
         assert np.array_equal(heap[t], ref), f"tree {t}: {(heap[t] != ref).sum()} points differ"
 
 
-@pytest.mark.parametrize("mode,D,leaf_max", [(0, 14, 0), (1, 15, 3)])
+@pytest.mark.parametrize("mode,D,leaf_max", [(0, 14, 0), (1, 15, 3), (1, 15, 64)])
 def test_grow_trees_deep_levels_bitexact(ctx, orc, mode, D, leaf_max):
     """Levels with 1024 - 8192 nodes per tree: per-node sums and histograms in
-    large LDS arrays, then global atomics; the trees equal the oracle's."""
+    large LDS arrays, then global atomics; with leaf_max 64 the deepest levels
+    hold few splitting nodes among many leaves. The trees equal the oracle's."""
     _log(f"[grow deep] mode={mode} D={D}")
     X = datagen.int_mixture(40000)
     n, d = X.shape
@@ -93,6 +94,28 @@ def test_grow_trees_deep_levels_bitexact(ctx, orc, mode, D, leaf_max):
     heap = heap.cpu().numpy().view(np.uint32)
     for t in range(T):
         ref = orc.grow_tree(P0, cm0, D, t, mode, leaf_max, 0)
+        assert np.array_equal(heap[t], ref), f"tree {t}: {(heap[t] != ref).sum()} points differ"
+
+
+@pytest.mark.parametrize("n,T,D,leaf_max,trees", [
+    (1000000, 16, 15, 512, (0, 7, 15)),  # the 1M x 128 build's split
+    (300000, 8, 17, 64, (0, 5)),         # depth 17, levels of up to 32768 nodes
+])
+def test_grow_trees_full_scale(ctx, orc, n, T, D, leaf_max, trees):
+    """Leaf-max trees at the size of the 1M x 128 workload: every tree's
+    assignment equals the oracle's on the trees listed (the others are
+    grown in the same launches)."""
+    _log(f"[grow full] n={n} T={T} D={D} leaf<={leaf_max}")
+    X = datagen.int_mixture(n)
+    d = X.shape[1]
+    R = orc.build_R(d, T * D, 1.0 / np.sqrt(d), 43)
+    P, cm = ctx.project(_dev(X), R)
+    heap = ctx.grow_trees(P, cm, T, D, 1, leaf_max).cpu().numpy().view(np.uint32)
+    Ph = P.cpu().numpy()
+    cmh = cm.cpu().numpy().view(np.uint32)
+    for t in trees:
+        ref = orc.grow_tree(Ph[t * D:(t + 1) * D].copy(), cmh[t * D:(t + 1) * D].copy(), D, 0, 1,
+                            leaf_max, 0)
         assert np.array_equal(heap[t], ref), f"tree {t}: {(heap[t] != ref).sum()} points differ"
 
 
