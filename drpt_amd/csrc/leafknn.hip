@@ -1318,7 +1318,19 @@ __global__ void k_leaf_gather_val(const int32_t* __restrict__ perm, int64_t tota
 // known list; the K best are written. Keys (D bits << 32 | index) order like
 // (D, index) because D >= 0.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t xor_lane(uint32_t v, int j) { return (uint32_t)__shfl_xor((int)v, j); }
+// v of lane ^ j: quad DPP for j = 1, 2 (no LDS trip), a swizzle within each
+// half wave for j = 4, 8, 16, a permute for larger j (j is a constant once the
+// callers' loops are unrolled, so the switch folds away)
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v, int j) {
+  switch (j) {
+    case 1: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+    case 2: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
+    case 4: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (4 << 10));
+    case 8: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (8 << 10));
+    case 16: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (16 << 10));
+    default: return (uint32_t)__shfl_xor((int)v, j);
+  }
+}
 __device__ __forceinline__ uint64_t xor_lane(uint64_t v, int j) {
   const uint32_t lo = xor_lane((uint32_t)v, j), hi = xor_lane((uint32_t)(v >> 32), j);
   return ((uint64_t)hi << 32) | lo;
@@ -1858,7 +1870,10 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge2(const RowMergeA
 // within 2 ulp, stay below 16 2^-24 relative) plus 1e-30. The row's
 // bound is the K-th smallest UB over distinct leafmates (+inf with fewer):
 // at least K distinct leafmates lie within it, so it bounds the final K-th
-// distance and every final neighbour passes the pooled filter. (n_i^2, e_i)
+// distance and every final neighbour passes the pooled filter. With more
+// than 64 candidate slots the bound is the K-th smallest of the lanes' minima
+// (one value per lane, each a distinct candidate): still at least K distinct
+// leafmates within it, from a sort a quarter or half the size. (n_i^2, e_i)
 // of each point come from nbe, the quantisation the keys were computed in.
 template <int E>
 __global__ void __launch_bounds__(64 * kMergeWaves)
@@ -1912,12 +1927,24 @@ k_row_bound(const uint64_t* __restrict__ src_key, int64_t tree_stride, int S, in
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     for (int j = lane; j < kHashSlots; j += 64) tb[j] = 0xFFFFFFFFu;
-    bitonic_wave<uint32_t, E>(ub, lane);
-    // element K - 1 of the sorted 64 E values: register (K - 1) / 64, lane (K - 1) % 64
     uint32_t kth = 0xFFFFFFFFu;
+    if (E == 1 || K > 64) {
+      bitonic_wave<uint32_t, E>(ub, lane);
+      // element K - 1 of the sorted 64 E values: register (K - 1) / 64, lane (K - 1) % 64
 #pragma unroll
-    for (int e = 0; e < E; ++e)
-      if (e == (K - 1) / 64) kth = (uint32_t)__shfl((int)ub[e], (K - 1) % 64);
+      for (int e = 0; e < E; ++e)
+        if (e == (K - 1) / 64) kth = (uint32_t)__shfl((int)ub[e], (K - 1) % 64);
+    } else {
+      // the K-th smallest of the lanes' minima: K lanes hold a value at most
+      // that large, each a distinct candidate, so it is a valid bound (at
+      // least the K-th smallest UB, equal to it unless one lane holds two of
+      // the K smallest); one 64-value sort instead of a 64 E-value one
+      uint32_t m[1] = {ub[0]};
+#pragma unroll
+      for (int e = 1; e < E; ++e) m[0] = ub[e] < m[0] ? ub[e] : m[0];
+      bitonic_wave<uint32_t, 1>(m, lane);
+      kth = (uint32_t)__shfl((int)m[0], K - 1);
+    }
     if (lane == 0) out_bound[row] = kth >= 0x7F800000u ? INFINITY : __uint_as_float(kth);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
