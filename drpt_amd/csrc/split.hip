@@ -33,6 +33,7 @@
 // Roofline: HBM-bound; algorithmic bytes 12 per point per tree per level
 // (4 B value + 4 B node read + 4 B node write).
 #include <algorithm>
+#include <utility>
 #include <cmath>
 #include <mutex>
 
@@ -53,6 +54,8 @@ constexpr int kThreads = 1024;
 // kHistLdsMax and the device's per-workgroup LDS
 constexpr size_t kSumLdsMax = 128 * 1024;
 constexpr size_t kHistLdsMax = 144 * 1024;
+// most node shards an LDS variant reads its points for before global atomics
+constexpr int kMaxShard = 4;
 
 __device__ __forceinline__ void col_exponents(uint32_t cm, int bitlen, int& kq1, int& kq2) {
   int E = (int)((cm >> 23) & 0xffu);
@@ -115,11 +118,37 @@ __device__ __forceinline__ void load_points(const uint32_t* __restrict__ nt,
   }
 }
 
-// Points [b0, b1) of tree blockIdx.y handled by this block.
-__device__ __forceinline__ void block_range(int64_t n, int64_t ppb, int64_t& b0, int64_t& b1) {
-  b0 = (int64_t)blockIdx.x * ppb;
+// Points [b0, b1) of tree blockIdx.y handled by point block pb.
+__device__ __forceinline__ void block_range(int64_t n, int64_t ppb, int64_t& b0, int64_t& b1,
+                                            int64_t pb = -1) {
+  b0 = (pb < 0 ? (int64_t)blockIdx.x : pb) * ppb;
   b1 = min(n, b0 + ppb);
 }
+
+// Node shards of the LDS variants: the level's NN nodes in 2^shl ranges of
+// NS = NN >> shl, block x = point block * 2^shl + shard; a shard's blocks
+// read every point of their range and keep the points of their own nodes, so
+// the per-node LDS arrays hold NS entries (levels whose arrays would not fit
+// read their points 2^shl times instead of using global atomics). kSh false:
+// one shard, with no shard arithmetic or range test in the kernel.
+template <bool kSh>
+struct Shard {
+  int lo, NS;
+  int64_t pb;
+  __device__ __forceinline__ Shard(int l, int shl) {
+    if (kSh) {
+      NS = 1 << (l - shl);
+      lo = (int)(blockIdx.x & ((1u << shl) - 1u)) << (l - shl);
+      pb = (int64_t)(blockIdx.x >> shl);
+    } else {
+      NS = 1 << l;
+      lo = 0;
+      pb = blockIdx.x;
+    }
+  }
+  // node index nd (relative to lo) of a level-l point is in this shard
+  __device__ __forceinline__ bool has(int nd) const { return !kSh || (unsigned)nd < (unsigned)NS; }
+};
 
 // Accumulation of per-node int64 sums (and counts when kCnt) into S / C
 // (LDS or global arrays). A batch whose active lanes all hold one node adds
@@ -329,22 +358,26 @@ __global__ void k_mu(int T, int l, int D, int mode, int leaf_max, int bitlen,
 
 // Sum of quantised fl(v - mu)^2 per split node. kLds: the tree's sums, means
 // and split flags in LDS ([NN] int64 + [NN] float + [NN] u32).
-template <bool kLds>
+template <bool kLds, bool kSh>
 __global__ void __launch_bounds__(kThreads)
 k_sum2(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n, int D, int l,
        const uint32_t* __restrict__ colmax, int bitlen, const float* __restrict__ mu,
-       const uint32_t* __restrict__ split, unsigned long long* __restrict__ S2, int64_t ppb) {
+       const uint32_t* __restrict__ split, unsigned long long* __restrict__ S2, int64_t ppb,
+       int shl) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int t = blockIdx.y;
   const int NN = 1 << l;
-  const float* mg = mu + (int64_t)t * NN;
-  const uint32_t* sg = split + (int64_t)t * NN;
-  unsigned long long* gS = S2 + (int64_t)t * NN;
+  const Shard<kSh> sh(l, shl);
+  const int NS = sh.NS;
+  // this shard's nodes [lo, lo + NS): pointers shifted to them
+  const float* mg = mu + (int64_t)t * NN + sh.lo;
+  const uint32_t* sg = split + (int64_t)t * NN + sh.lo;
+  unsigned long long* gS = S2 + (int64_t)t * NN + sh.lo;
   unsigned long long* lS = reinterpret_cast<unsigned long long*>(smem);
-  float* lm = reinterpret_cast<float*>(lS + (kLds ? NN : 0));
-  uint32_t* ls = reinterpret_cast<uint32_t*>(lm + (kLds ? NN : 0));
+  float* lm = reinterpret_cast<float*>(lS + (kLds ? NS : 0));
+  uint32_t* ls = reinterpret_cast<uint32_t*>(lm + (kLds ? NS : 0));
   if (kLds) {
-    for (int k = threadIdx.x; k < NN; k += kThreads) {
+    for (int k = threadIdx.x; k < NS; k += kThreads) {
       lS[k] = 0;
       lm[k] = mg[k];
       ls[k] = sg[k];
@@ -360,7 +393,7 @@ k_sum2(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n
   const uint32_t* nt = node + (int64_t)t * n;
   NodeAcc<false> acc(kLds ? lS : gS, nullptr);
   int64_t b0, b1;
-  block_range(n, ppb, b0, b1);
+  block_range(n, ppb, b0, b1, sh.pb);
   for (int64_t base = b0; base < b1; base += kThreads * kUnroll) {
     uint32_t hu[kUnroll];
     float pu[kUnroll];
@@ -374,7 +407,8 @@ k_sum2(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n
       if (i < b1) {
         uint32_t h = hu[u];
         act = (h >> l) == 1u;
-        nd = (int)(h - (uint32_t)NN);
+        nd = (int)(h - (uint32_t)NN) - sh.lo;
+        act = act && sh.has(nd);
         if (act) act = st[nd] != 0u;
         if (act) {
           float w = __fsub_rn(pu[u], mt[nd]);
@@ -388,7 +422,7 @@ k_sum2(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n
   acc.flush();
   if (kLds) {
     __syncthreads();
-    for (int k = threadIdx.x; k < NN; k += kThreads)
+    for (int k = threadIdx.x; k < NS; k += kThreads)
       if (lS[k]) atomicAdd(&gS[k], lS[k]);
   }
 }
@@ -456,27 +490,30 @@ __device__ __forceinline__ int slot_bsearch(float v, const float* e) {
 constexpr int kHS = 17;
 constexpr int kES = 15;
 
-template <bool kLds, bool kEL>
+template <bool kLds, bool kEL, bool kSh>
 __global__ void __launch_bounds__(kThreads)
 k_hist(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n, int D, int l,
        const float* __restrict__ E, const uint32_t* __restrict__ split, uint32_t* __restrict__ H,
-       int64_t ppb) {
-  // kLds: histograms [NN][kHS] in LDS; kEL: also the tree's edges [NN][kES]
-  // and split flags [NN] (staged once per block, read per point)
+       int64_t ppb, int shl) {
+  // kLds: histograms [NS][kHS] in LDS; kEL: also the shard's edges [NS][kES]
+  // and split flags [NS] (staged once per block, read per point)
   extern __shared__ __align__(16) unsigned char smem[];
   const int t = blockIdx.y;
   const int NN = 1 << l;
+  const Shard<kSh> sh(l, shl);
+  const int NS = sh.NS;
   uint32_t* lH = reinterpret_cast<uint32_t*>(smem);
-  float* lE = reinterpret_cast<float*>(lH + (kLds ? NN * kHS : 0));
-  uint32_t* lS = reinterpret_cast<uint32_t*>(lE + (kEL ? NN * kES : 0));
-  const float* Eg = E + (int64_t)t * NN * 16;
-  const uint32_t* Sg = split + (int64_t)t * NN;
+  float* lE = reinterpret_cast<float*>(lH + (kLds ? NS * kHS : 0));
+  uint32_t* lS = reinterpret_cast<uint32_t*>(lE + (kEL ? NS * kES : 0));
+  // this shard's nodes [lo, lo + NS): pointers shifted to them
+  const float* Eg = E + ((int64_t)t * NN + sh.lo) * 16;
+  const uint32_t* Sg = split + (int64_t)t * NN + sh.lo;
   if (kLds) {
-    for (int k = threadIdx.x; k < NN * kHS; k += kThreads) lH[k] = 0;
+    for (int k = threadIdx.x; k < NS * kHS; k += kThreads) lH[k] = 0;
     if (kEL) {
-      for (int k = threadIdx.x; k < NN * 16; k += kThreads)
+      for (int k = threadIdx.x; k < NS * 16; k += kThreads)
         if ((k & 15) < 14) lE[(k >> 4) * kES + (k & 15)] = Eg[k];
-      for (int k = threadIdx.x; k < NN; k += kThreads) lS[k] = Sg[k];
+      for (int k = threadIdx.x; k < NS; k += kThreads) lS[k] = Sg[k];
     }
     __syncthreads();
   }
@@ -484,9 +521,9 @@ k_hist(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n
   const float* Pc = P + col * n;
   const uint32_t* nt = node + (int64_t)t * n;
   const uint32_t* st = kEL ? lS : Sg;
-  uint32_t* gH = H + (int64_t)t * NN * 16;
+  uint32_t* gH = H + ((int64_t)t * NN + sh.lo) * 16;
   int64_t b0, b1;
-  block_range(n, ppb, b0, b1);
+  block_range(n, ppb, b0, b1, sh.pb);
   const int lane = threadIdx.x & 63;
   for (int64_t base = b0; base < b1; base += kThreads * kUnroll) {
     uint32_t hu[kUnroll];
@@ -500,7 +537,8 @@ k_hist(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n
       if (i < b1) {
         uint32_t h = hu[u];
         act = (h >> l) == 1u;
-        nd = (int)(h - (uint32_t)NN);
+        nd = (int)(h - (uint32_t)NN) - sh.lo;
+        act = act && sh.has(nd);
         if (act) act = st[nd] != 0u;
         if (act) {
           if (kEL) {
@@ -523,7 +561,7 @@ k_hist(const float* __restrict__ P, const uint32_t* __restrict__ node, int64_t n
   }
   if (kLds) {
     __syncthreads();
-    for (int k = threadIdx.x; k < NN * 16; k += kThreads) {
+    for (int k = threadIdx.x; k < NS * 16; k += kThreads) {
       const uint32_t v = lH[(k >> 4) * kHS + (k & 15)];
       if (v) atomicAdd(&gH[k], v);
     }
@@ -578,29 +616,32 @@ k_part(const float* __restrict__ P, uint32_t* __restrict__ node, int64_t n, int 
 // unsplit nodes are leaves and take no further part). kLds: the child sums
 // ([2NN] int64 sums + [2NN] counts) and the tree's medians and split flags
 // ([NN] float + [NN] u32) in LDS.
-template <bool kLds>
+template <bool kLds, bool kSh>
 __global__ void __launch_bounds__(kThreads)
 k_part_sum1(const float* __restrict__ P, uint32_t* __restrict__ node, int64_t n, int D, int l,
             const float* __restrict__ med, const uint32_t* __restrict__ split,
             const uint32_t* __restrict__ colmax, int bitlen, uint32_t* __restrict__ cnt,
-            unsigned long long* __restrict__ S1, int64_t ppb) {
+            unsigned long long* __restrict__ S1, int64_t ppb, int shl) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int t = blockIdx.y;
   const int NN = 1 << l, NC = 2 << l;
-  const float* mg = med + (int64_t)t * NN;
-  const uint32_t* sg = split + (int64_t)t * NN;
-  uint32_t* gc = cnt + (int64_t)t * NC;
-  unsigned long long* gS = S1 + (int64_t)t * NC;
+  const Shard<kSh> sh(l, shl);
+  const int NS = sh.NS, NCS = 2 * NS;
+  // this shard's parents [lo, lo + NS) and their children [2 lo, 2 lo + 2 NS)
+  const float* mg = med + (int64_t)t * NN + sh.lo;
+  const uint32_t* sg = split + (int64_t)t * NN + sh.lo;
+  uint32_t* gc = cnt + (int64_t)t * NC + 2 * sh.lo;
+  unsigned long long* gS = S1 + (int64_t)t * NC + 2 * sh.lo;
   unsigned long long* lS = reinterpret_cast<unsigned long long*>(smem);
-  uint32_t* lc = reinterpret_cast<uint32_t*>(lS + (kLds ? NC : 0));
-  float* lm = reinterpret_cast<float*>(lc + (kLds ? NC : 0));
-  uint32_t* ls = reinterpret_cast<uint32_t*>(lm + (kLds ? NN : 0));
+  uint32_t* lc = reinterpret_cast<uint32_t*>(lS + (kLds ? NCS : 0));
+  float* lm = reinterpret_cast<float*>(lc + (kLds ? NCS : 0));
+  uint32_t* ls = reinterpret_cast<uint32_t*>(lm + (kLds ? NS : 0));
   if (kLds) {
-    for (int k = threadIdx.x; k < NC; k += kThreads) {
+    for (int k = threadIdx.x; k < NCS; k += kThreads) {
       lS[k] = 0;
       lc[k] = 0;
     }
-    for (int k = threadIdx.x; k < NN; k += kThreads) {
+    for (int k = threadIdx.x; k < NS; k += kThreads) {
       lm[k] = mg[k];
       ls[k] = sg[k];
     }
@@ -615,7 +656,7 @@ k_part_sum1(const float* __restrict__ P, uint32_t* __restrict__ node, int64_t n,
   uint32_t* nt = node + (int64_t)t * n;
   NodeAcc<true> acc(kLds ? lS : gS, kLds ? lc : gc);
   int64_t b0, b1;
-  block_range(n, ppb, b0, b1);
+  block_range(n, ppb, b0, b1, sh.pb);
   for (int64_t base = b0; base < b1; base += kThreads * kUnroll) {
     uint32_t hu[kUnroll];
     float pu[kUnroll], pn[kUnroll];
@@ -634,13 +675,13 @@ k_part_sum1(const float* __restrict__ P, uint32_t* __restrict__ node, int64_t n,
       long long q = 0;
       if (i < b1) {
         const uint32_t h = hu[u];
-        if ((h >> l) == 1u) {
-          const int np = (int)(h - (uint32_t)NN);
+        const int np = (int)(h - (uint32_t)NN) - sh.lo;
+        if ((h >> l) == 1u && sh.has(np)) {
           if (st[np]) {
             const uint32_t hc = 2u * h + ((pu[u] <= mt[np]) ? 0u : 1u);
             nt[i] = hc;
             act = true;
-            nd = (int)(hc - (uint32_t)NC);
+            nd = (int)(hc - (uint32_t)NC) - 2 * sh.lo;
             q = quant(pn[u], kq1);
           }
         }
@@ -651,7 +692,7 @@ k_part_sum1(const float* __restrict__ P, uint32_t* __restrict__ node, int64_t n,
   acc.flush();
   if (kLds) {
     __syncthreads();
-    for (int k = threadIdx.x; k < NC; k += kThreads) {
+    for (int k = threadIdx.x; k < NCS; k += kThreads) {
       if (lc[k]) {
         atomicAdd(&gc[k], lc[k]);
         atomicAdd(&gS[k], lS[k]);
@@ -702,14 +743,20 @@ static void raise_split_lds(drpt_ctx* ctx) {
   if (!ctx->split_lds_big && ctx->max_lds > 64 * 1024) {
     const int bytes = (int)std::min<size_t>((size_t)ctx->max_lds, kSumLdsMax);
     const int hbytes = (int)std::min<size_t>((size_t)ctx->max_lds, kHistLdsMax);
-    if (hipFuncSetAttribute((const void*)k_sum2<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            bytes) == hipSuccess &&
-        hipFuncSetAttribute((const void*)k_part_sum1<true>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess &&
-        hipFuncSetAttribute((const void*)k_hist<true, false>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, hbytes) == hipSuccess &&
-        hipFuncSetAttribute((const void*)k_hist<true, true>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, hbytes) == hipSuccess)
+    // every LDS variant, sharded or not
+    const std::pair<const void*, int> fns[] = {
+        {(const void*)k_sum2<true, false>, bytes},
+        {(const void*)k_sum2<true, true>, bytes},
+        {(const void*)k_part_sum1<true, false>, bytes},
+        {(const void*)k_part_sum1<true, true>, bytes},
+        {(const void*)k_hist<true, false, false>, hbytes},
+        {(const void*)k_hist<true, false, true>, hbytes},
+        {(const void*)k_hist<true, true, false>, hbytes}};
+    bool ok = true;
+    for (const auto& f : fns)
+      ok = ok && hipFuncSetAttribute(f.first, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     f.second) == hipSuccess;
+    if (ok)
       ctx->split_lds_big = true;
     else
       (void)hipGetLastError();
@@ -785,47 +832,73 @@ int split_levels(drpt_ctx* ctx, const float* P, const uint32_t* colmax, int64_t 
     hipLaunchKernelGGL(k_mu, dim3(ngrid), dim3(256), 0, s, T, l, D, mode, leaf_max, bitlen, colmax,
                        cnt, S1, mu, split, S2, H);
     DRPT_HIP(hipGetLastError());
-    const size_t sum2_lds = (size_t)NN * 16;
-    if (sum2_lds <= lds_max) {
-      const int64_t ppb = std::max<int64_t>(8192, NN * 8);
-      hipLaunchKernelGGL(k_sum2<true>, grid_of(ppb), dim3(kThreads), sum2_lds, s, P, node, n, D, l,
-                         colmax, bitlen, mu, split, S2, ppb);
+    // LDS variants: the smallest node shard count 2^shl (up to kMaxShard)
+    // whose per-shard arrays fit; beyond it, global atomics. Shards re-read
+    // every point, which pays while most points are still active: at every
+    // level of fixed-depth trees, and in leaf-max trees while an average node
+    // holds more than leaf_max points (deeper, most points sit in leaves and
+    // the few atomics of the active ones cost less than the extra reads).
+    const bool busy = mode == DRPT_TREE_FIXED || ((int64_t)leaf_max << l) < n;
+    auto shards = [&](size_t bytes_per_node, size_t cap) -> int {
+      for (int shl = 0; (1 << shl) <= (busy ? kMaxShard : 1) && shl <= l; ++shl)
+        if ((size_t)(NN >> shl) * bytes_per_node <= cap) return shl;
+      return -1;
+    };
+    auto sgrid = [&](int64_t ppb, int shl) {
+      return dim3((unsigned)(((n + ppb - 1) / ppb) << shl), (unsigned)T);
+    };
+    const int sh2 = shards(16, lds_max);
+    if (sh2 >= 0) {
+      const int64_t NS = NN >> sh2;
+      const int64_t ppb = std::max<int64_t>(8192, NS * 8);
+      auto* f = sh2 ? k_sum2<true, true> : k_sum2<true, false>;
+      hipLaunchKernelGGL(f, sgrid(ppb, sh2), dim3(kThreads), (size_t)NS * 16, s, P, node, n, D, l,
+                         colmax, bitlen, mu, split, S2, ppb, sh2);
     } else {
-      hipLaunchKernelGGL(k_sum2<false>, grid_of(8192), dim3(kThreads), 0, s, P, node, n, D, l,
-                         colmax, bitlen, mu, split, S2, (int64_t)8192);
+      hipLaunchKernelGGL((k_sum2<false, false>), grid_of(8192), dim3(kThreads), 0, s, P, node, n, D,
+                         l, colmax, bitlen, mu, split, S2, (int64_t)8192, 0);
     }
     DRPT_HIP(hipGetLastError());
     // k_edges also clears the next level's counts and sums
     hipLaunchKernelGGL(k_edges, dim3(ngrid), dim3(256), 0, s, T, l, D, bitlen, colmax, S2, mu,
                        split, E, last ? 0 : 1, cnt, S1);
     DRPT_HIP(hipGetLastError());
-    const size_t hist_lds = (size_t)NN * kHS * 4, edge_lds = (size_t)NN * (kES * 4 + 4);
-    const bool lds_hist = hist_lds <= hist_max;
-    const bool lds_edges = lds_hist && hist_lds + edge_lds <= hist_max;
-    const int64_t ppb_hist = lds_hist ? std::max<int64_t>(8192, NN * 32) : 8192;
-    if (lds_edges)
-      hipLaunchKernelGGL((k_hist<true, true>), grid_of(ppb_hist), dim3(kThreads),
-                         hist_lds + edge_lds, s, P, node, n, D, l, E, split, H, ppb_hist);
-    else if (lds_hist)
-      hipLaunchKernelGGL((k_hist<true, false>), grid_of(ppb_hist), dim3(kThreads), hist_lds, s, P,
-                         node, n, D, l, E, split, H, ppb_hist);
-    else
-      hipLaunchKernelGGL((k_hist<false, false>), grid_of(ppb_hist), dim3(kThreads), 0, s, P, node,
-                         n, D, l, E, split, H, ppb_hist);
+    // histograms: edges staged as well where they fit unsharded; else the
+    // histograms alone, sharded if need be
+    const size_t hist_node = (size_t)kHS * 4, edge_node = (size_t)kES * 4 + 4;
+    const int shh = shards(hist_node, hist_max);
+    const bool lds_edges = shh == 0 && (size_t)NN * (hist_node + edge_node) <= hist_max;
+    if (lds_edges) {
+      const int64_t ppb = std::max<int64_t>(8192, NN * 32);
+      hipLaunchKernelGGL((k_hist<true, true, false>), grid_of(ppb), dim3(kThreads),
+                         (size_t)NN * (hist_node + edge_node), s, P, node, n, D, l, E, split, H,
+                         ppb, 0);
+    } else if (shh >= 0) {
+      const int64_t NS = NN >> shh;
+      const int64_t ppb = std::max<int64_t>(8192, NS * 32);
+      auto* f = shh ? k_hist<true, false, true> : k_hist<true, false, false>;
+      hipLaunchKernelGGL(f, sgrid(ppb, shh), dim3(kThreads), (size_t)NS * hist_node, s, P, node, n,
+                         D, l, E, split, H, ppb, shh);
+    } else {
+      hipLaunchKernelGGL((k_hist<false, false, false>), grid_of(8192), dim3(kThreads), 0, s, P, node,
+                         n, D, l, E, split, H, (int64_t)8192, 0);
+    }
     DRPT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_median, dim3(ngrid), dim3(256), 0, s, T, l, H, E, split, med);
     DRPT_HIP(hipGetLastError());
     if (!last) {
-      // partition fused with the children's counts and sums
-      const int64_t NC = 2 * NN;
-      const size_t part_lds = (size_t)NC * 12 + (size_t)NN * 8;
-      if (part_lds <= lds_max) {
-        const int64_t ppc = std::max<int64_t>(8192, NC * 8);
-        hipLaunchKernelGGL(k_part_sum1<true>, grid_of(ppc), dim3(kThreads), part_lds, s, P, node, n,
-                           D, l, med, split, colmax, bitlen, cnt, S1, ppc);
+      // partition fused with the children's counts and sums (per parent node:
+      // two children's count + int64 sum, the median and the split flag)
+      const int shp = shards(32, lds_max);
+      if (shp >= 0) {
+        const int64_t NS = NN >> shp;
+        const int64_t ppc = std::max<int64_t>(8192, 2 * NS * 8);
+        auto* f = shp ? k_part_sum1<true, true> : k_part_sum1<true, false>;
+        hipLaunchKernelGGL(f, sgrid(ppc, shp), dim3(kThreads), (size_t)NS * 32, s, P, node, n, D,
+                           l, med, split, colmax, bitlen, cnt, S1, ppc, shp);
       } else {
-        hipLaunchKernelGGL(k_part_sum1<false>, grid_of(8192), dim3(kThreads), 0, s, P, node, n, D,
-                           l, med, split, colmax, bitlen, cnt, S1, (int64_t)8192);
+        hipLaunchKernelGGL((k_part_sum1<false, false>), grid_of(8192), dim3(kThreads), 0, s, P, node,
+                           n, D, l, med, split, colmax, bitlen, cnt, S1, (int64_t)8192, 0);
       }
       DRPT_HIP(hipGetLastError());
     } else {
