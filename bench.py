@@ -245,6 +245,7 @@ def cpu_baseline(w, seed, repeats=3, budget_s=25.0):
 HBM_PEAK = 8.0e12   # bytes/s
 MFMA_BF16 = 2.5e15  # flop/s, dense
 MFMA_FP8 = 5.0e15
+MFMA_I8 = 5.0e15    # op/s, dense
 FP32_PEAK = 157.3e12
 
 
@@ -275,8 +276,10 @@ def phase_rooflines(p, d, L, xbytes, tm, ph, prec):
         kp = p.kprime if p.kprime else min(64, 2 * K + 4)
         add("prep", ph["prep_ms"], n * d * (xbytes + 2 + (1 if prec == "fp8" else 0)) + 8.0 * n
             + 12.0 * T * n * (2 if prec == "fp8" else 1), HBM_PEAK, "B")
-        add("candidate_pass", ph["cand_ms"], gram * first / T, MFMA_FP8 if prec == "fp8" else MFMA_BF16,
-            "flop")
+        # integer data in [0, 255]: the LDS-resident passes' int8 Gram
+        i8 = bool(ph.get("int8_passes"))
+        gpeak, gunit = (MFMA_I8, "op") if i8 else (MFMA_FP8 if prec == "fp8" else MFMA_BF16, "flop")
+        add("candidate_pass", ph["cand_ms"], gram * first / T, gpeak, gunit)
         if ph["rerank"] < 0:
             # first bounds from approximate distances: the keys, one (|x~|^2, e) gather per slot
             add("first_bound", ph["known_ms"], n * first * kp * 16.0, HBM_PEAK, "B")
@@ -284,7 +287,7 @@ def phase_rooflines(p, d, L, xbytes, tm, ph, prec):
             add("known_merge", ph["known_ms"],
                 n * (ph["rerank"] * d * xbytes + first * kp * 8.0 + 8.0 * K), HBM_PEAK, "B")
         add("bound_gather", ph["bound_ms"], 12.0 * T * n, HBM_PEAK, "B")
-        add("pool_pass", ph["pool_ms"], gram, MFMA_BF16, "flop")
+        add("pool_pass", ph["pool_ms"], gram, gpeak, gunit)
         add("pool_merge", ph["pool_merge_ms"], n * (16.0 * K * T + 4.0 * K), HBM_PEAK, "B")
     ideal = sum(v["ideal_ms"] for v in out.values())
     return out, ideal / tm["total_ms"] if tm["total_ms"] > 0 else None
@@ -386,9 +389,11 @@ def main():
     tmean = dict(tm_last)
     tmean.update(phases)
     rl_phases, sum_frac = phase_rooflines(p, d, L, xbytes, tmean, stages, prec)
+    # integer data in [0, 255]: the candidate and pool Grams ran on int8 rows
+    int8 = prec != "fp32" and bool(stages.get("int8_passes"))
     res = {"metric": METRIC, "value": n / (ms / 1e3), "unit": "points/s", "n_gpus": 1,
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": prec,
+           "scaling": "weak", "vs_baseline": None, "dtype": "int8" if int8 else prec,
            "data": "synthetic (seeded generator, see drpt_amd/datagen.py)",
            "config": {"workload": w["desc"], "n": n, "d": d, "ntrees": p.ntrees, "nn": p.nn,
                       "depth": p.depth, "leaf_max": p.leaf_max, "precision": prec,
@@ -466,11 +471,12 @@ def main():
         # leaf-resident kernel and, for leaves larger than its LDS, the streaming one)
         fl = flops
         kernel = "k_pool_leaf + k_knn_lowp (pool mode)"
-        peak = MFMA_BF16
+        peak = MFMA_I8 if int8 else MFMA_BF16
         res["pool_launches"] = stages.get("tree_groups")
     achieved = fl / (kms / 1e3)
     res["roofline"] = {"bound": "mfma" if prec != "fp32" else "fp32", "kernel": kernel,
-                       "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                       "achieved": achieved / 1e12, "peak": peak / 1e12,
+                       "unit": "TOP/s" if int8 else "TFLOP/s",
                        "frac": achieved / peak, "traffic": pmc_traffic(a.config, prec, kernel),
                        "flops_per_launch": fl, "kernel_ms": kms}
     if not a.no_cpu:

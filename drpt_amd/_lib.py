@@ -73,7 +73,8 @@ class Phases(ctypes.Structure):
         ("pool_merge_ms", ctypes.c_double), ("bound_ms", ctypes.c_double),
         ("exact_ms", ctypes.c_double), ("tree_groups", ctypes.c_int64),
         ("first_trees", ctypes.c_int64), ("pool_cap", ctypes.c_int64), ("rerank", ctypes.c_int64),
-        ("pool_entries", ctypes.c_int64), ("reserved", ctypes.c_int64 * 3),
+        ("pool_entries", ctypes.c_int64), ("int8_passes", ctypes.c_int64),
+        ("reserved", ctypes.c_int64 * 2),
     ]
 
     def as_dict(self):

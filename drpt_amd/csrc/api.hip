@@ -158,6 +158,7 @@ int drpt_create(drpt_ctx** out, int device) {
       lds > 0)
     ctx->max_lds = lds;
   if (const char* g = getenv("DRPT_GRAPHS")) ctx->use_graphs = strcmp(g, "0") != 0;
+  if (const char* g = getenv("DRPT_INT8")) ctx->use_int8 = strcmp(g, "0") != 0;
   if (const char* b = getenv("DRPT_ROW_EXACT_BATCH")) ctx->row_exact_batch = std::atoll(b);
   if (hipStreamCreate(&ctx->stream) != hipSuccess) {
     delete ctx;
@@ -590,7 +591,7 @@ static int leaf_stats(drpt_ctx* ctx, const drpt_params* p, const Grouping& G, dr
 // quantised copy (bf16, or fp8 e4m3 with a power-of-two scale per row),
 // |x~|^2 and |x~ - x| per point, then their leaf-ordered copies
 static int prep_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
-                     KnnArgs* a) {
+                     KnnArgs* a, uint8_t* xi8 = nullptr, int32_t* notu8 = nullptr) {
   const int64_t n = p->n;
   const bool fp8 = p->precision == DRPT_FP8;
   const std::string tag = fp8 ? "lowp8." : "lowp.";  // both may be live in one build
@@ -601,7 +602,7 @@ static int prep_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X, const i
   DRPT_CHECK(ws(ctx, (tag + "x").c_str(), (size_t)n * p->d * (fp8 ? 1 : 2), &xq));
   DRPT_CHECK(ws(ctx, (tag + "nbe").c_str(), n, &nbe));
   if (fp8) DRPT_CHECK(ws(ctx, (tag + "pexp").c_str(), n, &pexp));
-  DRPT_CHECK(launch_prep_lowp(ctx, X, p->dtype, n, p->d, p->precision, xq, nbe, pexp));
+  DRPT_CHECK(launch_prep_lowp(ctx, X, p->dtype, n, p->d, p->precision, xq, nbe, pexp, xi8, notu8));
   DRPT_CHECK(ws(ctx, (tag + "lnbT").c_str(), (size_t)p->ntrees * n, &lnbT));
   DRPT_CHECK(ws(ctx, (tag + "lerrT").c_str(), (size_t)p->ntrees * n, &lerrT));
   if (fp8) DRPT_CHECK(ws(ctx, (tag + "lexpT").c_str(), (size_t)p->ntrees * n, &lexpT));
@@ -744,6 +745,10 @@ struct LowpRun {
   const void* Xm = nullptr;
   const int32_t* xpos = nullptr;
   int32_t* lossy = nullptr;  // device flag: some row differs from its bf16 copy
+  // float X: its int8 copy (x - 128) and a device flag, some value not an
+  // integer in [0, 255]
+  uint8_t* xi8 = nullptr;
+  int32_t* notu8 = nullptr;
 };
 
 static std::string plan_key(const drpt_params* p) {
@@ -813,7 +818,16 @@ static int lowp_setup(drpt_ctx* ctx, const drpt_params* p, const void* X,
     r->xpos = pos;
   }
   DRPT_HIP(hipEventRecord(ctx->evp[0], ctx->stream));
-  DRPT_CHECK(prep_lowp(ctx, &r->pp, X, r->G.perm, &r->a));
+  if (p->dtype == DRPT_F32) {
+    DRPT_CHECK(ws(ctx, "lowp.xi8", (size_t)n * p->d, &r->xi8));
+    DRPT_CHECK(ws(ctx, "lowp.notu8", 1, &r->notu8));
+    DRPT_CHECK(fill_u32(ctx, (uint32_t*)r->notu8, 0u, 1));
+  }
+  DRPT_CHECK(prep_lowp(ctx, &r->pp, X, r->G.perm, &r->a, r->xi8, r->notu8));
+  if (ctx->use_int8) {  // the LDS-resident passes' int8 instances
+    r->a.Xi8 = r->xi8;
+    r->a.notu8 = r->notu8;
+  }
   {
     float2* nbe;
     DRPT_CHECK(ws(ctx, "lowp.nbe", (size_t)n, &nbe));
@@ -862,6 +876,7 @@ static int lowp_pass(drpt_ctx* ctx, LowpRun* r, int mode, const ItemPlan& plan,
   const drpt_params* pr = mode == kModeCand ? &r->pc : &r->pp;
   a.items = r->G.items;
   a.range = range;
+  a.item_cap = r->G.item_cap;
   // leaves of sizes [lo, hi] with Gram pipeline `ring`: the LDS-resident pool
   // pass (ring 2) takes those it can stage, the streaming kernel the rest
   // (candidate pass: BF16 rows and k' in {12, 16, 24} only)
@@ -1008,6 +1023,8 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
     f.xpos = r.xpos;
     f.Xb = static_cast<const uint16_t*>(r.a.Xq);  // the pooled passes' bf16 copy
     f.lossy = r.lossy;
+    f.Xi8 = r.xi8;
+    f.notu8 = r.notu8;
     f.n = n;
     f.d = p->d;
     f.K = K;
@@ -1030,10 +1047,11 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
     kin = f.out_idx;
     kdn = f.out_dist;
   }
-  int32_t nov = 0;
+  int32_t nov = 0, notu8 = 1;
   unsigned long long npool = 0;
   DRPT_HIP(hipMemcpyAsync(&nov, r.novf, 4, hipMemcpyDeviceToHost, s));
   DRPT_HIP(hipMemcpyAsync(&npool, pool_sum, 8, hipMemcpyDeviceToHost, s));
+  if (r.a.notu8 != nullptr) DRPT_HIP(hipMemcpyAsync(&notu8, r.notu8, 4, hipMemcpyDeviceToHost, s));
   DRPT_HIP(hipStreamSynchronize(s));
   if (nov > 0)
     DRPT_CHECK(launch_row_exact(ctx, &r.pp, X, G.perm, G.items, G.range + 4, r.olist, r.novf, nov,
@@ -1057,6 +1075,10 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   ph.pool_cap = r.pool_cap;
   ph.rerank = plan.rerank < 0 ? -1 : plan.rerank > 0 ? std::max(K, std::min(plan.rerank, 2 * K)) : 2 * K;
   ph.pool_entries = (int64_t)npool;
+  bool lds_pass = false;
+  for (int b = 0; b < kSizeBuckets; ++b) lds_pass = lds_pass || plan.ring[b] == 2;
+  ph.int8_passes = notu8 == 0 && lds_pass && pool_leaf_int8(p->d) &&
+                   pool_leaf_rows(ctx, p->d, kModePool) > 0;
   if (timings) {
     timings->group_ms = ms_between(ctx->ev[6], ctx->ev[7]);
     timings->knn_ms = ms_between(ctx->ev[7], ctx->ev[5]);

@@ -53,10 +53,14 @@ struct drpt_ctx {
   };
   std::map<std::string, Graph> graphs;
   bool use_graphs = true;  // environment DRPT_GRAPHS=0: launch eagerly
+  // environment DRPT_INT8=0: the LDS-resident passes keep their bf16
+  // instances for integer data in [0, 255] as well
+  bool use_int8 = true;
   // environment DRPT_ROW_EXACT_BATCH=r: at most r rows per batch of the exact
   // row path (its scratch is otherwise capped by bytes)
   int64_t row_exact_batch = 0;
   unsigned long long pool_leaf_lds_set = 0;  // LDS-resident pass instances cleared for their dynamic LDS
+  std::map<const void*, int> fn_lds;         // dynamic LDS bytes a kernel has been cleared for
   // work-item plans found by drpt_autotune, by workload shape (ItemPlan)
   std::map<std::string, std::vector<int>> plans;
 };
@@ -190,9 +194,13 @@ int launch_apply_local(drpt_ctx* ctx, const int32_t* perm_in, int32_t* perm_out,
                        const int32_t* seg_size, int nseg, const int32_t* tables);
 
 // quantised rows (bf16: n x d halves; fp8: n x d bytes with a power-of-two
-// scale 2^pexp[i] per row), nbe[i] = (|x~|^2, |x~ - x|) of the dequantised rows
+// scale 2^pexp[i] per row), nbe[i] = (|x~|^2, |x~ - x|) of the dequantised rows;
+// bf16 of float X with notu8 given (zeroed by the caller): also the int8 copy
+// xi8 = x - 128 (n x d bytes) and *notu8 = 1 when some value is not an integer
+// in [0, 255]
 int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, int precision,
-                     void* Xq, float2* nbe, int8_t* pexp);
+                     void* Xq, float2* nbe, int8_t* pexp, uint8_t* xi8 = nullptr,
+                     int32_t* notu8 = nullptr);
 // *lossy = 1 when some row's quantisation error (nbe[i].y) is not 0 (the caller zeroes it)
 int launch_lossy_flag(drpt_ctx* ctx, const float2* nbe, int64_t n, int32_t* lossy);
 // leaf-ordered |x~|^2, |x~ - x| and (fp8) scale exponents: (lnbT[i], lerrT[i])
@@ -240,6 +248,12 @@ struct KnnArgs {
   int32_t* ovf_flag;     // [n]
   int32_t* ovf_list;     // rows whose pool overflowed
   int32_t* novf;
+  // optional int8 copy of X (x - 128, n x d) and a device flag, 0 when every
+  // value of X is an integer in [0, 255]: the LDS-resident passes then run
+  // their int8 instances (and their bf16 ones return at once)
+  const void* Xi8;
+  const int32_t* notu8;
+  int64_t item_cap;      // capacity of the item list (items gathered by size need as much)
 };
 // kModeTree: exact per-tree lists (bound pass + filter pass); kModeCand: the
 // k' widened candidates of each row (k' = kprime_eff(p)); kModePool: filter
@@ -252,6 +266,8 @@ int lowp_slots(const drpt_params* p);  // k' actually used (a multiple of 4)
 // widened candidates, TK in {3, 4, 6}; BF16 rows) over the items of a.range
 // with sizes in [a.size_lo, a.size_hi]
 int pool_leaf_rows(drpt_ctx* ctx, int d, int mode);
+// the int8 instances apply at dimension d (given an int8 copy and its flag)
+bool pool_leaf_int8(int d);
 int launch_pool_leaf(drpt_ctx* ctx, int mode, int kp, const KnnArgs& a);
 
 struct RowMergeArgs {
@@ -278,6 +294,10 @@ struct RowMergeArgs {
   // the bf16 rows (same values, half the bytes)
   const uint16_t* Xb;
   const int32_t* lossy;
+  // optional int8 copy of X (x - 128) and a device flag, 0 when every value
+  // of X is an integer in [0, 255] (the copy then holds them all): read before Xb
+  const uint8_t* Xi8;
+  const int32_t* notu8;
   int32_t* out_idx;          // [n][K]
   float* out_dist;
   float* out_bound;          // optional [n]: out_dist[row][K - 1] (the row's bound)

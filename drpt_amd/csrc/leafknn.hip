@@ -1793,9 +1793,12 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge2(const RowMergeA
       __builtin_amdgcn_wave_barrier();
     }
     const int nfmax = max(nf, __shfl_xor(nf, 32));
-    // rows exactly representable in bf16 (integer data, for one): read the
-    // bf16 copy, the same values in half the bytes
-    const bool b16 = RL && A.Xb != nullptr && (d & 7) == 0 && *A.lossy == 0;
+    // every value an integer in [0, 255]: read the int8 copy (x - 128, one
+    // byte per value); else rows exactly representable in bf16: the bf16 copy
+    // (the same values in half the bytes); the fma chain over the same
+    // values gives the same D either way
+    const bool u8c = RL && A.Xi8 != nullptr && (d & 15) == 0 && *A.notu8 == 0;
+    const bool b16 = !u8c && RL && A.Xb != nullptr && (d & 7) == 0 && *A.lossy == 0;
     for (int c0 = 0; c0 < nfmax; c0 += 32) {
       const int j = c0 + hl;
       uint64_t key = ~0ull;
@@ -1803,7 +1806,26 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge2(const RowMergeA
         const int id = fr[j];
         const XT* xc = X + (A.xpos != nullptr ? (int64_t)A.xpos[id] : (int64_t)id) * d;
         float acc;
-        if (b16) {
+        if (u8c) {
+          const float* qf = qrow[w][h];
+          const uint8_t* xu = A.Xi8 + (int64_t)id * d;
+          acc = 0.0f;
+#pragma unroll 2
+          for (int k = 0; k < d; k += 16) {
+            const uint4 v = *reinterpret_cast<const uint4*>(xu + k);
+            const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const float4 a = *reinterpret_cast<const float4*>(&qf[k + 4 * q]);
+              const float av[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const float t = __fsub_rn(av[u], (float)(((vw[q] >> (8 * u)) & 0xFFu) ^ 0x80u));
+                acc = fmaf(t, t, acc);
+              }
+            }
+          }
+        } else if (b16) {
           const float* qf = qrow[w][h];
           const uint16_t* xb = A.Xb + (int64_t)id * d;
           acc = 0.0f;
@@ -2131,18 +2153,23 @@ k_knn_fallback(const XT* __restrict__ X, int d, const int32_t* __restrict__ perm
 // (rounded once) for d >= 512, where d FP32 additions would loosen the filter
 // bound.
 template <typename XT>
-__device__ __forceinline__ void prep_bf16_row(const XT* __restrict__ X, int64_t row, int d, int lane,
-                                              uint16_t* __restrict__ Xb,
-                                              float2* __restrict__ nbe) {
+__device__ __forceinline__ bool prep_bf16_row(const XT* __restrict__ X, int64_t row, int d, int lane,
+                                              uint16_t* __restrict__ Xb, float2* __restrict__ nbe,
+                                              uint8_t* __restrict__ Xu) {
   const bool wide = d >= 512;
   float s = 0.0f;
   double s64 = 0.0, e2 = 0.0;
+  bool bad = false;  // Xu: some value of the lane's is not an integer in [0, 255]
   for (int k = lane; k < d; k += 64) {
     const float x = (float)X[row * d + k];
     uint32_t bits = __float_as_uint(x);
     bits += 0x7fffu + ((bits >> 16) & 1u);
     const uint16_t b = (uint16_t)(bits >> 16);
     Xb[row * d + k] = b;
+    if (Xu != nullptr) {
+      if (x >= 0.0f && x <= 255.0f && x == rintf(x)) Xu[row * d + k] = (uint8_t)x ^ 0x80u;
+      else bad = true;
+    }
     const float xb = __uint_as_float((uint32_t)b << 16);
     if (wide) s64 += (double)xb * (double)xb;
     else s = fmaf(xb, xb, s);
@@ -2163,18 +2190,24 @@ __device__ __forceinline__ void prep_bf16_row(const XT* __restrict__ X, int64_t 
     if (e2 > 0.0) ef = ef * (1.0f + 1.0e-6f) + 1.0e-30f;
     nbe[row] = make_float2(wide ? (float)s64 : s, ef);
   }
+  return bad;
 }
 
 // One wave per row, the waves of the grid striding over the rows (a grid of
 // a few thousand blocks instead of one block per four rows).
+// Xu (optional): the int8 copy x - 128, written where the values are integers
+// in [0, 255]; *notu8 = 1 when some value is not (the caller zeroes it).
 template <typename XT>
 __global__ void k_prep_bf16(const XT* __restrict__ X, int64_t n, int d, uint16_t* __restrict__ Xb,
-                            float2* __restrict__ nbe) {
+                            float2* __restrict__ nbe, uint8_t* __restrict__ Xu,
+                            int32_t* __restrict__ notu8) {
   const int lane = threadIdx.x & 63;
   const int64_t wstride = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  bool bad = false;
   for (int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; row < n;
        row += wstride)
-    prep_bf16_row(X, row, d, lane, Xb, nbe);
+    bad = prep_bf16_row(X, row, d, lane, Xb, nbe, Xu) || bad;
+  if (Xu != nullptr && __ballot(bad) != 0 && lane == 0) atomicOr(notu8, 1);
 }
 
 // FP8 quantisation, one wave per row: scale 2^e with e the smallest exponent
@@ -2313,7 +2346,7 @@ int dispatch_exact(drpt_ctx* ctx, const drpt_params* p, const XT* X, const int32
 }  // namespace
 
 int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, int precision,
-                     void* Xq, float2* nbe, int8_t* pexp) {
+                     void* Xq, float2* nbe, int8_t* pexp, uint8_t* xi8, int32_t* notu8) {
   if (precision != DRPT_BF16 && precision != DRPT_FP8) {
     set_error("prep_lowp: precision not supported");
     return DRPT_ERR_UNSUPPORTED;
@@ -2331,10 +2364,10 @@ int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, 
                          (const float*)X, n, d, (uint8_t*)Xq, nbe, pexp);
   } else if (dtype == DRPT_U8)
     hipLaunchKernelGGL(k_prep_bf16<uint8_t>, dim3(sblocks), dim3(256), 0, ctx->stream,
-                       (const uint8_t*)X, n, d, (uint16_t*)Xq, nbe);
+                       (const uint8_t*)X, n, d, (uint16_t*)Xq, nbe, nullptr, nullptr);
   else
     hipLaunchKernelGGL(k_prep_bf16<float>, dim3(sblocks), dim3(256), 0, ctx->stream,
-                       (const float*)X, n, d, (uint16_t*)Xq, nbe);
+                       (const float*)X, n, d, (uint16_t*)Xq, nbe, notu8 ? xi8 : nullptr, notu8);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }

@@ -27,9 +27,23 @@
 // with one atomic per list; known ids are not dropped here (the row merge's
 // hash drops them before any exact distance).
 //
+// Integer rows (kI8). When every value of X is an integer in [0, 255], the
+// rows are staged as int8 x - 128 (distances do not change under the shift)
+// and the Gram G' runs on v_mfma_i32_32x32x32_i8: exact in int32, twice the
+// bf16 rate, 128 bytes per row of d <= 128, so two blocks share a CU and one
+// block's leaf set-up overlaps the other's tasks. With n'_j = |x_j - 128|^2
+// (exact, from the staged row) the accumulator starts at -floor(n'_j / 2) and
+// the filter keeps acc_ij >= thr_i = floor((n'_i - B_i) / 2) in integers: any
+// leafmate with D(i, j) = n'_i + n'_j - 2 G'_ij <= B_i passes (the test is
+// the exact one up to the halving). Candidates start at -ceil(n'_j / 2), so
+// that a = n'_i - 2 acc >= D, the bound their keys must give. The bf16 and
+// int8 launches are both issued; a device flag (set by the quantisation
+// kernel) lets exactly one of them run.
+//
 // Roofline: MFMA-bound, 2 d m^2 flops per leaf; HBM bytes per leaf m * 2d
-// (rows) + 16 m (ids, norms, errors, bounds).
+// (rows; m * d for int8) + 16 m (ids, norms, errors, bounds).
 #include <climits>
+#include <type_traits>
 
 #include "internal.hpp"
 #include "lowp_common.hpp"
@@ -42,14 +56,23 @@ using namespace lowp;
 
 constexpr int kPlWaves = 8;
 constexpr int kPlThreads = 64 * kPlWaves;
+constexpr int kPlThreads8 = 256;  // int8 instances: 4 waves, two blocks per CU
 constexpr int kPlPitch = 256;   // LDS bytes per leaf row (16 chunks of 16 bytes)
 constexpr int kPlSteps = 8;     // k-steps of 16 elements: rows of at most 128 bf16
 constexpr int kPlList = 8;      // pool entries a lane holds per query row
 constexpr int kPlQueues = 8;    // ticket queues, one per XCD (blocks go round-robin)
 constexpr int kPlMisc = 40;     // ints of bookkeeping after the leaf arrays
 
+constexpr int kPlPitch8 = 128;  // int8 rows: 8 chunks of 16 bytes (d <= 128)
+constexpr int kPlSteps8 = 4;    // k-steps of 32 int8 elements
+
+typedef int intx4 __attribute__((ext_vector_type(4)));
+typedef int intx16 __attribute__((ext_vector_type(16)));
+
+// chunk c of LDS row r for row pitch P (chunks XOR-swizzled by the row)
+template <int P>
 __device__ __forceinline__ uint4 row_chunk(const unsigned char* rows, int r, int c) {
-  return *reinterpret_cast<const uint4*>(rows + r * kPlPitch + ((c ^ (r & 15)) << 4));
+  return *reinterpret_cast<const uint4*>(rows + r * P + ((c ^ (r & (P / 16 - 1))) << 4));
 }
 
 // Per-lane list of 8 leaf positions in 4 registers (newest in w[0] bits 0-15).
@@ -118,10 +141,28 @@ __device__ __forceinline__ uint32_t pass_bits(const floatx16& acc, float thr) {
     fb = __builtin_amdgcn_alignbit(fb, __float_as_uint(acc[e] - thr), 31);
   return ~fb & 0xFFFFu;
 }
+// integer accumulators: the sign of acc[e] - thr (|acc| < 2^23 and thr in
+// [-2^30, 2^30], so the difference does not wrap)
+__device__ __forceinline__ uint32_t pass_bits(const intx16& acc, int thr) {
+  uint32_t fb = 0;
+#pragma unroll
+  for (int e = 15; e >= 0; --e) fb = __builtin_amdgcn_alignbit(fb, (uint32_t)(acc[e] - thr), 31);
+  return ~fb & 0xFFFFu;
+}
+
+// sum of squares of the 16 int8 values of a chunk
+__device__ __forceinline__ int sq16(uint4 v) {
+  int s = __builtin_amdgcn_sdot4((int)v.x, (int)v.x, 0, false);
+  s = __builtin_amdgcn_sdot4((int)v.y, (int)v.y, s, false);
+  s = __builtin_amdgcn_sdot4((int)v.z, (int)v.z, s, false);
+  return __builtin_amdgcn_sdot4((int)v.w, (int)v.w, s, false);
+}
 
 // LDS layout of k_pool_leaf for mmax leaf rows: the rows, then per row its id,
-// -|x~|^2 / 2 and threshold, then kPlMisc ints of bookkeeping.
+// -|x~|^2 / 2 and threshold (int8: the integer forms), then kPlMisc ints of
+// bookkeeping.
 __host__ __device__ constexpr size_t pool_leaf_row_bytes(int) { return kPlPitch + 12; }
+__host__ __device__ constexpr size_t pool_leaf_row_bytes8() { return kPlPitch8 + 12; }
 
 // misc (ints): next-item slots [8 s .. 8 s + 6] (a LeafItem and a valid flag),
 // task counter [16], per-wave maxima (floats) [20 .. 35]
@@ -162,22 +203,36 @@ __device__ __forceinline__ void fetch_item(const KnnArgs& A, unsigned* myq, int6
 // registers, and the item after it is fetched by thread 0, while the leaf's
 // tasks run; after them (one barrier) the next leaf is installed (ids and
 // maxima in LDS, its rows by global_load_lds_dwordx4, thresholds).
-template <int KS, int MODE, int TK>
-__global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, int mmax) {
+//
+// kI8: int8 rows (A.Xi8), KS k-steps of 32 elements, 256-thread blocks, two
+// per CU (each holds one leaf, so one's set-up overlaps the other's tasks). With
+// A.notu8 given, the bf16 instance returns at once when *notu8 == 0 and the
+// int8 one when it is not.
+template <int KS, int MODE, int TK, bool kI8>
+__global__ void __launch_bounds__(kI8 ? kPlThreads8 : kPlThreads, kI8 ? 2 : 1) k_pool_leaf(const KnnArgs A, int mmax) {
   static_assert(MODE == kModePool || MODE == kModeCand, "pool or candidate mode");
   constexpr bool kPool = MODE == kModePool;
+  if (A.notu8 != nullptr && ((*A.notu8 == 0) != kI8)) return;
+  typedef typename std::conditional<kI8, intx16, floatx16>::type acc_t;
+  typedef typename std::conditional<kI8, int, float>::type thr_t;
+  constexpr int kPitch = kI8 ? kPlPitch8 : kPlPitch;
   extern __shared__ __align__(16) unsigned char smem[];
   unsigned char* rows = smem;
-  int32_t* lid = reinterpret_cast<int32_t*>(smem + (size_t)mmax * kPlPitch);
-  float* chalf = reinterpret_cast<float*>(lid + mmax);
-  float* thrs = chalf + mmax;  // per leaf row: filter threshold (pool) or |x~|^2 (candidates)
+  int32_t* lid = reinterpret_cast<int32_t*>(smem + (size_t)mmax * kPitch);
+  // per leaf row: the accumulator's start (-|x~|^2 / 2; int8: -floor or
+  // -ceil(n' / 2)) and the filter threshold (pool) or squared norm (candidates)
+  thr_t* chalf = reinterpret_cast<thr_t*>(lid + mmax);
+  thr_t* thrs = chalf + mmax;
   int* misc = reinterpret_cast<int*>(thrs + mmax);
   float* red = reinterpret_cast<float*>(misc + kMiscRed);  // per wave: max |x~|^2 [w], max error [8 + w]
-  const uint8_t* __restrict__ Xq = static_cast<const uint8_t*>(A.Xq);
-  constexpr int d = 16 * KS;
-  constexpr int rb = 2 * d;      // bytes per bf16 row
+  const uint8_t* __restrict__ Xq = static_cast<const uint8_t*>(kI8 ? A.Xi8 : A.Xq);
+  constexpr int d = kI8 ? 32 * KS : 16 * KS;
+  constexpr int rb = 32 * KS;    // bytes per staged row
   constexpr int L = rb >> 4;     // 16-byte chunks per row
-  constexpr int kRowsPerThread = 2;  // leaf rows whose metadata a thread loads
+  constexpr int kRowsPerDma = 1024 / kPitch;  // rows one wave-wide DMA instruction covers
+  // block: 8 waves (bf16, one block per CU) or 4 (int8, two per CU)
+  constexpr int kThr = kI8 ? kPlThreads8 : kPlThreads, kWv = kThr / 64;
+  constexpr int kRowsPerThread = kI8 ? 3 : 2;  // leaf rows whose metadata a thread loads
   const int64_t n = A.n;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int half = lane >> 5;
@@ -192,20 +247,22 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
       (srel >= 0 && (srel & 4) == 0) ? 1u << (((srel >> 3) << 2) | (srel & 3)) : 0u;
   const double u = 5.9604644775390625e-08;  // 2^-24
 
-  // metadata of a leaf's rows j = tid + h * kPlThreads, loaded into registers
+  // metadata of a leaf's rows j = tid + h * kThr, loaded into registers
   int rid[kRowsPerThread];
   float rnb[kRowsPerThread], rer[kRowsPerThread], rbd[kRowsPerThread];
   auto load_meta = [&](const LeafItem& it) {
     const int64_t lofs = (int64_t)it.tree * n + it.start;
 #pragma unroll
     for (int h = 0; h < kRowsPerThread; ++h) {
-      const int j = tid + h * kPlThreads;
+      const int j = tid + h * kThr;
       rid[h] = 0;
       rnb[h] = rer[h] = rbd[h] = 0.0f;
       if (j < it.size) {
         rid[h] = A.perm[lofs + j];
-        rnb[h] = A.lnbT[lofs + j];
-        rer[h] = A.lerrT[lofs + j];
+        if constexpr (!kI8) {
+          rnb[h] = A.lnbT[lofs + j];
+          rer[h] = A.lerrT[lofs + j];
+        }
         if constexpr (kPool) rbd[h] = A.boundT[lofs + j];
       }
     }
@@ -216,56 +273,100 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
   // thr = round_down((|x~|^2 - A_cut(B) - eta) / 2)
   auto install = [&](const LeafItem& it) {
     const int m = it.size;
-    float mn = 0.0f, me = 0.0f;
+    if constexpr (kI8) {
+      // ids; the rows; then from each staged row n' = |x - 128|^2 (exact),
+      // the accumulator start and the threshold
 #pragma unroll
-    for (int h = 0; h < kRowsPerThread; ++h) {
-      const int j = tid + h * kPlThreads;
-      if (j < m) {
-        lid[j] = rid[h];
-        chalf[j] = -0.5f * rnb[h];
-        mn = fmaxf(mn, rnb[h]);
-        me = fmaxf(me, rer[h]);
+      for (int h = 0; h < kRowsPerThread; ++h) {
+        const int j = tid + h * kThr;
+        if (j < m) lid[j] = rid[h];
       }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      mn = fmaxf(mn, __shfl_xor(mn, off));
-      me = fmaxf(me, __shfl_xor(me, off));
-    }
-    if (lane == 0) {
-      red[wave] = mn;
-      red[8 + wave] = me;
-    }
-    if (tid == 0) misc[kMiscTask] = 0;
-    __syncthreads();  // ids and maxima visible
-    for (int blk = wave; blk * 4 < m; blk += kPlWaves) {
-      const int slot = blk * 64 + lane;
-      const int r = slot >> 4, c = (slot & 15) ^ (r & 15);
-      const uint32_t dst = __builtin_amdgcn_readfirstlane(
-          (uint32_t)(uintptr_t)(lds_ptr_t)(rows + blk * 64 * 16));
-      if (r < m && c < L) lds_dma16(Xq + (int64_t)lid[r] * rb + c * 16, dst);
-    }
-    float nmax = red[0], emax = red[8];
-#pragma unroll
-    for (int w = 1; w < kPlWaves; ++w) {
-      nmax = fmaxf(nmax, red[w]);
-      emax = fmaxf(emax, red[8 + w]);
-    }
-    const double nm = sqrt((double)nmax * (1.0 + (d + 4.0) * u));
-#pragma unroll
-    for (int h = 0; h < kRowsPerThread; ++h) {
-      const int j = tid + h * kPlThreads;
-      if (!kPool && j < m) thrs[j] = rnb[h];
-      if (kPool && j < m) {
-        const float acut = acut_of(rbd[h], rnb[h], rer[h], nm, emax, d);
-        const double ni = sqrt((double)rnb[h] * (1.0 + (d + 4.0) * u));
-        const double eta = (2.0 * d + 16.0) * u * (ni + nm) * (ni + nm);
-        const float t = __double2float_rd(0.5 * ((double)rnb[h] - (double)acut - eta));
-        thrs[j] = t == 0.0f ? -0.0f : t;  // a zero difference from it is then +0
+      if (tid == 0) misc[kMiscTask] = 0;
+      __syncthreads();  // ids visible
+      for (int blk = wave; blk * kRowsPerDma < m; blk += kWv) {
+        const int slot = blk * 64 + lane;
+        const int r = slot / (kPitch / 16), c = (slot % (kPitch / 16)) ^ (r & (kPitch / 16 - 1));
+        const uint32_t dst = __builtin_amdgcn_readfirstlane(
+            (uint32_t)(uintptr_t)(lds_ptr_t)(rows + blk * 64 * 16));
+        if (r < m && c < L) lds_dma16(Xq + (int64_t)lid[r] * rb + c * 16, dst);
       }
+      wait_vmcnt<0>();
+      __syncthreads();  // rows visible
+#pragma unroll
+      for (int h = 0; h < kRowsPerThread; ++h) {
+        const int j = tid + h * kThr;
+        if (j < m) {
+          int nq = 0;
+#pragma unroll
+          for (int c = 0; c < L; ++c) nq += sq16(row_chunk<kPitch>(rows, j, c));
+          if constexpr (kPool) {
+            const float B = rbd[h];
+            int t = -(1 << 30);
+            if (B < INFINITY) {
+              const double v = floor(((double)nq - (double)B) * 0.5);
+              t = (int)fmin(fmax(v, -1073741824.0), 1073741824.0);
+            }
+            chalf[j] = -(nq >> 1);
+            thrs[j] = t;
+          } else {
+            chalf[j] = -((nq + 1) >> 1);
+            thrs[j] = nq;
+          }
+        }
+      }
+      __syncthreads();  // starts and thresholds visible to every wave
+    } else {
+      float mn = 0.0f, me = 0.0f;
+#pragma unroll
+      for (int h = 0; h < kRowsPerThread; ++h) {
+        const int j = tid + h * kThr;
+        if (j < m) {
+          lid[j] = rid[h];
+          chalf[j] = -0.5f * rnb[h];
+          mn = fmaxf(mn, rnb[h]);
+          me = fmaxf(me, rer[h]);
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        mn = fmaxf(mn, __shfl_xor(mn, off));
+        me = fmaxf(me, __shfl_xor(me, off));
+      }
+      if (lane == 0) {
+        red[wave] = mn;
+        red[8 + wave] = me;
+      }
+      if (tid == 0) misc[kMiscTask] = 0;
+      __syncthreads();  // ids and maxima visible
+      for (int blk = wave; blk * 4 < m; blk += kWv) {
+        const int slot = blk * 64 + lane;
+        const int r = slot >> 4, c = (slot & 15) ^ (r & 15);
+        const uint32_t dst = __builtin_amdgcn_readfirstlane(
+            (uint32_t)(uintptr_t)(lds_ptr_t)(rows + blk * 64 * 16));
+        if (r < m && c < L) lds_dma16(Xq + (int64_t)lid[r] * rb + c * 16, dst);
+      }
+      float nmax = red[0], emax = red[8];
+#pragma unroll
+      for (int w = 1; w < kWv; ++w) {
+        nmax = fmaxf(nmax, red[w]);
+        emax = fmaxf(emax, red[8 + w]);
+      }
+      const double nm = sqrt((double)nmax * (1.0 + (d + 4.0) * u));
+#pragma unroll
+      for (int h = 0; h < kRowsPerThread; ++h) {
+        const int j = tid + h * kThr;
+        if (!kPool && j < m) thrs[j] = rnb[h];
+        if (kPool && j < m) {
+          const float acut = acut_of(rbd[h], rnb[h], rer[h], nm, emax, d);
+          const double ni = sqrt((double)rnb[h] * (1.0 + (d + 4.0) * u));
+          const double eta = (2.0 * d + 16.0) * u * (ni + nm) * (ni + nm);
+          const float t = __double2float_rd(0.5 * ((double)rnb[h] - (double)acut - eta));
+          thrs[j] = t == 0.0f ? -0.0f : t;  // a zero difference from it is then +0
+        }
+      }
+      wait_vmcnt<0>();
+      __syncthreads();  // rows and thresholds visible to every wave
     }
-    wait_vmcnt<0>();
-    __syncthreads();  // rows and thresholds visible to every wave
   };
 
   // items fetched one leaf ahead: the next item's metadata loads are in
@@ -296,10 +397,10 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
     const int ntile = (m + 31) >> 5;
     int nc = 1, best = INT_MAX;
     // candidates: one or two column ranges (each keeps TK keys per lane)
-    const int ncmax = MODE == kModeCand ? 2 : 4 * kPlWaves;
+    const int ncmax = MODE == kModeCand ? 2 : 4 * kWv;
     for (int c = 1; c <= ntile && c <= ncmax; ++c) {
       const int tp = (ntile + c - 1) / c;
-      const int cost = ((nq * c + kPlWaves - 1) / kPlWaves) * (tp + 1);
+      const int cost = ((nq * c + kWv - 1) / kWv) * (tp + 1);
       if (cost < best) {
         best = cost;
         nc = c;
@@ -325,7 +426,7 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
       const int qb = row0 + 64 * qi;
       const int t0 = cj * tper, t1 = min(ntile, t0 + tper);
       uint4 qop[2][KS];
-      float thr[2];
+      thr_t thr[2];
       int id[2];
       PosList lst[2];
       uint32_t key[2][MODE == kModeCand ? TK : 1];  // candidates: ascending packed keys
@@ -335,10 +436,13 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
         const bool valid = r < row_end;
         const int rr = valid ? r : row0;
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) qop[g][ks] = row_chunk(rows, rr, 2 * ks + half);
+        for (int ks = 0; ks < KS; ++ks) qop[g][ks] = row_chunk<kPitch>(rows, rr, 2 * ks + half);
         thr[g] = thrs[rr];
         id[g] = lid[rr];
-        if (!valid) thr[g] = INFINITY;  // and its bits are cleared below
+        if (!valid) {  // and its bits are cleared below
+          if constexpr (kI8) thr[g] = 1 << 30;
+          else thr[g] = INFINITY;
+        }
 #pragma unroll
         for (int s2 = 0; s2 < (MODE == kModeCand ? TK : 1); ++s2) key[g][s2] = 0xFFFFFFFFu;
       }
@@ -346,39 +450,39 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
       lst[1].reset();
       const bool valid0 = qb + (lane & 31) < row_end;
       const bool valid1 = qb + 32 + (lane & 31) < row_end;
-      auto tile = [&](int t, floatx16& a0, floatx16& a1) {
+      auto mma = [](uint4 a, uint4 b, const acc_t& c) -> acc_t {
+        if constexpr (kI8)
+          return __builtin_amdgcn_mfma_i32_32x32x32_i8(__builtin_bit_cast(intx4, a),
+                                                       __builtin_bit_cast(intx4, b), c, 0, 0, 0);
+        else
+          return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                         __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+      };
+      auto tile = [&](int t, acc_t& a0, acc_t& a1) {
         const int c0 = 32 * t;
         const int r = c0 + (lane & 31);
         uint4 av[KS];
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) av[ks] = row_chunk(rows, r, 2 * ks + half);
-        floatx16 ci;
+        for (int ks = 0; ks < KS; ++ks) av[ks] = row_chunk<kPitch>(rows, r, 2 * ks + half);
+        acc_t ci;
         const int cb = c0 + 4 * half;
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-          const float4 v = *reinterpret_cast<const float4*>(&chalf[cb + 8 * g]);
-          ci[4 * g] = v.x;
-          ci[4 * g + 1] = v.y;
-          ci[4 * g + 2] = v.z;
-          ci[4 * g + 3] = v.w;
+          const uint4 v = *reinterpret_cast<const uint4*>(&chalf[cb + 8 * g]);
+          ci[4 * g] = __builtin_bit_cast(thr_t, v.x);
+          ci[4 * g + 1] = __builtin_bit_cast(thr_t, v.y);
+          ci[4 * g + 2] = __builtin_bit_cast(thr_t, v.z);
+          ci[4 * g + 3] = __builtin_bit_cast(thr_t, v.w);
         }
-        a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av[0]),
-                                                     __builtin_bit_cast(bf16x8, qop[0][0]), ci,
-                                                     0, 0, 0);
-        a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av[0]),
-                                                     __builtin_bit_cast(bf16x8, qop[1][0]), ci,
-                                                     0, 0, 0);
+        a0 = mma(av[0], qop[0][0], ci);
+        a1 = mma(av[0], qop[1][0], ci);
 #pragma unroll
         for (int ks = 1; ks < KS; ++ks) {
-          a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av[ks]),
-                                                       __builtin_bit_cast(bf16x8, qop[0][ks]),
-                                                       a0, 0, 0, 0);
-          a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av[ks]),
-                                                       __builtin_bit_cast(bf16x8, qop[1][ks]),
-                                                       a1, 0, 0, 0);
+          a0 = mma(av[ks], qop[0][ks], a0);
+          a1 = mma(av[ks], qop[1][ks], a1);
         }
       };
-      auto epilogue = [&](int t, const floatx16& a0, const floatx16& a1) {
+      auto epilogue = [&](int t, const acc_t& a0, const acc_t& a1) {
         const int c0 = 32 * t;
         const int cb = c0 + 4 * half;
         uint32_t colmask = 0xFFFFu;
@@ -396,7 +500,7 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
           // leafmates give)
 #pragma unroll
           for (int g = 0; g < 2; ++g) {
-            const floatx16& ac = g == 0 ? a0 : a1;
+            const acc_t& ac = g == 0 ? a0 : a1;
             uint32_t vm = (g == 0 ? valid0 : valid1) ? colmask : 0u;
             if (c0 == qb + 32 * g) vm &= ~selfbit;
 #pragma unroll
@@ -405,7 +509,9 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
 #pragma unroll
               for (int u4 = 0; u4 < 4; ++u4) {
                 const int e = 4 * q4 + u4;
-                const float a = fmaxf(fmaf(-2.0f, ac[e], thr[g]), 0.0f);
+                float a;
+                if constexpr (kI8) a = (float)(thr[g] - 2 * ac[e]);  // >= D >= 0
+                else a = fmaxf(fmaf(-2.0f, ac[e], thr[g]), 0.0f);
                 const uint32_t ke = (vm >> e) & 1u
                                         ? (__float_as_uint(a) & 0xFFFFF800u) | (uint32_t)(cb + 8 * q4 + u4)
                                         : 0xFFFFFFFFu;
@@ -440,13 +546,13 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
       };
       // two accumulator sets: the next tile's MFMAs are in flight during this
       // tile's epilogue
-      floatx16 a0, a1, b0, b1;
+      acc_t a0, a1, b0, b1;
       tile(t0, a0, a1);
       if constexpr (kPool) {
         const uint32_t vm0 = valid0 ? 0xFFFFu : 0u, vm1 = valid1 ? 0xFFFFu : 0u;
         // filter masks of tile t (branch-free, so that they share a basic
         // block with the next tile's MFMAs and the scheduler interleaves them)
-        auto filter = [&](int t, const floatx16& x0, const floatx16& x1, uint32_t& pm0,
+        auto filter = [&](int t, const acc_t& x0, const acc_t& x1, uint32_t& pm0,
                           uint32_t& pm1) {
           const int c0 = 32 * t;
           const uint32_t cm = c0 + 32 > m ? lastmask : 0xFFFFu;
@@ -531,6 +637,59 @@ __global__ void __launch_bounds__(kPlThreads, 1) k_pool_leaf(const KnnArgs A, in
   }
 }
 
+template <int MODE, int TK>
+const void* pool_leaf_fn(int ks) {
+  switch (ks) {
+    case 1: return (const void*)k_pool_leaf<1, MODE, TK, false>;
+    case 2: return (const void*)k_pool_leaf<2, MODE, TK, false>;
+    case 3: return (const void*)k_pool_leaf<3, MODE, TK, false>;
+    case 4: return (const void*)k_pool_leaf<4, MODE, TK, false>;
+    case 5: return (const void*)k_pool_leaf<5, MODE, TK, false>;
+    case 6: return (const void*)k_pool_leaf<6, MODE, TK, false>;
+    case 7: return (const void*)k_pool_leaf<7, MODE, TK, false>;
+    default: return (const void*)k_pool_leaf<8, MODE, TK, false>;
+  }
+}
+
+template <int MODE, int TK>
+const void* pool_leaf_fn8(int ks) {
+  switch (ks) {
+    case 1: return (const void*)k_pool_leaf<1, MODE, TK, true>;
+    case 2: return (const void*)k_pool_leaf<2, MODE, TK, true>;
+    case 3: return (const void*)k_pool_leaf<3, MODE, TK, true>;
+    default: return (const void*)k_pool_leaf<4, MODE, TK, true>;
+  }
+}
+
+template <bool kI8>
+const void* pool_leaf_instance(int mode, int tk, int ks) {
+  if (mode == kModePool)
+    return kI8 ? pool_leaf_fn8<kModePool, 1>(ks) : pool_leaf_fn<kModePool, 1>(ks);
+  if (tk == 3) return kI8 ? pool_leaf_fn8<kModeCand, 3>(ks) : pool_leaf_fn<kModeCand, 3>(ks);
+  if (tk == 4) return kI8 ? pool_leaf_fn8<kModeCand, 4>(ks) : pool_leaf_fn<kModeCand, 4>(ks);
+  return kI8 ? pool_leaf_fn8<kModeCand, 6>(ks) : pool_leaf_fn<kModeCand, 6>(ks);
+}
+
+// one launch of instance fn over the items of b with sizes [b.size_lo,
+// b.size_hi], mmax staged rows, `lds` bytes of dynamic LDS, `per_cu` blocks per CU
+int launch_pool_leaf_one(drpt_ctx* ctx, const void* fn, KnnArgs b, int mmax, size_t lds, int per_cu,
+                         int threads) {
+  auto it = ctx->fn_lds.find(fn);
+  if (it == ctx->fn_lds.end() || it->second < (int)lds) {
+    DRPT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    ctx->fn_lds[fn] = (int)lds;
+  }
+  unsigned* q;
+  DRPT_CHECK(ws(ctx, "knn.queue_pl", kPlQueues, &q));
+  DRPT_CHECK(fill_u32(ctx, q, 0u, kPlQueues));
+  b.queue = q;
+  void* args[] = {(void*)&b, (void*)&mmax};
+  DRPT_HIP(hipLaunchKernel(fn, dim3((unsigned)(ctx->num_cus * per_cu)), dim3(threads), args, lds,
+                           ctx->stream));
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
+}
+
 }  // namespace
 
 // Largest leaf (rows, a multiple of 32: the tiles read whole 32-row groups)
@@ -543,19 +702,7 @@ int pool_leaf_rows(drpt_ctx* ctx, int d, int mode) {
   return rows >= 64 ? (int)std::min<int64_t>(rows, 2 * kPlThreads) : 0;
 }
 
-template <int MODE, int TK>
-const void* pool_leaf_fn(int ks) {
-  switch (ks) {
-    case 1: return (const void*)k_pool_leaf<1, MODE, TK>;
-    case 2: return (const void*)k_pool_leaf<2, MODE, TK>;
-    case 3: return (const void*)k_pool_leaf<3, MODE, TK>;
-    case 4: return (const void*)k_pool_leaf<4, MODE, TK>;
-    case 5: return (const void*)k_pool_leaf<5, MODE, TK>;
-    case 6: return (const void*)k_pool_leaf<6, MODE, TK>;
-    case 7: return (const void*)k_pool_leaf<7, MODE, TK>;
-    default: return (const void*)k_pool_leaf<8, MODE, TK>;
-  }
-}
+bool pool_leaf_int8(int d) { return d >= 32 && (d & 31) == 0 && d <= 32 * kPlSteps8; }
 
 int launch_pool_leaf(drpt_ctx* ctx, int mode, int kp, const KnnArgs& a0) {
   if (mode != kModePool && mode != kModeCand) {
@@ -567,36 +714,48 @@ int launch_pool_leaf(drpt_ctx* ctx, int mode, int kp, const KnnArgs& a0) {
     set_error("internal: pool_leaf launched for leaves it cannot stage");
     return DRPT_ERR_ARG;
   }
-  const size_t lds = (size_t)mmax * pool_leaf_row_bytes(mode) + kPlMisc * 4;
-  const int ks = a0.d / 16;
-  const void* fn = nullptr;
-  int slot = ks;
-  if (mode == kModePool) {
-    fn = pool_leaf_fn<kModePool, 1>(ks);
-  } else {
-    const int tk = kp / 4;
+  int tk = 1;
+  if (mode == kModeCand) {
+    tk = kp / 4;
     if (kp != 4 * tk || (tk != 3 && tk != 4 && tk != 6)) {
       set_error("internal: pool_leaf candidate mode needs k' = 12, 16 or 24");
       return DRPT_ERR_ARG;
     }
-    fn = tk == 3 ? pool_leaf_fn<kModeCand, 3>(ks) : tk == 4 ? pool_leaf_fn<kModeCand, 4>(ks)
-                                                            : pool_leaf_fn<kModeCand, 6>(ks);
-    slot = 8 * (tk == 3 ? 1 : tk == 4 ? 2 : 3) + ks;
   }
-  if (!(ctx->pool_leaf_lds_set & (1ull << slot))) {
-    DRPT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    ctx->pool_leaf_lds_set |= 1ull << slot;
-  }
+  // the int8 instances where the rows have an int8 copy: issued after the
+  // bf16 one, the device flag picks which runs
+  const bool i8 = a0.Xi8 != nullptr && a0.notu8 != nullptr && pool_leaf_int8(a0.d);
   KnnArgs a = a0;
-  unsigned* q;
-  DRPT_CHECK(ws(ctx, "knn.queue_pl", kPlQueues, &q));
-  DRPT_CHECK(fill_u32(ctx, q, 0u, kPlQueues));
-  a.queue = q;
-  void* args[] = {(void*)&a, (void*)&mmax};
-  DRPT_HIP(hipLaunchKernel(fn, dim3((unsigned)ctx->num_cus), dim3(kPlThreads), args, lds,
-                           ctx->stream));
-  DRPT_HIP(hipGetLastError());
-  return DRPT_OK;
+  if (!i8) a.notu8 = nullptr;
+  const size_t lds = (size_t)mmax * pool_leaf_row_bytes(mode) + kPlMisc * 4;
+  DRPT_CHECK(launch_pool_leaf_one(ctx, pool_leaf_instance<false>(mode, tk, a0.d / 16), a, mmax, lds, 1,
+                                  kPlThreads));
+  if (!i8) return DRPT_OK;
+  const void* fn8 = pool_leaf_instance<true>(mode, tk, a0.d / 32);
+  // two blocks per CU for the leaves whose rows fit half the LDS, one for the rest
+  const int m2 = (int)(((int64_t)ctx->max_lds / 2 - kPlMisc * 4) / (int64_t)pool_leaf_row_bytes8()) & ~31;
+  if (m2 >= 64 && a0.size_lo <= m2) {
+    KnnArgs b = a;
+    b.size_hi = std::min(a0.size_hi, m2);
+    DRPT_CHECK(launch_pool_leaf_one(ctx, fn8, b, m2, (size_t)m2 * pool_leaf_row_bytes8() + kPlMisc * 4, 2,
+                                    kPlThreads8));
+    if (a0.size_hi <= m2) return DRPT_OK;
+    a.size_lo = m2 + 1;
+  }
+  if (a0.item_cap > 0) {
+    // the few larger leaves: their items gathered first, so that the launch
+    // does not walk (and skip) every other item
+    LeafItem* big;
+    int64_t* big_range;
+    DRPT_CHECK(ws(ctx, "knn.items_big8", (size_t)a0.item_cap, &big));
+    DRPT_CHECK(ws(ctx, "knn.range_big8", 2, &big_range));
+    DRPT_CHECK(launch_select_items(ctx, a0.items, a0.range, a.size_lo, a.size_hi, a0.item_cap, big,
+                                   big_range));
+    a.items = big;
+    a.range = big_range;
+  }
+  return launch_pool_leaf_one(ctx, fn8, a, mmax, (size_t)mmax * pool_leaf_row_bytes8() + kPlMisc * 4, 1,
+                              kPlThreads8);
 }
 
 }  // namespace drpt

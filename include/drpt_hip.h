@@ -91,7 +91,9 @@ typedef struct {
   int64_t rerank;       /* exact re-rank depth of the first bounds (-1: bounds from the
                            approximate distances alone) */
   int64_t pool_entries; /* leafmates the pooled filter passes appended, over all rows and passes */
-  int64_t reserved[3];
+  int64_t int8_passes;  /* 1: the LDS-resident candidate and pool passes ran on int8 rows x - 128
+                           (every value of X an integer in [0, 255]; exact int32 Gram) */
+  int64_t reserved[2];
 } drpt_phases;
 
 /* Thread-local message of the last failing call. */

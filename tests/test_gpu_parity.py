@@ -30,11 +30,22 @@ def _bits(a):
 CASES = {
     "mnist_u8_3000": lambda: datagen.mnist_like(3000),
     "int128_f32_4000": lambda: datagen.int_mixture(4000),
+    # even integers up to 436: exact in bf16 but not u8 values
+    "int128x2_f32_4000": lambda: datagen.int_mixture(4000) * 2,
+    # u8 values but one (256, still exact in bf16)
+    "int128_one256_f32_4000": lambda: _with_value(datagen.int_mixture(4000), 123, 5, 256.0),
+    "gauss128_f32_3000": lambda: (np.random.default_rng(8).standard_normal((3000, 128)) * 40.0
+                                  + 100.0).astype(np.float32),
     "gauss_d37_1500": lambda: np.random.default_rng(5).standard_normal((1500, 37)).astype(np.float32),
     "gauss960_800": lambda: datagen.gauss_mixture(800),
     # rows scaled by 10^U(-3, 3), signs mixed, one zero row: per-row fp8 scales
     "scaled_d64_2000": lambda: _scaled_rows(2000, 64),
 }
+
+
+def _with_value(X, i, j, v):
+    X[i, j] = v
+    return X
 
 
 def _scaled_rows(n, d):
@@ -219,10 +230,13 @@ LOWP_BUILD_CASES = [("int128_f32_4000", 4), ("int128_f32_4000", 7), ("gauss_d37_
 
 @pytest.mark.parametrize("prec,case,T", [("bf16",) + c for c in LOWP_BUILD_CASES] +
                          [("fp8",) + c for c in LOWP_BUILD_CASES] +
-                         [("fp8", "scaled_d64_2000", 6)])
+                         [("fp8", "scaled_d64_2000", 6), ("bf16", "int128x2_f32_4000", 6),
+                          ("bf16", "int128_one256_f32_4000", 6), ("bf16", "gauss128_f32_3000", 6)])
 def test_lowp_lists_equal_fp32(ctx, prec, case, T):
     """Whole build: BF16 / FP8 (trees beyond the first four pruned against the
-    merged bound) returns the FP32 build's lists."""
+    merged bound) returns the FP32 build's lists. The pool merge's exact
+    distances read the u8 copy (int128: values 0-218), the bf16 copy (x2: even
+    values to 436; one256: one value 256) or X (gauss128)."""
     _log(f"[{prec}==fp32] {case} T={T}")
     X = CASES[case]()
     n, d = X.shape
