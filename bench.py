@@ -274,10 +274,11 @@ def phase_rooflines(p, d, L, xbytes, tm, ph, prec):
         q = 1 if prec == "fp8" else 2
         first = max(1, int(ph["first_trees"]))
         kp = p.kprime if p.kprime else min(64, 2 * K + 4)
-        add("prep", ph["prep_ms"], n * d * (xbytes + 2 + (1 if prec == "fp8" else 0)) + 8.0 * n
-            + 12.0 * T * n * (2 if prec == "fp8" else 1), HBM_PEAK, "B")
-        # integer data in [0, 255]: the LDS-resident passes' int8 Gram
+        # integer data in [0, 255]: the LDS-resident passes' int8 Gram (and the
+        # quantisation kernel's int8 copy)
         i8 = bool(ph.get("int8_passes"))
+        add("prep", ph["prep_ms"], n * d * (xbytes + 2 + (1 if prec == "fp8" or i8 else 0)) + 8.0 * n
+            + 12.0 * T * n * (2 if prec == "fp8" else 1), HBM_PEAK, "B")
         gpeak, gunit = (MFMA_I8, "op") if i8 else (MFMA_FP8 if prec == "fp8" else MFMA_BF16, "flop")
         add("candidate_pass", ph["cand_ms"], gram * first / T, gpeak, gunit)
         if ph["rerank"] < 0:
