@@ -1785,20 +1785,36 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge2(const RowMergeA
     // exact D of the new ids, 32 at a time per row
     const int64_t qpos = A.xpos != nullptr ? (int64_t)A.xpos[row] : row;
     const XT* xr = X + qpos * d;
-    if constexpr (RL) {
-      float* qf = qrow[w][h];
-      if (act)
-        for (int k = hl * 4; k < d; k += 128) *reinterpret_cast<float4*>(&qf[k]) = load4f(xr + k);
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-    }
-    const int nfmax = max(nf, __shfl_xor(nf, 32));
-    // every value an integer in [0, 255]: read the int8 copy (x - 128, one
-    // byte per value); else rows exactly representable in bf16: the bf16 copy
-    // (the same values in half the bytes); the fma chain over the same
-    // values gives the same D either way
+    // every value an integer in [0, 255]: the int8 copy x' = x - 128 and
+    // D = n'_i + n'_j - 2 x'_i . x'_j in int32 (v_dot4: every partial sum an
+    // exact integer, D < 2^24, so it equals the fma chain's value); else rows
+    // exactly representable in bf16: the fma chain over the bf16 copy (the
+    // same values in half the bytes)
     const bool u8c = RL && A.Xi8 != nullptr && (d & 15) == 0 && *A.notu8 == 0;
     const bool b16 = !u8c && RL && A.Xb != nullptr && (d & 7) == 0 && *A.lossy == 0;
+    int ni = 0;  // u8c: n'_i
+    if constexpr (RL) {
+      float* qf = qrow[w][h];
+      if (u8c) {  // the query row's int8 bytes
+        if (act && hl < (d >> 4))
+          reinterpret_cast<uint4*>(qf)[hl] = *reinterpret_cast<const uint4*>(A.Xi8 + row * d + hl * 16);
+      } else if (act) {
+        for (int k = hl * 4; k < d; k += 128) *reinterpret_cast<float4*>(&qf[k]) = load4f(xr + k);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (u8c && act) {
+        const uint4* qa = reinterpret_cast<const uint4*>(qf);
+        for (int c = 0; c < (d >> 4); ++c) {
+          const uint4 a = qa[c];
+          ni = __builtin_amdgcn_sdot4((int)a.x, (int)a.x, ni, false);
+          ni = __builtin_amdgcn_sdot4((int)a.y, (int)a.y, ni, false);
+          ni = __builtin_amdgcn_sdot4((int)a.z, (int)a.z, ni, false);
+          ni = __builtin_amdgcn_sdot4((int)a.w, (int)a.w, ni, false);
+        }
+      }
+    }
+    const int nfmax = max(nf, __shfl_xor(nf, 32));
     for (int c0 = 0; c0 < nfmax; c0 += 32) {
       const int j = c0 + hl;
       uint64_t key = ~0ull;
@@ -1807,24 +1823,22 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge2(const RowMergeA
         const XT* xc = X + (A.xpos != nullptr ? (int64_t)A.xpos[id] : (int64_t)id) * d;
         float acc;
         if (u8c) {
-          const float* qf = qrow[w][h];
-          const uint8_t* xu = A.Xi8 + (int64_t)id * d;
-          acc = 0.0f;
-#pragma unroll 2
-          for (int k = 0; k < d; k += 16) {
-            const uint4 v = *reinterpret_cast<const uint4*>(xu + k);
-            const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const float4 a = *reinterpret_cast<const float4*>(&qf[k + 4 * q]);
-              const float av[4] = {a.x, a.y, a.z, a.w};
-#pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                const float t = __fsub_rn(av[u], (float)(((vw[q] >> (8 * u)) & 0xFFu) ^ 0x80u));
-                acc = fmaf(t, t, acc);
-              }
-            }
+          const uint4* qa = reinterpret_cast<const uint4*>(qrow[w][h]);
+          const uint4* xb = reinterpret_cast<const uint4*>(A.Xi8 + (int64_t)id * d);
+          int g = 0, nj = 0;
+#pragma unroll 4
+          for (int c = 0; c < (d >> 4); ++c) {
+            const uint4 b = xb[c], a = qa[c];
+            g = __builtin_amdgcn_sdot4((int)a.x, (int)b.x, g, false);
+            g = __builtin_amdgcn_sdot4((int)a.y, (int)b.y, g, false);
+            g = __builtin_amdgcn_sdot4((int)a.z, (int)b.z, g, false);
+            g = __builtin_amdgcn_sdot4((int)a.w, (int)b.w, g, false);
+            nj = __builtin_amdgcn_sdot4((int)b.x, (int)b.x, nj, false);
+            nj = __builtin_amdgcn_sdot4((int)b.y, (int)b.y, nj, false);
+            nj = __builtin_amdgcn_sdot4((int)b.z, (int)b.z, nj, false);
+            nj = __builtin_amdgcn_sdot4((int)b.w, (int)b.w, nj, false);
           }
+          acc = (float)(ni + nj - 2 * g);
         } else if (b16) {
           const float* qf = qrow[w][h];
           const uint16_t* xb = A.Xb + (int64_t)id * d;

@@ -255,16 +255,20 @@ def test_lowp_lists_equal_fp32(ctx, prec, case, T):
 
 
 @pytest.mark.parametrize("prec", ["bf16", "fp8"])
-@pytest.mark.parametrize("copies,nbase,T,overflow", [(20, 20, 8, False), (400, 5, 3, True),
-                                                     (400, 5, 6, True)])
-def test_lowp_duplicates_overflow(ctx, orc, prec, copies, nbase, T, overflow):
+@pytest.mark.parametrize("copies,nbase,T,overflow,d", [(20, 20, 8, False, 16), (400, 5, 3, True, 16),
+                                                       (400, 5, 6, True, 16), (40, 10, 8, False, 32),
+                                                       (400, 5, 3, True, 32)])
+def test_lowp_duplicates_overflow(ctx, orc, prec, copies, nbase, T, overflow, d):
     """Each vector repeated: ties at distance 0 fill the candidate pools (20
     copies over 8 trees: the same duplicates in every tree's pool segment,
     merged once; 400 copies: segments overflow and those rows are redone
-    exactly over every tree); the lists still equal the oracle's."""
-    _log(f"[edge] {prec} duplicates copies={copies} T={T}")
+    exactly over every tree); the lists still equal the oracle's. d = 32
+    (integer values: the int8 pool pass, pools of (id, distance) pairs): 40
+    copies over 8 trees put 312 entries in every pool, past the 256 the
+    two-rows-per-wave merge takes, so those rows go to the full-width merge."""
+    _log(f"[edge] {prec} duplicates copies={copies} T={T} d={d}")
     rng = np.random.default_rng(4)
-    base = rng.integers(0, 50, size=(nbase, 16)).astype(np.float32)
+    base = rng.integers(0, 50, size=(nbase, d)).astype(np.float32)
     X = np.repeat(base, copies, axis=0)
     n, d = X.shape
     p = drpt_amd.leafmax_params(n, d, T, 10, leaf_max=300, precision=prec, seed=8)
