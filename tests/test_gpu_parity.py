@@ -30,6 +30,7 @@ def _bits(a):
 CASES = {
     "mnist_u8_3000": lambda: datagen.mnist_like(3000),
     "int128_f32_4000": lambda: datagen.int_mixture(4000),
+    "int128_u8_4000": lambda: datagen.int_mixture(4000).astype(np.uint8),
     # even integers up to 436: exact in bf16 but not u8 values
     "int128x2_f32_4000": lambda: datagen.int_mixture(4000) * 2,
     # u8 values but one (256, still exact in bf16)
