@@ -606,13 +606,20 @@ static int prep_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X, const i
   DRPT_CHECK(ws(ctx, (tag + "lnbT").c_str(), (size_t)p->ntrees * n, &lnbT));
   DRPT_CHECK(ws(ctx, (tag + "lerrT").c_str(), (size_t)p->ntrees * n, &lerrT));
   if (fp8) DRPT_CHECK(ws(ctx, (tag + "lexpT").c_str(), (size_t)p->ntrees * n, &lexpT));
-  DRPT_CHECK(launch_leaf_gather(ctx, perm, (int64_t)p->ntrees * n, nbe, pexp, lnbT, lerrT, lexpT));
+  // with the int8 instances possible, the copies only when the data are not
+  // integers in [0, 255] (the int8 instances read none; the streaming kernel
+  // then reads nbe)
+  const int32_t* gate = notu8 != nullptr && ctx->use_int8 && pool_leaf_int8(p->d) ? notu8 : nullptr;
+  DRPT_CHECK(launch_leaf_gather(ctx, perm, (int64_t)p->ntrees * n, nbe, pexp, lnbT, lerrT, lexpT,
+                                gate));
   a->X = X;
   a->Xq = xq;
   a->precision = p->precision;
   a->lexpT = lexpT;
   a->lnbT = lnbT;
   a->lerrT = lerrT;
+  a->nbe = nbe;
+  a->lnb_gate = gate;
   a->perm = perm;
   a->n = n;
   a->d = p->d;

@@ -206,7 +206,8 @@ int launch_lossy_flag(drpt_ctx* ctx, const float2* nbe, int64_t n, int32_t* loss
 // leaf-ordered |x~|^2, |x~ - x| and (fp8) scale exponents: (lnbT[i], lerrT[i])
 // = nbe[perm[i]], lexpT[i] = pexp[perm[i]] (pexp may be null)
 int launch_leaf_gather(drpt_ctx* ctx, const int32_t* perm, int64_t total, const float2* nbe,
-                       const int8_t* pexp, float* lnbT, float* lerrT, int8_t* lexpT);
+                       const int8_t* pexp, float* lnbT, float* lerrT, int8_t* lexpT,
+                       const int32_t* gate = nullptr);
 // FP32 exact per-tree lists for items[range[0] .. range[1]) (range in device memory)
 int launch_leaf_knn(drpt_ctx* ctx, const drpt_params* p, const void* X, const int32_t* perm,
                     const LeafItem* items, const int64_t* range, int32_t* cand_idx,
@@ -254,6 +255,10 @@ struct KnnArgs {
   const void* Xi8;
   const int32_t* notu8;
   int64_t item_cap;      // capacity of the item list (items gathered by size need as much)
+  // optional: lnbT / lerrT were not written when *lnb_gate == 0; the streaming
+  // kernel then reads (|x~|^2, |x~ - x|) from nbe by id
+  const float2* nbe;
+  const int32_t* lnb_gate;
 };
 // kModeTree: exact per-tree lists (bound pass + filter pass); kModeCand: the
 // k' widened candidates of each row (k' = kprime_eff(p)); kModePool: filter

@@ -880,6 +880,8 @@ __global__ void __launch_bounds__(4 * lowp_rows<MODE>(), lowp_rows<MODE>() == 64
   const int64_t begin = A.range[0] + qx * chunk;
   const int64_t end = min(A.range[1], begin + chunk);
   unsigned* myq = A.queue + qx;
+  // the leaf-ordered norms and errors were not written: gather them from nbe
+  const bool direct = A.lnb_gate != nullptr && *A.lnb_gate == 0;
   if (tid == 0) S.ticket[0] = (int)atomicAdd(myq, 1u);
   if constexpr (MODE == kModePool) {
     if (tid == 0) S.flush_flag[0] = S.flush_flag[1] = 0;
@@ -894,6 +896,7 @@ __global__ void __launch_bounds__(4 * lowp_rows<MODE>(), lowp_rows<MODE>() == 64
     const int32_t* pl = A.perm + lofs;
     const int m = it.size;
     const int row_end = it.row0 + it.rows;
+    auto lerr_at = [&](int j) -> float { return direct ? A.nbe[S.lid[j]].y : A.lerrT[lofs + j]; };
     if (m < A.size_lo || m > A.size_hi) {  // another launch's size bucket
       __syncthreads();
       item = begin + S.ticket[parity];
@@ -922,15 +925,17 @@ __global__ void __launch_bounds__(4 * lowp_rows<MODE>(), lowp_rows<MODE>() == 64
       item = begin + S.ticket[parity];
       continue;
     }
-    // stage ids and norms of the leaf (leaf-ordered arrays: coalesced); leaf maxima
+    // stage ids and norms of the leaf (leaf-ordered arrays: coalesced; or,
+    // where they were not written, gathered from nbe); leaf maxima
     float mn = 0.0f, me_ = 0.0f;
     for (int j = tid; j < m; j += NT) {
-      const float v = A.lnbT[lofs + j];
-      S.lid[j] = pl[j];
-      S.lnb[j] = v;
+      const int id = pl[j];
+      const float2 ne = direct ? A.nbe[id] : make_float2(A.lnbT[lofs + j], A.lerrT[lofs + j]);
+      S.lid[j] = id;
+      S.lnb[j] = ne.x;
       if constexpr (kScaled) S.lexp[j] = A.lexpT[lofs + j];
-      mn = fmaxf(mn, v);
-      me_ = fmaxf(me_, A.lerrT[lofs + j]);
+      mn = fmaxf(mn, ne.x);
+      me_ = fmaxf(me_, ne.y);
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -1065,14 +1070,14 @@ __global__ void __launch_bounds__(4 * lowp_rows<MODE>(), lowp_rows<MODE>() == 64
           const uint64_t hit = __ballot(iv != INT_MAX && rank == K - 1);
           const float B = hit ? lane_read(dv, __builtin_ctzll(hit)) : INFINITY;
           if (lane == 0) {
-            S.cacut[r] = acut_of(B, S.lnb[row0 + r], A.lerrT[lofs + row0 + r], nm, emax, d);
+            S.cacut[r] = acut_of(B, S.lnb[row0 + r], lerr_at(row0 + r), nm, emax, d);
             S.ccnt[r] = KP;
           }
         }
       } else {  // kModePool: external bound, known ids of the block's rows
         if (tid < rows) {
-          S.cacut[tid] = acut_of(A.boundT[lofs + row0 + tid], S.lnb[row0 + tid],
-                                 A.lerrT[lofs + row0 + tid], nm, emax, d);
+          S.cacut[tid] = acut_of(A.boundT[lofs + row0 + tid], S.lnb[row0 + tid], lerr_at(row0 + tid),
+                                 nm, emax, d);
           S.ccnt[tid] = 0;
         }
         if constexpr (MODE == kModePool) {
@@ -1277,11 +1282,13 @@ __global__ void __launch_bounds__(4 * lowp_rows<MODE>(), lowp_rows<MODE>() == 64
   }
 }
 
-// Leaf-ordered copies of |x~|^2 and |x~ - x| (position j of tree t's leaf order).
+// Leaf-ordered copies of |x~|^2 and |x~ - x| (position j of tree t's leaf
+// order); none when *gate == 0 (the passes that read them then gather nbe).
 __global__ void k_leaf_gather(const int32_t* __restrict__ perm, int64_t total,
                               const float2* __restrict__ nbe, const int8_t* __restrict__ pexp,
                               float* __restrict__ lnbT, float* __restrict__ lerrT,
-                              int8_t* __restrict__ lexpT) {
+                              int8_t* __restrict__ lexpT, const int32_t* __restrict__ gate) {
+  if (gate != nullptr && *gate == 0) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int id = perm[i];
@@ -2387,10 +2394,11 @@ int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, 
 }
 
 int launch_leaf_gather(drpt_ctx* ctx, const int32_t* perm, int64_t total, const float2* nbe,
-                       const int8_t* pexp, float* lnbT, float* lerrT, int8_t* lexpT) {
+                       const int8_t* pexp, float* lnbT, float* lerrT, int8_t* lexpT,
+                       const int32_t* gate) {
   const int64_t blocks = std::min<int64_t>((total + 255) / 256, (int64_t)ctx->num_cus * 16);
   hipLaunchKernelGGL(k_leaf_gather, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0,
-                     ctx->stream, perm, total, nbe, pexp, lnbT, lerrT, lexpT);
+                     ctx->stream, perm, total, nbe, pexp, lnbT, lerrT, lexpT, gate);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }
