@@ -1933,16 +1933,30 @@ k_row_bound(const uint64_t* __restrict__ src_key, int64_t tree_stride, int S, in
   const float ce = (2.0f * d + 16.0f) * u;
   const int nslots = S * ntsrc;
   const int64_t nwaves = (int64_t)gridDim.x * kMergeWaves;
-  for (int64_t row = (int64_t)blockIdx.x * kMergeWaves + w; row < n; row += nwaves) {
-    const float2 mine = nbe[row];
-    const float ni = sqrtf(mine.x * wn);
-    uint32_t ub[E];
+  // a row's keys and own (|x~|^2, e), loaded one row ahead
+  auto load_row = [&](int64_t r, uint64_t (&kv)[E], float2& mine) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int j = e * 64 + lane;
       const int t = j / S, sl = j - t * S;
-      uint64_t kv = ~0ull;
-      if (j < nslots) kv = src_key[t * tree_stride + row * S + sl];
+      kv[e] = (r < n && j < nslots) ? src_key[t * tree_stride + r * S + sl] : ~0ull;
+    }
+    mine = r < n ? nbe[r] : make_float2(0.0f, 0.0f);
+  };
+  uint64_t kvn[E];
+  float2 minen;
+  load_row((int64_t)blockIdx.x * kMergeWaves + w, kvn, minen);
+  for (int64_t row = (int64_t)blockIdx.x * kMergeWaves + w; row < n; row += nwaves) {
+    uint64_t kvr[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) kvr[e] = kvn[e];
+    const float2 mine = minen;
+    load_row(row + nwaves, kvn, minen);
+    const float ni = sqrtf(mine.x * wn);
+    uint32_t ub[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const uint64_t kv = kvr[e];
       const int id = kv != ~0ull ? (int)(uint32_t)kv : -1;
       bool fresh = false;
       if (id >= 0 && id != (int)row) {
