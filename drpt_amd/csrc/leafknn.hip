@@ -1733,13 +1733,45 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge2(const RowMergeA
   const int64_t npairs = (nr + 1) >> 1;
   const int64_t chunk = (npairs + 7) >> 3;
   const int64_t lo = xcd * chunk, hi = min(npairs, lo + chunk);
-  for (int64_t pi = lo + (int64_t)(blockIdx.x >> 3) * kMergeWaves + w; pi < hi;
-       pi += nblk * kMergeWaves) {
+  const int64_t step = nblk * kMergeWaves;
+  // the row of pair p (this half wave's), and the loads of a row that do not
+  // depend on its pool count: its skip flag, count, known list entry and first
+  // 32 pool slots; issued one pair (the row index two pairs) ahead
+  auto row_of = [&](int64_t p) -> int64_t {
+    const int64_t r = 2 * p + h;
+    return p < hi && r < nr ? (A.order != nullptr ? (int64_t)A.order[r] : r) : -1;
+  };
+  struct Pre {
+    int skip, cnt, kid, src0;
+    float kdv;
+  };
+  auto load_pre = [&](int64_t r) -> Pre {
+    Pre q{0, 0, -1, -1, INFINITY};
+    if (r >= 0) {
+      q.skip = A.skip_flag != nullptr ? A.skip_flag[r] : 0;
+      q.cnt = A.cnt_tree[r];
+      if (A.known_idx != nullptr && hl < K) {
+        q.kid = A.known_idx[r * K + hl];
+        q.kdv = A.known_dist[r * K + hl];
+      }
+      if (hl < A.S) q.src0 = A.src[r * A.S + hl];  // read before the count is known
+    }
+    return q;
+  };
+  const int64_t p0 = lo + (int64_t)(blockIdx.x >> 3) * kMergeWaves + w;
+  int64_t row1 = row_of(p0);
+  Pre pre1 = load_pre(row1);
+  int64_t row2 = row_of(p0 + step);
+  for (int64_t pi = p0; pi < hi; pi += step) {
     const int64_t ri = 2 * pi + h;
     const bool has = ri < nr;
-    const int64_t row = has ? (A.order != nullptr ? (int64_t)A.order[ri] : ri) : 0;
-    const bool skip = has && A.skip_flag != nullptr && A.skip_flag[row] != 0;
-    const int cnt0 = has && !skip ? min(A.cnt_tree[row], A.S) : 0;
+    const int64_t row = has ? row1 : 0;
+    const Pre cur = pre1;
+    pre1 = load_pre(row2);
+    row1 = row2;
+    row2 = row_of(pi + 2 * step);
+    const bool skip = has && cur.skip != 0;
+    const int cnt0 = has && !skip ? min(cur.cnt, A.S) : 0;
     const bool heavy = cnt0 > kM2Fresh;  // to the full-width pass
     if (heavy && hl == 0) A.heavy[atomicAdd(A.nheavy, 1)] = (int32_t)row;
     const bool act = has && !skip && !heavy;
@@ -1747,8 +1779,8 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge2(const RowMergeA
     int kid = -1;
     float kdv = INFINITY;
     if (has && A.known_idx != nullptr && hl < K) {
-      kid = A.known_idx[row * K + hl];
-      kdv = A.known_dist[row * K + hl];
+      kid = cur.kid;
+      kdv = cur.kdv;
     }
     if (skip && hl < K) {  // left to the exact row path; the known list passes through
       A.out_idx[row * K + hl] = kid;
@@ -1770,7 +1802,7 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_row_merge2(const RowMergeA
     int nf = 0;
     for (int s0 = 0; s0 < cmax; s0 += 32) {
       const int sl = s0 + hl;
-      const int id = sl < cnt ? A.src[row * A.S + sl] : -1;
+      const int id = sl < cnt ? (s0 == 0 ? cur.src0 : A.src[row * A.S + sl]) : -1;
       bool isnew = false;
       if (id >= 0 && id != (int)row) {
         uint32_t hh = ((uint32_t)id * 2654435761u) >> (32 - kM2HashBits);
