@@ -199,6 +199,8 @@ int drpt_destroy(drpt_ctx* ctx) {
   for (auto& e : ctx->evp)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->ev_pass) (void)hipEventDestroy(e);
+  if (ctx->ev_cm) (void)hipEventDestroy(ctx->ev_cm);
+  if (ctx->pin_cm) (void)hipHostFree(ctx->pin_cm);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return DRPT_OK;
@@ -570,12 +572,11 @@ static int group_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
   return DRPT_OK;
 }
 
-// leaf statistics of a grouping into timings (reads the device counters: syncs)
-static int leaf_stats(drpt_ctx* ctx, const drpt_params* p, const Grouping& G, drpt_timings* tm) {
-  unsigned long long st[4] = {0, 0, 0, 0};
-  DRPT_HIP(hipMemcpyAsync(st, G.stats, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
-  DRPT_HIP(hipStreamSynchronize(ctx->stream));
-  if (!tm) return DRPT_OK;
+// leaf statistics of a grouping into timings, from the device counters st
+// (copied back by the caller) or the host's own figures
+static void leaf_stats_of(const drpt_params* p, const Grouping& G, const unsigned long long* st,
+                          drpt_timings* tm) {
+  if (!tm) return;
   if (G.host_gram >= 0.0) {
     tm->gram_flops = G.host_gram;
     tm->n_leaves = G.host_leaves;
@@ -585,6 +586,14 @@ static int leaf_stats(drpt_ctx* ctx, const drpt_params* p, const Grouping& G, dr
     tm->n_leaves = (int64_t)st[2];
     tm->max_leaf = (int64_t)st[1];
   }
+}
+
+// leaf statistics of a grouping into timings (reads the device counters: syncs)
+static int leaf_stats(drpt_ctx* ctx, const drpt_params* p, const Grouping& G, drpt_timings* tm) {
+  unsigned long long st[4] = {0, 0, 0, 0};
+  DRPT_HIP(hipMemcpyAsync(st, G.stats, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+  DRPT_HIP(hipStreamSynchronize(ctx->stream));
+  leaf_stats_of(p, G, st, tm);
   return DRPT_OK;
 }
 
@@ -1056,15 +1065,18 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
   }
   int32_t nov = 0, notu8 = 1;
   unsigned long long npool = 0;
+  unsigned long long gst[4] = {0, 0, 0, 0};
   DRPT_HIP(hipMemcpyAsync(&nov, r.novf, 4, hipMemcpyDeviceToHost, s));
   DRPT_HIP(hipMemcpyAsync(&npool, pool_sum, 8, hipMemcpyDeviceToHost, s));
   if (r.a.notu8 != nullptr) DRPT_HIP(hipMemcpyAsync(&notu8, r.notu8, 4, hipMemcpyDeviceToHost, s));
-  DRPT_HIP(hipStreamSynchronize(s));
+  DRPT_HIP(hipMemcpyAsync(gst, G.stats, sizeof(gst), hipMemcpyDeviceToHost, s));
+  DRPT_HIP(hipStreamSynchronize(s));  // one wait for all four
   if (nov > 0)
     DRPT_CHECK(launch_row_exact(ctx, &r.pp, X, G.perm, G.items, G.range + 4, r.olist, r.novf, nov,
                                 out_idx, out_dist));
   DRPT_HIP(hipEventRecord(ctx->ev[5], s));
-  DRPT_CHECK(leaf_stats(ctx, p, G, timings));
+  DRPT_HIP(hipEventSynchronize(ctx->ev[5]));  // the stage times below read it
+  leaf_stats_of(p, G, gst, timings);
   drpt_phases& ph = ctx->last_phases;
   ph = drpt_phases{};
   ph.group_ms = ms_between(ctx->ev[6], ctx->evp[0]);
@@ -1121,18 +1133,29 @@ static int build_front(drpt_ctx* ctx, const drpt_params* p, const void* X, uint3
   DRPT_HIP(hipEventRecord(ctx->ev[0], s));
   DRPT_CHECK(drpt_project(ctx, X, p->dtype, n, p->d, R.data(), L, P, colmax));
   DRPT_HIP(hipEventRecord(ctx->ev[1], s));
-  // a non-finite projection cannot be split (the fixed-point statistics need finite values)
-  std::vector<uint32_t> cm(L);
-  DRPT_HIP(hipMemcpyAsync(cm.data(), colmax, 4 * L, hipMemcpyDeviceToHost, s));
-  DRPT_HIP(hipStreamSynchronize(s));
-  for (uint32_t b : cm)
-    if (b >= 0x7f800000u) {
-      set_error("non-finite projected values (input contains inf/nan or overflows float)");
-      return DRPT_ERR_NONFINITE;
-    }
+  // a non-finite projection cannot be split (the fixed-point statistics need
+  // finite values): the column maxima come back to pinned memory while the
+  // splits run (their work is bounded whatever the values), and the error is
+  // returned before the leaf stage
+  if (ctx->pin_cm_cap < L) {
+    if (ctx->pin_cm) DRPT_HIP(hipHostFree(ctx->pin_cm));
+    ctx->pin_cm = nullptr;
+    ctx->pin_cm_cap = 0;
+    DRPT_HIP(hipHostMalloc((void**)&ctx->pin_cm, 4 * (size_t)L));
+    ctx->pin_cm_cap = L;
+  }
+  if (!ctx->ev_cm) DRPT_HIP(hipEventCreateWithFlags(&ctx->ev_cm, hipEventDisableTiming));
+  DRPT_HIP(hipMemcpyAsync(ctx->pin_cm, colmax, 4 * (size_t)L, hipMemcpyDeviceToHost, s));
+  DRPT_HIP(hipEventRecord(ctx->ev_cm, s));
   const int mode = p->local_mode == DRPT_LOCAL_REFERENCE ? DRPT_TREE_FIXED : DRPT_TREE_LEAFMAX;
   DRPT_CHECK(grow_trees_impl(ctx, P, colmax, n, T, D, mode, p->leaf_max, heap));
   DRPT_HIP(hipEventRecord(ctx->ev[2], s));
+  DRPT_HIP(hipEventSynchronize(ctx->ev_cm));
+  for (int c = 0; c < L; ++c)
+    if (ctx->pin_cm[c] >= 0x7f800000u) {
+      set_error("non-finite projected values (input contains inf/nan or overflows float)");
+      return DRPT_ERR_NONFINITE;
+    }
   *heap_out = heap;
   return DRPT_OK;
 }

@@ -56,6 +56,11 @@ struct drpt_ctx {
   // environment DRPT_INT8=0: the LDS-resident passes keep their bf16
   // instances for integer data in [0, 255] as well
   bool use_int8 = true;
+  // pinned host copy of a build's column maxima (checked for non-finite
+  // values while the splits run) and the event that marks it copied
+  uint32_t* pin_cm = nullptr;
+  int pin_cm_cap = 0;
+  hipEvent_t ev_cm = nullptr;
   // environment DRPT_ROW_EXACT_BATCH=r: at most r rows per batch of the exact
   // row path (its scratch is otherwise capped by bytes)
   int64_t row_exact_batch = 0;

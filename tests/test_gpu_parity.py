@@ -428,6 +428,33 @@ def test_lowp_k32_dense_clusters_equal_fp32(ctx, prec):
     assert np.array_equal(_bits(out["fp32"][1]), _bits(out[prec][1]))
 
 
+DRPT_ERR_NONFINITE = -6  # include/drpt_hip.h
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf])
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_build_nonfinite_input_refused(ctx, prec, bad):
+    """A non-finite value in X makes the projection non-finite: the build
+    returns DRPT_ERR_NONFINITE (the splits have run on the device meanwhile),
+    and the next build of finite data on the same context is unaffected."""
+    _log(f"[nonfinite] {prec} {bad}")
+    X = datagen.int_mixture(3000)
+    n, d = X.shape
+    p = drpt_amd.leafmax_params(n, d, 4, 10, leaf_max=200, precision=prec, seed=5)
+    Xb = X.copy()
+    Xb[1234, 7] = bad
+    with pytest.raises(drpt_amd.DrptError) as ei:
+        ctx.build_graph(p, _dev(Xb))
+    assert ei.value.code == DRPT_ERR_NONFINITE
+    assert "non-finite" in str(ei.value)
+    oi, od, _ = ctx.build_graph(p, _dev(X))
+    q = drpt_amd.leafmax_params(n, d, 4, 10, leaf_max=200, precision=prec, seed=5)
+    with drpt_amd.Context(0) as fresh:
+        ri, rd, _ = fresh.build_graph(q, _dev(X))
+    assert np.array_equal(oi.cpu().numpy(), ri.cpu().numpy())
+    assert np.array_equal(_bits(od.cpu().numpy()), _bits(rd.cpu().numpy()))
+
+
 @pytest.mark.parametrize("prec,K", [("bf16", 10), ("bf16", 24)])
 def test_lowp_leaves_beyond_stage_equal_fp32(ctx, prec, K):
     """Leaves larger than the 2048 points a block stages: their rows take the
