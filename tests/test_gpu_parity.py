@@ -91,11 +91,12 @@ def test_grow_trees_bitexact(ctx, orc, case, mode, leaf_max):
         assert np.array_equal(heap[t], ref), f"tree {t}: {(heap[t] != ref).sum()} points differ"
 
 
-@pytest.mark.parametrize("mode,D,leaf_max", [(0, 14, 0), (1, 15, 3), (1, 15, 64)])
+@pytest.mark.parametrize("mode,D,leaf_max", [(0, 14, 0), (1, 15, 3), (1, 15, 64), (1, 18, 2)])
 def test_grow_trees_deep_levels_bitexact(ctx, orc, mode, D, leaf_max):
     """Levels with 1024 - 8192 nodes per tree: per-node sums and histograms in
     large LDS arrays, then global atomics; with leaf_max 64 the deepest levels
-    hold few splitting nodes among many leaves. The trees equal the oracle's."""
+    hold few splitting nodes among many leaves; depth 18 with leaf_max 2 takes
+    heap indices beyond 2^16. The trees equal the oracle's."""
     _log(f"[grow deep] mode={mode} D={D}")
     X = datagen.int_mixture(40000)
     n, d = X.shape
