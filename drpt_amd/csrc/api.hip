@@ -200,7 +200,8 @@ int drpt_destroy(drpt_ctx* ctx) {
     if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->ev_pass) (void)hipEventDestroy(e);
   if (ctx->ev_cm) (void)hipEventDestroy(ctx->ev_cm);
-  if (ctx->pin_cm) (void)hipHostFree(ctx->pin_cm);
+  if (ctx->pin_cm.host) (void)hipHostFree(ctx->pin_cm.host);
+  if (ctx->pin_st.host) (void)hipHostFree(ctx->pin_st.host);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return DRPT_OK;
@@ -1063,14 +1064,21 @@ static int graph_knn_lowp(drpt_ctx* ctx, const drpt_params* p, const void* X,
     kin = f.out_idx;
     kdn = f.out_dist;
   }
-  int32_t nov = 0, notu8 = 1;
-  unsigned long long npool = 0;
-  unsigned long long gst[4] = {0, 0, 0, 0};
-  DRPT_HIP(hipMemcpyAsync(&nov, r.novf, 4, hipMemcpyDeviceToHost, s));
-  DRPT_HIP(hipMemcpyAsync(&npool, pool_sum, 8, hipMemcpyDeviceToHost, s));
-  if (r.a.notu8 != nullptr) DRPT_HIP(hipMemcpyAsync(&notu8, r.notu8, 4, hipMemcpyDeviceToHost, s));
-  DRPT_HIP(hipMemcpyAsync(gst, G.stats, sizeof(gst), hipMemcpyDeviceToHost, s));
-  DRPT_HIP(hipStreamSynchronize(s));  // one wait for all four
+  // overflow rows, pool entries, leaf statistics and the integer flag: one
+  // small kernel's stores into pinned memory, one wait
+  DRPT_CHECK(pinned_words(ctx, ctx->pin_st, 12));
+  const bool has_u8 = r.a.notu8 != nullptr;
+  DRPT_CHECK(words_to_host(ctx, ctx->pin_st.dev, (const uint32_t*)r.novf, 1,
+                           (const uint32_t*)pool_sum, 2, (const uint32_t*)G.stats, 8,
+                           has_u8 ? (const uint32_t*)r.notu8 : nullptr, has_u8 ? 1 : 0));
+  DRPT_HIP(hipStreamSynchronize(s));
+  const uint32_t* hw = ctx->pin_st.host;
+  const int32_t nov = (int32_t)hw[0];
+  const int32_t notu8 = has_u8 ? (int32_t)hw[11] : 1;
+  const unsigned long long npool = (unsigned long long)hw[1] | ((unsigned long long)hw[2] << 32);
+  unsigned long long gst[4];
+  for (int k = 0; k < 4; ++k)
+    gst[k] = (unsigned long long)hw[3 + 2 * k] | ((unsigned long long)hw[4 + 2 * k] << 32);
   if (nov > 0)
     DRPT_CHECK(launch_row_exact(ctx, &r.pp, X, G.perm, G.items, G.range + 4, r.olist, r.novf, nov,
                                 out_idx, out_dist));
@@ -1137,22 +1145,16 @@ static int build_front(drpt_ctx* ctx, const drpt_params* p, const void* X, uint3
   // finite values): the column maxima come back to pinned memory while the
   // splits run (their work is bounded whatever the values), and the error is
   // returned before the leaf stage
-  if (ctx->pin_cm_cap < L) {
-    if (ctx->pin_cm) DRPT_HIP(hipHostFree(ctx->pin_cm));
-    ctx->pin_cm = nullptr;
-    ctx->pin_cm_cap = 0;
-    DRPT_HIP(hipHostMalloc((void**)&ctx->pin_cm, 4 * (size_t)L));
-    ctx->pin_cm_cap = L;
-  }
+  DRPT_CHECK(pinned_words(ctx, ctx->pin_cm, L));
   if (!ctx->ev_cm) DRPT_HIP(hipEventCreateWithFlags(&ctx->ev_cm, hipEventDisableTiming));
-  DRPT_HIP(hipMemcpyAsync(ctx->pin_cm, colmax, 4 * (size_t)L, hipMemcpyDeviceToHost, s));
+  DRPT_CHECK(words_to_host(ctx, ctx->pin_cm.dev, colmax, L));
   DRPT_HIP(hipEventRecord(ctx->ev_cm, s));
   const int mode = p->local_mode == DRPT_LOCAL_REFERENCE ? DRPT_TREE_FIXED : DRPT_TREE_LEAFMAX;
   DRPT_CHECK(grow_trees_impl(ctx, P, colmax, n, T, D, mode, p->leaf_max, heap));
   DRPT_HIP(hipEventRecord(ctx->ev[2], s));
   DRPT_HIP(hipEventSynchronize(ctx->ev_cm));
   for (int c = 0; c < L; ++c)
-    if (ctx->pin_cm[c] >= 0x7f800000u) {
+    if (ctx->pin_cm.host[c] >= 0x7f800000u) {
       set_error("non-finite projected values (input contains inf/nan or overflows float)");
       return DRPT_ERR_NONFINITE;
     }

@@ -30,6 +30,14 @@ void set_error(const std::string& msg);
     if (rc_ != DRPT_OK) return rc_; \
   } while (0)
 
+// pinned, device-mapped host words (host: the host's address, dev: the
+// device's); grown by pinned_words, freed by drpt_destroy
+struct PinnedWords {
+  uint32_t* host = nullptr;
+  uint32_t* dev = nullptr;
+  int cap = 0;
+};
+
 }  // namespace drpt
 
 // Device workspace: named buffers, grown on demand and kept for the context's
@@ -58,8 +66,9 @@ struct drpt_ctx {
   bool use_int8 = true;
   // pinned host copy of a build's column maxima (checked for non-finite
   // values while the splits run) and the event that marks it copied
-  uint32_t* pin_cm = nullptr;
-  int pin_cm_cap = 0;
+  // pinned host words written by k_words_to_host: the projection's column
+  // maxima, and the counters a low-precision build reads back at its end
+  drpt::PinnedWords pin_cm, pin_st;
   hipEvent_t ev_cm = nullptr;
   // environment DRPT_ROW_EXACT_BATCH=r: at most r rows per batch of the exact
   // row path (its scratch is otherwise capped by bytes)
@@ -91,6 +100,13 @@ int host_local_depth(int64_t gmin, int nn, int offset);
 // ---- launchers (one per .hip unit) ------------------------------------------
 // util.hip: capture-safe fills (a kernel, not hipMemsetAsync)
 int fill_u32(drpt_ctx* ctx, uint32_t* p, uint32_t v, size_t count);
+// pw holds at least count words
+int pinned_words(drpt_ctx* ctx, PinnedWords& pw, int count);
+// s0[0..c0), s1[0..c1), ... into dst (a PinnedWords dev pointer), on the
+// context's stream
+int words_to_host(drpt_ctx* ctx, uint32_t* dst, const uint32_t* s0, int c0,
+                  const uint32_t* s1 = nullptr, int c1 = 0, const uint32_t* s2 = nullptr,
+                  int c2 = 0, const uint32_t* s3 = nullptr, int c3 = 0);
 // *acc += p[0] + ... + p[count - 1] (stream-ordered)
 int sum_u32(drpt_ctx* ctx, const uint32_t* p, size_t count, unsigned long long* acc);
 // Issues record() on the context's stream: eagerly when graphs are off, else

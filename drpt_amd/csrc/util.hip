@@ -26,7 +26,51 @@ __global__ void k_sum_u32(const uint32_t* __restrict__ p, size_t count,
   }
   if ((threadIdx.x & 63) == 0 && s != 0) atomicAdd(acc, s);
 }
+
+// Up to four small device word arrays, one after another, into pinned host
+// memory by plain stores from one block: no copy engine or staging buffer on
+// the stream's path; the host reads dst after a later synchronisation.
+__global__ void k_words_to_host(uint32_t* __restrict__ dst, const uint32_t* __restrict__ s0,
+                                int c0, const uint32_t* __restrict__ s1, int c1,
+                                const uint32_t* __restrict__ s2, int c2,
+                                const uint32_t* __restrict__ s3, int c3) {
+  for (int i = threadIdx.x; i < c0 + c1 + c2 + c3; i += blockDim.x) {
+    uint32_t v;
+    if (i < c0) v = s0[i];
+    else if (i < c0 + c1) v = s1[i - c0];
+    else if (i < c0 + c1 + c2) v = s2[i - c0 - c1];
+    else v = s3[i - c0 - c1 - c2];
+    dst[i] = v;
+  }
+}
 }  // namespace
+
+int words_to_host(drpt_ctx* ctx, uint32_t* dst, const uint32_t* s0, int c0, const uint32_t* s1,
+                  int c1, const uint32_t* s2, int c2, const uint32_t* s3, int c3) {
+  const int total = c0 + c1 + c2 + c3;
+  if (total <= 0) return DRPT_OK;
+  hipLaunchKernelGGL(k_words_to_host, dim3(1), dim3(256), 0, ctx->stream, dst, s0, c0, s1, c1, s2,
+                     c2, s3, c3);
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
+}
+
+int pinned_words(drpt_ctx* ctx, PinnedWords& pw, int count) {
+  (void)ctx;
+  if (pw.cap >= count) return DRPT_OK;
+  if (pw.host) DRPT_HIP(hipHostFree(pw.host));
+  pw.host = nullptr;
+  pw.dev = nullptr;
+  pw.cap = 0;
+  // coherent: the kernel's stores reach host memory without a cache flush
+  DRPT_HIP(hipHostMalloc((void**)&pw.host, 4 * (size_t)count,
+                         hipHostMallocMapped | hipHostMallocCoherent));
+  void* dev = nullptr;
+  DRPT_HIP(hipHostGetDevicePointer(&dev, pw.host, 0));
+  pw.dev = (uint32_t*)dev;
+  pw.cap = count;
+  return DRPT_OK;
+}
 
 // A fill kernel rather than hipMemsetAsync, so the sequence stays valid under
 // stream capture into a hipGraph.
