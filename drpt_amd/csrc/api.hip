@@ -202,6 +202,8 @@ int drpt_destroy(drpt_ctx* ctx) {
   if (ctx->ev_cm) (void)hipEventDestroy(ctx->ev_cm);
   if (ctx->pin_cm.host) (void)hipHostFree(ctx->pin_cm.host);
   if (ctx->pin_st.host) (void)hipHostFree(ctx->pin_st.host);
+  if (ctx->pin_up.host) (void)hipHostFree(ctx->pin_up.host);
+  if (ctx->ev_up) (void)hipEventDestroy(ctx->ev_up);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return DRPT_OK;
@@ -301,17 +303,33 @@ int drpt_project(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, cons
         wd.push_back(has ? (double)w[colptr[c] + k] : 0.0);
       }
   }
-  int32_t *dcp, *dri32, *dgk, *dgcol;
-  uint16_t* dri;
-  float* dw;
-  double* dwd;
-  DRPT_CHECK(upload(ctx, "proj.colptr", colptr, &dcp));
-  DRPT_CHECK(upload(ctx, "proj.rowidx", rowidx, &dri));
-  DRPT_CHECK(upload(ctx, "proj.w", w, &dw));
-  DRPT_CHECK(upload(ctx, "proj.gk", gk, &dgk));
-  DRPT_CHECK(upload(ctx, "proj.gcol", gcol, &dgcol));
-  DRPT_CHECK(upload(ctx, "proj.rowidx32", ridx32, &dri32));
-  DRPT_CHECK(upload(ctx, "proj.wd", wd, &dwd));
+  // the seven term arrays at 256-byte offsets in one pinned staging buffer,
+  // sent with one copy; the staging buffer is refilled only once the
+  // previous call's copy out of it has completed
+  const std::pair<const void*, size_t> parts[7] = {
+      {colptr.data(), colptr.size() * 4}, {rowidx.data(), rowidx.size() * 2},
+      {w.data(), w.size() * 4},           {gk.data(), gk.size() * 4},
+      {gcol.data(), gcol.size() * 4},     {ridx32.data(), ridx32.size() * 4},
+      {wd.data(), wd.size() * 8}};
+  size_t off[8] = {0};
+  for (int i = 0; i < 7; ++i) off[i + 1] = off[i] + ((std::max<size_t>(parts[i].second, 1) + 255) & ~(size_t)255);
+  if (ctx->ev_up) DRPT_HIP(hipEventSynchronize(ctx->ev_up));
+  else DRPT_HIP(hipEventCreateWithFlags(&ctx->ev_up, hipEventDisableTiming));
+  DRPT_CHECK(pinned_words(ctx, ctx->pin_up, (int)(off[7] / 4)));
+  unsigned char* stage = (unsigned char*)ctx->pin_up.host;
+  for (int i = 0; i < 7; ++i)
+    if (parts[i].second) std::memcpy(stage + off[i], parts[i].first, parts[i].second);
+  unsigned char* dev;
+  DRPT_CHECK(ws(ctx, "proj.terms", off[7], &dev));
+  DRPT_HIP(hipMemcpyAsync(dev, stage, off[7], hipMemcpyHostToDevice, ctx->stream));
+  DRPT_HIP(hipEventRecord(ctx->ev_up, ctx->stream));
+  const int32_t* dcp = (const int32_t*)(dev + off[0]);
+  const uint16_t* dri = (const uint16_t*)(dev + off[1]);
+  const float* dw = (const float*)(dev + off[2]);
+  const int32_t* dgk = (const int32_t*)(dev + off[3]);
+  const int32_t* dgcol = (const int32_t*)(dev + off[4]);
+  const int32_t* dri32 = (const int32_t*)(dev + off[5]);
+  const double* dwd = (const double*)(dev + off[6]);
   return launch_project(ctx, X, dtype, n, d, dcp, dri, dw, dgk, dgcol, dri32, dwd, L, colptr[L], P,
                         colmax);
 }

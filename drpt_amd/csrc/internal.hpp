@@ -70,6 +70,9 @@ struct drpt_ctx {
   // maxima, and the counters a low-precision build reads back at its end
   drpt::PinnedWords pin_cm, pin_st;
   hipEvent_t ev_cm = nullptr;
+  // staging of the projection's term arrays, and the event after its copy
+  drpt::PinnedWords pin_up;
+  hipEvent_t ev_up = nullptr;
   // environment DRPT_ROW_EXACT_BATCH=r: at most r rows per batch of the exact
   // row path (its scratch is otherwise capped by bytes)
   int64_t row_exact_batch = 0;
