@@ -372,7 +372,7 @@ static int group_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
   hipStream_t s = ctx->stream;
   // leaf sizes by heap index
   uint32_t *hcount, *cursor;
-  int64_t *hoff, *tree_items, *base, *range;
+  int64_t *hoff, *base, *range;
   int32_t* perm;
   LeafItem* items;
   unsigned long long* stats;
@@ -380,7 +380,6 @@ static int group_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
   DRPT_CHECK(ws(ctx, "group.hcount", T * HS, &hcount));
   DRPT_CHECK(ws(ctx, "group.cursor", T * HS, &cursor));
   DRPT_CHECK(ws(ctx, "group.hoff", T * HS, &hoff));
-  DRPT_CHECK(ws(ctx, "group.tree_items", T, &tree_items));
   DRPT_CHECK(ws(ctx, "group.base", T + 1, &base));
   DRPT_CHECK(ws(ctx, "group.range", 6, &range));
   DRPT_CHECK(ws(ctx, "group.stats", 4, &stats));
@@ -398,8 +397,8 @@ static int group_impl(drpt_ctx* ctx, const drpt_params* p, const void* X,
 
   if (p->local_mode == DRPT_LOCAL_LEAFMAX) {
     // everything on the device: offsets, work items, scatter
-    DRPT_CHECK(launch_device_items(ctx, hcount, T, D, plan, first_trees, hoff, tree_items,
-                                   base, range, items, stats));
+    DRPT_CHECK(launch_device_items(ctx, hcount, n, T, D, plan, first_trees, hoff, base, range,
+                                   items, stats));
     if (!items_only) DRPT_CHECK(launch_scatter_leaves(ctx, leaf_heap, n, T, D, hoff, cursor, perm));
   } else {
     // reference local stage: the host derives the position tables, so the

@@ -261,56 +261,129 @@ __device__ __forceinline__ void block_scan2(long long& a, long long& b, long lon
   __syncthreads();
 }
 
-// Per tree (one block): hoff[h] = first position of heap slot h's points in
-// perm[t] (slots in ascending order); tree totals of work items (blocks of
-// rpi rows) and leaves; sum of m^2 and max m over leaves.
-__global__ void __launch_bounds__(1024)
-k_leaf_scan(const uint32_t* __restrict__ hcount, int HS, ItemPlan plan, int64_t* __restrict__ hoff,
-            int64_t* __restrict__ tree_items, unsigned long long* __restrict__ stats) {
+// Leaf offsets and work items in three passes over chunks of kScanChunk heap
+// slots (one block per tree and chunk, then one block over the chunk totals),
+// so the slots of all trees are scanned by T * HS / kScanChunk blocks at once.
+constexpr int kScanChunk = 1024;
+
+// Per (tree blockIdx.y, chunk blockIdx.x): cs[0] = points, cs[1] = work items
+// (blocks of rpi rows), cs[2] = sum of m^2 over its leaves, cs[3] = leaves << 32
+// | largest m, at cs = csum + 4 (t NC + j).
+__global__ void __launch_bounds__(kScanChunk)
+k_chunk_sums(const uint32_t* __restrict__ hcount, int HS, ItemPlan plan,
+             int64_t* __restrict__ csum) {
   __shared__ long long sa[17], sb[17];
-  const int t = blockIdx.x;
-  const uint32_t* hc = hcount + (int64_t)t * HS;
-  int64_t* ho = hoff + (int64_t)t * HS;
-  long long cp = 0, ci = 0, leaves = 0;
-  unsigned long long m2 = 0, mx = 0;
-  for (int base = 0; base < HS; base += blockDim.x) {
-    const int h = base + threadIdx.x;
-    const long long c = h < HS ? (long long)hc[h] : 0;
-    const int rpi = plan.rpi[size_bucket((int)min(c, (long long)INT_MAX))];
-    long long a = c, b = (c + rpi - 1) / rpi, ta, tb;
-    block_scan2(a, b, ta, tb, sa, sb);
-    if (h < HS) ho[h] = cp + a;
-    cp += ta;
-    ci += tb;
-    if (c > 0) {
-      ++leaves;
-      m2 += (unsigned long long)(c * c);
-      mx = max(mx, (unsigned long long)c);
-    }
+  __shared__ unsigned int smax;
+  const int t = blockIdx.y, j = blockIdx.x, NC = gridDim.x;
+  const int h = j * kScanChunk + threadIdx.x;
+  if (threadIdx.x == 0) smax = 0;
+  const long long c = h < HS ? (long long)hcount[(int64_t)t * HS + h] : 0;
+  const int rpi = plan.rpi[size_bucket((int)min(c, (long long)INT_MAX))];
+  long long a = c, b = (c + rpi - 1) / rpi, ta, tb;
+  block_scan2(a, b, ta, tb, sa, sb);
+  long long m2 = c * c, lv = c > 0 ? 1 : 0, tm2, tlv;
+  block_scan2(m2, lv, tm2, tlv, sa, sb);
+  unsigned int mx = (unsigned int)c;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = max(mx, (unsigned int)__shfl_xor((int)mx, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(&smax, mx);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t* cs = csum + 4 * ((int64_t)t * NC + j);
+    cs[0] = ta;
+    cs[1] = tb;
+    cs[2] = tm2;
+    cs[3] = (int64_t)(((unsigned long long)tlv << 32) | smax);
   }
-  if (threadIdx.x == 0) tree_items[t] = ci;
-  atomicAdd(&stats[0], m2);
-  atomicMax(&stats[1], mx);
-  atomicAdd(&stats[2], (unsigned long long)leaves);
 }
 
-// Item ranges: base[t] = sum of items of trees < t; range[] = {0, base[a],
-// base[a], base[T]} (trees < a, then the others) and {0, base[T]} (all).
-__global__ void k_tree_bases(const int64_t* __restrict__ tree_items, int T, int a,
-                             int64_t* __restrict__ base, int64_t* __restrict__ range) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  long long acc = 0;
-  for (int t = 0; t < T; ++t) {
-    base[t] = acc;
-    acc += tree_items[t];
+// One block over the T * NC chunk totals in tree-major order: cs[0] becomes
+// the chunk's first position within its tree's perm (each tree's counts sum
+// to n), cs[1] its first work item (all trees); base[t] = first item of tree
+// t, base[T] = all items; range = {0, base[a], base[a], base[T], 0, base[T]};
+// stats = {sum of m^2, largest m, leaves}.
+__global__ void __launch_bounds__(1024)
+k_chunk_bases(int64_t* __restrict__ csum, int T, int NC, int64_t n, int a_first,
+              int64_t* __restrict__ base, int64_t* __restrict__ range,
+              unsigned long long* __restrict__ stats) {
+  __shared__ long long sa[17], sb[17];
+  __shared__ unsigned int smax;
+  if (threadIdx.x == 0) smax = 0;
+  const int64_t total = (int64_t)T * NC;
+  long long cp = 0, ci = 0, m2 = 0, lv = 0;
+  unsigned int mx = 0;
+  for (int64_t w = 0; w < total; w += blockDim.x) {
+    const int64_t k = w + threadIdx.x;
+    long long a = 0, b = 0, ta, tb;
+    if (k < total) {
+      const int64_t* cs = csum + 4 * k;
+      a = cs[0];
+      b = cs[1];
+      m2 += cs[2];
+      const unsigned long long pk = (unsigned long long)cs[3];
+      lv += (long long)(pk >> 32);
+      mx = max(mx, (unsigned int)pk);
+    }
+    block_scan2(a, b, ta, tb, sa, sb);
+    if (k < total) {
+      const int t = (int)(k / NC);
+      csum[4 * k] = cp + a - (int64_t)t * n;
+      csum[4 * k + 1] = ci + b;
+      if (k % NC == 0) base[t] = ci + b;
+    }
+    cp += ta;
+    ci += tb;
   }
-  base[T] = acc;
-  range[0] = 0;
-  range[1] = base[a];
-  range[2] = base[a];
-  range[3] = acc;
-  range[4] = 0;
-  range[5] = acc;
+  long long tm2, tlv;
+  block_scan2(m2, lv, tm2, tlv, sa, sb);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = max(mx, (unsigned int)__shfl_xor((int)mx, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(&smax, mx);
+  __syncthreads();  // base[] of every tree written (block scope) and smax complete
+  if (threadIdx.x == 0) {
+    base[T] = ci;
+    const long long ba = a_first < T ? base[a_first] : ci;
+    range[0] = 0;
+    range[1] = ba;
+    range[2] = ba;
+    range[3] = ci;
+    range[4] = 0;
+    range[5] = ci;
+    stats[0] = (unsigned long long)tm2;
+    stats[1] = smax;
+    stats[2] = (unsigned long long)tlv;
+  }
+}
+
+// Per (tree, chunk): hoff[h] = first position of heap slot h's points in
+// perm[t], and the work items (blocks of plan.rpi[bucket] rows) of every
+// non-empty slot, in slot order from the chunk's first item.
+__global__ void __launch_bounds__(kScanChunk)
+k_chunk_emit(const uint32_t* __restrict__ hcount, int HS, ItemPlan plan,
+             const int64_t* __restrict__ csum, int64_t* __restrict__ hoff,
+             LeafItem* __restrict__ items) {
+  __shared__ long long sa[17], sb[17];
+  const int t = blockIdx.y, j = blockIdx.x, NC = gridDim.x;
+  const int h = j * kScanChunk + threadIdx.x;
+  const long long c = h < HS ? (long long)hcount[(int64_t)t * HS + h] : 0;
+  const int rpi = plan.rpi[size_bucket((int)min(c, (long long)INT_MAX))];
+  long long a = c, b = (c + rpi - 1) / rpi, ta, tb;
+  block_scan2(a, b, ta, tb, sa, sb);
+  const int64_t* cs = csum + 4 * ((int64_t)t * NC + j);
+  const int64_t start = cs[0] + a;
+  if (h < HS) hoff[(int64_t)t * HS + h] = start;
+  if (c > 0) {
+    LeafItem it;
+    it.tree = t;
+    it.size = (int32_t)c;
+    it.start = start;
+    const int64_t i0 = cs[1] + b;
+    for (long long r0 = 0, k = 0; r0 < c; r0 += rpi, ++k) {
+      it.row0 = (int32_t)r0;
+      it.rows = (int32_t)min((long long)rpi, c - r0);
+      items[i0 + k] = it;
+    }
+  }
 }
 
 __global__ void k_tree_range(const int64_t* __restrict__ base, int t0, int t1,
@@ -318,37 +391,6 @@ __global__ void k_tree_range(const int64_t* __restrict__ base, int t0, int t1,
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     range[0] = base[t0];
     range[1] = base[t1];
-  }
-}
-
-// Per tree (one block): the work items (blocks of plan.rpi[bucket] rows) of
-// every non-empty slot.
-__global__ void __launch_bounds__(1024)
-k_emit_items(const uint32_t* __restrict__ hcount, const int64_t* __restrict__ hoff, int HS,
-             ItemPlan plan, const int64_t* __restrict__ base, LeafItem* __restrict__ items) {
-  __shared__ long long sa[17], sb[17];
-  const int t = blockIdx.x;
-  const uint32_t* hc = hcount + (int64_t)t * HS;
-  const int64_t* ho = hoff + (int64_t)t * HS;
-  long long ci = base[t];
-  for (int b0 = 0; b0 < HS; b0 += blockDim.x) {
-    const int h = b0 + threadIdx.x;
-    const long long c = h < HS ? (long long)hc[h] : 0;
-    const int rpi = plan.rpi[size_bucket((int)min(c, (long long)INT_MAX))];
-    long long a = (c + rpi - 1) / rpi, z = 0, ta, tz;
-    block_scan2(a, z, ta, tz, sa, sb);
-    if (c > 0) {
-      LeafItem it;
-      it.tree = t;
-      it.size = (int32_t)c;
-      it.start = ho[h];
-      for (long long r0 = 0, k = 0; r0 < c; r0 += rpi, ++k) {
-        it.row0 = (int32_t)r0;
-        it.rows = (int32_t)min((long long)rpi, c - r0);
-        items[ci + a + k] = it;
-      }
-    }
-    ci += ta;
   }
 }
 
@@ -510,8 +552,8 @@ int launch_apply_local(drpt_ctx* ctx, const int32_t* perm_in, int32_t* perm_out,
 
 namespace drpt {
 
-int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, const ItemPlan& plan,
-                        int first_trees, int64_t* hoff, int64_t* tree_items, int64_t* base,
+int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int64_t n, int T, int D,
+                        const ItemPlan& plan, int first_trees, int64_t* hoff, int64_t* base,
                         int64_t* range, LeafItem* items, unsigned long long* stats) {
   const int HS = 1 << D;
   bool ok = first_trees >= 0 && first_trees <= T;
@@ -520,14 +562,17 @@ int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, con
     set_error("internal: invalid work item layout");
     return DRPT_ERR_ARG;
   }
-  hipLaunchKernelGGL(k_leaf_scan, dim3(T), dim3(1024), 0, ctx->stream, hcount, HS, plan, hoff,
-                     tree_items, stats);
+  const int NC = (HS + kScanChunk - 1) / kScanChunk;
+  int64_t* csum;
+  DRPT_CHECK(ws(ctx, "group.chunks", 4 * (size_t)T * NC, &csum));
+  hipLaunchKernelGGL(k_chunk_sums, dim3(NC, T), dim3(kScanChunk), 0, ctx->stream, hcount, HS, plan,
+                     csum);
   DRPT_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_tree_bases, dim3(1), dim3(64), 0, ctx->stream, tree_items, T, first_trees,
-                     base, range);
+  hipLaunchKernelGGL(k_chunk_bases, dim3(1), dim3(1024), 0, ctx->stream, csum, T, NC, n,
+                     first_trees, base, range, stats);
   DRPT_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_emit_items, dim3(T), dim3(1024), 0, ctx->stream, hcount, hoff, HS, plan,
-                     base, items);
+  hipLaunchKernelGGL(k_chunk_emit, dim3(NC, T), dim3(kScanChunk), 0, ctx->stream, hcount, HS, plan,
+                     csum, hoff, items);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
 }

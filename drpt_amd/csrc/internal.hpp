@@ -190,12 +190,12 @@ inline ItemPlan uniform_plan(int rpi) {
 int launch_count_leaves(drpt_ctx* ctx, const uint32_t* heap, int64_t n, int T, int D, uint32_t* hcount);
 int launch_scatter_leaves(drpt_ctx* ctx, const uint32_t* heap, int64_t n, int T, int D,
                           const int64_t* hoff, uint32_t* cursor, int32_t* perm);
-// device-side grouping for leaf-max builds: heap offsets, per-tree item
-// counts, item ranges (trees < first_trees / the others / all) and the work
-// items (blocks of plan.rpi[bucket] rows); stats[0] += sum m^2, stats[1] = max
-// m, stats[2] += leaves
-int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int T, int D, const ItemPlan& plan,
-                        int first_trees, int64_t* hoff, int64_t* tree_items, int64_t* base,
+// device-side grouping for leaf-max builds (hcount: n points per tree): heap
+// offsets, first work item of each tree (base[T + 1]), item ranges (trees <
+// first_trees / the others / all) and the work items (blocks of
+// plan.rpi[bucket] rows); stats = {sum m^2, max m, leaves}
+int launch_device_items(drpt_ctx* ctx, const uint32_t* hcount, int64_t n, int T, int D,
+                        const ItemPlan& plan, int first_trees, int64_t* hoff, int64_t* base,
                         int64_t* range, LeafItem* items, unsigned long long* stats);
 // Xp[i] = X[perm[i]] (rows of rowbytes bytes) and pos[perm[i]] = i, i < n
 int launch_permute_rows(drpt_ctx* ctx, const void* X, size_t rowbytes, const int32_t* perm,

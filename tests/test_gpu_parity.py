@@ -219,7 +219,34 @@ def test_tree_knn_lowp_bitexact(ctx, orc, prec, case, K, leaf_max, kprime):
     ci, cd, tm = ctx.tree_knn(p, _dev(X), _dev(heaps.view(np.int32)))
     ci, cd = ci.cpu().numpy(), cd.cpu().numpy()
     _log(f"  fallback rows {tm.n_fallback_rows} of {2 * n}, largest leaf {tm.max_leaf}")
+    sizes = np.concatenate([np.unique(heaps[t], return_counts=True)[1] for t in range(2)])
+    assert (tm.n_leaves, tm.max_leaf) == (len(sizes), sizes.max())
+    assert tm.gram_flops == 2.0 * d * float((sizes.astype(np.int64) ** 2).sum())
     for t in range(2):
+        perm, offs = _groups(heaps[t])
+        ri, rd = orc.leaf_knn(X, perm, offs, K)
+        assert np.array_equal(ci[t], ri)
+        assert np.array_equal(_bits(cd[t]), _bits(rd))
+
+
+def test_tree_knn_many_heap_slots(ctx, orc):
+    """Depth 18 with five trees: 2^18 heap slots per tree, so the leaf offsets
+    and work items are scanned over 1280 chunks of 1024 slots (two windows of
+    the chunk-total scan); lists and leaf statistics equal the oracle's."""
+    _log("[tree_knn bf16] int128_f32_4000 depth 18, T=5")
+    X = CASES["int128_f32_4000"]()
+    n, d = X.shape
+    T, K = 5, 10
+    p = drpt_amd.leafmax_params(n, d, T, K, leaf_max=64, precision="bf16", seed=5, depth=18)
+    p.dtype = drpt_amd.DRPT_F32
+    R = orc.build_R(d, T * p.depth, p.density, p.seed)
+    P0, cm0 = orc.project(X, R)
+    heaps = np.stack([orc.grow_tree(P0, cm0, p.depth, t, 1, p.leaf_max, 0) for t in range(T)])
+    ci, cd, tm = ctx.tree_knn(p, _dev(X), _dev(heaps.view(np.int32)))
+    ci, cd = ci.cpu().numpy(), cd.cpu().numpy()
+    sizes = np.concatenate([np.unique(heaps[t], return_counts=True)[1] for t in range(T)])
+    assert (tm.n_leaves, tm.max_leaf) == (len(sizes), sizes.max())
+    for t in range(T):
         perm, offs = _groups(heaps[t])
         ri, rd = orc.leaf_knn(X, perm, offs, K)
         assert np.array_equal(ci[t], ri)
