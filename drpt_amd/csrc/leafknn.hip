@@ -2277,6 +2277,70 @@ __global__ void k_prep_bf16(const XT* __restrict__ X, int64_t n, int d, uint16_t
   if (Xu != nullptr && __ballot(bad) != 0 && lane == 0) atomicOr(notu8, 1);
 }
 
+// The same per-row results for float rows of d % 4 == 0 values read as float4
+// chunks: G lanes per row (a power of two >= d / 4, at most 64), 64 / G rows
+// per wave, the waves striding over groups of rows; each lane stores its four
+// bf16 values as one 8-byte word and its four int8 values as one 4-byte word
+// (where a value is not an integer in [0, 255] that word is never read: the
+// flag is set).
+template <int G>
+__global__ void k_prep_bf16_v4(const float* __restrict__ X, int64_t n, int d,
+                               uint16_t* __restrict__ Xb, float2* __restrict__ nbe,
+                               uint8_t* __restrict__ Xu, int32_t* __restrict__ notu8) {
+  constexpr int RPW = 64 / G;
+  const int lane = threadIdx.x & 63;
+  const int sub = lane & (G - 1), rw = lane / G;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int q = d >> 2;
+  const bool wide = d >= 512;
+  bool bad = false;
+  for (int64_t r0 = wave * RPW; r0 < n; r0 += nwaves * RPW) {
+    const int64_t row = r0 + rw;
+    float s = 0.0f;
+    double s64 = 0.0, e2 = 0.0;
+    if (row < n) {
+      const float4* xr = reinterpret_cast<const float4*>(X + row * d);
+      uint2* br = reinterpret_cast<uint2*>(Xb + row * d);
+      uint32_t* ur = Xu != nullptr ? reinterpret_cast<uint32_t*>(Xu + row * d) : nullptr;
+      for (int k = sub; k < q; k += G) {
+        const float4 x4 = xr[k];
+        const float xs[4] = {x4.x, x4.y, x4.z, x4.w};
+        uint32_t bw[4], u = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float x = xs[e];
+          uint32_t bits = __float_as_uint(x);
+          bits += 0x7fffu + ((bits >> 16) & 1u);
+          bw[e] = bits >> 16;
+          if (x >= 0.0f && x <= 255.0f && x == rintf(x)) u |= ((uint32_t)(uint8_t)x ^ 0x80u) << (8 * e);
+          else bad = true;
+          const float xb = __uint_as_float(bw[e] << 16);
+          if (wide) s64 += (double)xb * (double)xb;
+          else s = fmaf(xb, xb, s);
+          const double df = (double)xb - (double)x;
+          e2 += df * df;
+        }
+        br[k] = make_uint2(bw[0] | (bw[1] << 16), bw[2] | (bw[3] << 16));
+        if (ur != nullptr) ur[k] = u;
+      }
+    }
+    // sums over the G lanes of each row (xor offsets below G stay in the row)
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+      if (wide) s64 += __shfl_xor(s64, off);
+      else s += __shfl_xor(s, off);
+      e2 += __shfl_xor(e2, off);
+    }
+    if (sub == 0 && row < n) {
+      float ef = (float)sqrt(e2);
+      if (e2 > 0.0) ef = ef * (1.0f + 1.0e-6f) + 1.0e-30f;
+      nbe[row] = make_float2(wide ? (float)s64 : s, ef);
+    }
+  }
+  if (Xu != nullptr && __ballot(bad) != 0 && lane == 0) atomicOr(notu8, 1);
+}
+
 // FP8 quantisation, one wave per row: scale 2^e with e the smallest exponent
 // for which max |x| / 2^e <= 448 (the e4m3 maximum, so no element saturates),
 // clamped to [-120, 120]; q = e4m3(x / 2^e) rounded to nearest by the
@@ -2432,7 +2496,18 @@ int launch_prep_lowp(drpt_ctx* ctx, const void* X, int dtype, int64_t n, int d, 
   } else if (dtype == DRPT_U8)
     hipLaunchKernelGGL(k_prep_bf16<uint8_t>, dim3(sblocks), dim3(256), 0, ctx->stream,
                        (const uint8_t*)X, n, d, (uint16_t*)Xq, nbe, nullptr, nullptr);
-  else
+  else if (d % 4 == 0 && ((uintptr_t)X & 15) == 0) {
+    // float4 chunks: G lanes per row
+    uint8_t* xu = notu8 ? xi8 : nullptr;
+    const int q = d / 4;
+    const int G = q <= 8 ? 8 : q <= 16 ? 16 : q <= 32 ? 32 : 64;
+    const int64_t waves = (n * G + 63) / 64;
+    const unsigned vblocks = (unsigned)std::min<int64_t>((waves + 3) / 4, (int64_t)ctx->num_cus * 8);
+    auto* f = G == 8 ? k_prep_bf16_v4<8> : G == 16 ? k_prep_bf16_v4<16>
+            : G == 32 ? k_prep_bf16_v4<32> : k_prep_bf16_v4<64>;
+    hipLaunchKernelGGL(f, dim3(vblocks), dim3(256), 0, ctx->stream, (const float*)X, n, d,
+                       (uint16_t*)Xq, nbe, xu, notu8);
+  } else
     hipLaunchKernelGGL(k_prep_bf16<float>, dim3(sblocks), dim3(256), 0, ctx->stream,
                        (const float*)X, n, d, (uint16_t*)Xq, nbe, notu8 ? xi8 : nullptr, notu8);
   DRPT_HIP(hipGetLastError());
