@@ -1308,13 +1308,33 @@ __global__ void k_lossy_flag(const float2* __restrict__ nbe, int64_t n, int32_t*
   if (__ballot(any) != 0 && (threadIdx.x & 63) == 0) atomicOr(lossy, 1);
 }
 
-// leaf-ordered gather of one value per point: outT[i] = src[perm[i] * stride + offset]
-__global__ void k_leaf_gather_val(const int32_t* __restrict__ perm, int64_t total,
-                                  const float* __restrict__ src, int stride, int offset,
-                                  float* __restrict__ outT) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x)
-    outT[i] = src[(int64_t)perm[i] * stride + offset];
+// leaf-ordered gather of one value per point: outT[i] = src[perm[i] * stride + offset];
+// each thread loads four perm entries, then their four values, before storing
+// (four independent gathers in flight instead of one dependent pair)
+constexpr int kGatherUnroll = 4;
+__global__ void __launch_bounds__(256)
+k_leaf_gather_val(const int32_t* __restrict__ perm, int64_t total, const float* __restrict__ src,
+                  int stride, int offset, float* __restrict__ outT) {
+  const int64_t span = (int64_t)blockDim.x * kGatherUnroll;
+  for (int64_t base = (int64_t)blockIdx.x * span; base < total; base += (int64_t)gridDim.x * span) {
+    int32_t pi[kGatherUnroll];
+    float v[kGatherUnroll];
+#pragma unroll
+    for (int u = 0; u < kGatherUnroll; ++u) {
+      const int64_t i = base + u * blockDim.x + threadIdx.x;
+      pi[u] = i < total ? perm[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kGatherUnroll; ++u) {
+      const int64_t i = base + u * blockDim.x + threadIdx.x;
+      v[u] = i < total ? src[(int64_t)pi[u] * stride + offset] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kGatherUnroll; ++u) {
+      const int64_t i = base + u * blockDim.x + threadIdx.x;
+      if (i < total) outT[i] = v[u];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2569,7 +2589,9 @@ int launch_lowp_knn(drpt_ctx* ctx, const drpt_params* p, int mode, KnnArgs a) {
 
 int launch_leaf_gather_val(drpt_ctx* ctx, const int32_t* perm, int64_t total, const float* src,
                            int stride, int offset, float* outT) {
-  const int64_t blocks = std::min<int64_t>((total + 255) / 256, (int64_t)ctx->num_cus * 16);
+  const int64_t blocks =
+      std::min<int64_t>((total + 256 * kGatherUnroll - 1) / (256 * kGatherUnroll),
+                        (int64_t)ctx->num_cus * 16);
   hipLaunchKernelGGL(k_leaf_gather_val, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0,
                      ctx->stream, perm, total, src, stride, offset, outT);
   DRPT_HIP(hipGetLastError());

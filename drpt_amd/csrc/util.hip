@@ -10,10 +10,12 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t count) {
   for (; i < count; i += stride) p[i] = v;
 }
 
-// acc += sum of p[0 .. count): per-thread sums, a wave reduction and one
-// atomic per wave
-__global__ void k_sum_u32(const uint32_t* __restrict__ p, size_t count,
-                          unsigned long long* __restrict__ acc) {
+// acc += sum of p[0 .. count): per-thread sums, wave reductions, the block's
+// waves summed in LDS and one atomic per block (a few hundred on the one
+// address instead of one per wave)
+__global__ void __launch_bounds__(256)
+k_sum_u32(const uint32_t* __restrict__ p, size_t count, unsigned long long* __restrict__ acc) {
+  __shared__ unsigned long long ws[4];
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   unsigned long long s = 0;
@@ -24,7 +26,12 @@ __global__ void k_sum_u32(const uint32_t* __restrict__ p, size_t count,
     const unsigned hi = (unsigned)__shfl_xor((int)(unsigned)(s >> 32), off);
     s += ((unsigned long long)hi << 32) | lo;
   }
-  if ((threadIdx.x & 63) == 0 && s != 0) atomicAdd(acc, s);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = ws[0] + ws[1] + ws[2] + ws[3];
+    if (t != 0) atomicAdd(acc, t);
+  }
 }
 
 // Up to four small device word arrays, one after another, into pinned host
@@ -85,8 +92,8 @@ int fill_u32(drpt_ctx* ctx, uint32_t* p, uint32_t v, size_t count) {
 
 int sum_u32(drpt_ctx* ctx, const uint32_t* p, size_t count, unsigned long long* acc) {
   if (count == 0) return DRPT_OK;
-  size_t blocks = (count + 1023) / 1024;
-  if (blocks > 1024) blocks = 1024;
+  size_t blocks = (count + 4095) / 4096;
+  if (blocks > (size_t)ctx->num_cus * 2) blocks = (size_t)ctx->num_cus * 2;
   hipLaunchKernelGGL(k_sum_u32, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, p, count, acc);
   DRPT_HIP(hipGetLastError());
   return DRPT_OK;
