@@ -885,9 +885,8 @@ static int lowp_setup(drpt_ctx* ctx, const drpt_params* p, const void* X,
   DRPT_CHECK(ws(ctx, "knn.ovf_flag", n, &r->oflag));
   DRPT_CHECK(ws(ctx, "knn.ovf_list", n, &r->olist));
   DRPT_CHECK(ws(ctx, "knn.novf", 1, &r->novf));
-  DRPT_CHECK(fill_u32(ctx, (uint32_t*)r->pcnt, 0u, n));
-  DRPT_CHECK(fill_u32(ctx, (uint32_t*)r->oflag, 0u, n));
-  DRPT_CHECK(fill_u32(ctx, (uint32_t*)r->novf, 0u, 1));
+  DRPT_CHECK(fill3_u32(ctx, (uint32_t*)r->pcnt, 0u, n, (uint32_t*)r->oflag, 0u, n,
+                       (uint32_t*)r->novf, 0u, 1));
   KnnArgs& a = r->a;
   a.items = r->G.items;
   a.out_idx = r->pool;
@@ -967,8 +966,9 @@ static int lowp_known(drpt_ctx* ctx, LowpRun* r, const void* X, const ItemPlan& 
     DRPT_CHECK(ws(ctx, r->pc.precision == DRPT_FP8 ? "lowp8.nbe" : "lowp.nbe", (size_t)r->n, &nbe));
     DRPT_CHECK(launch_row_bound(ctx, r->s1, r->n * r->KP, r->KP, r->first, r->n, r->pp.d, r->K,
                                 nbe, r->bnd));
-    DRPT_CHECK(fill_u32(ctx, (uint32_t*)r->ki, 0xFFFFFFFFu, (size_t)r->n * r->K));  // id -1
-    DRPT_CHECK(fill_u32(ctx, (uint32_t*)r->kd, 0x7F800000u, (size_t)r->n * r->K));  // +inf
+    // ids -1, distances +inf
+    DRPT_CHECK(fill3_u32(ctx, (uint32_t*)r->ki, 0xFFFFFFFFu, (size_t)r->n * r->K,
+                         (uint32_t*)r->kd, 0x7F800000u, (size_t)r->n * r->K));
     DRPT_HIP(hipEventRecord(ctx->evp[2], ctx->stream));
     return DRPT_OK;
   }

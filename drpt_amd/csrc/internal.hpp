@@ -103,6 +103,9 @@ int host_local_depth(int64_t gmin, int nn, int offset);
 // ---- launchers (one per .hip unit) ------------------------------------------
 // util.hip: capture-safe fills (a kernel, not hipMemsetAsync)
 int fill_u32(drpt_ctx* ctx, uint32_t* p, uint32_t v, size_t count);
+// fills of three word ranges (p_i[0 .. c_i) = v_i) in one launch (capture-safe)
+int fill3_u32(drpt_ctx* ctx, uint32_t* p0, uint32_t v0, size_t c0, uint32_t* p1, uint32_t v1,
+              size_t c1, uint32_t* p2 = nullptr, uint32_t v2 = 0, size_t c2 = 0);
 // pw holds at least count words
 int pinned_words(drpt_ctx* ctx, PinnedWords& pw, int count);
 // s0[0..c0), s1[0..c1), ... into dst (a PinnedWords dev pointer), on the

@@ -1,4 +1,6 @@
 // Small device utilities.
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace drpt {
@@ -34,6 +36,18 @@ k_sum_u32(const uint32_t* __restrict__ p, size_t count, unsigned long long* __re
   }
 }
 
+// Up to three fills of 32-bit words with their own values in one launch.
+__global__ void k_fill3_u32(uint32_t* __restrict__ p0, uint32_t v0, size_t c0,
+                            uint32_t* __restrict__ p1, uint32_t v1, size_t c1,
+                            uint32_t* __restrict__ p2, uint32_t v2, size_t c2) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < c0 + c1 + c2; i += stride) {
+    if (i < c0) p0[i] = v0;
+    else if (i < c0 + c1) p1[i - c0] = v1;
+    else p2[i - c0 - c1] = v2;
+  }
+}
+
 // Up to four small device word arrays, one after another, into pinned host
 // memory by plain stores from one block: no copy engine or staging buffer on
 // the stream's path; the host reads dst after a later synchronisation.
@@ -51,6 +65,17 @@ __global__ void k_words_to_host(uint32_t* __restrict__ dst, const uint32_t* __re
   }
 }
 }  // namespace
+
+int fill3_u32(drpt_ctx* ctx, uint32_t* p0, uint32_t v0, size_t c0, uint32_t* p1, uint32_t v1,
+              size_t c1, uint32_t* p2, uint32_t v2, size_t c2) {
+  const size_t total = c0 + c1 + c2;
+  if (total == 0) return DRPT_OK;
+  const size_t blocks = std::min<size_t>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill3_u32, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, p0, v0, c0, p1,
+                     v1, c1, p2, v2, c2);
+  DRPT_HIP(hipGetLastError());
+  return DRPT_OK;
+}
 
 int words_to_host(drpt_ctx* ctx, uint32_t* dst, const uint32_t* s0, int c0, const uint32_t* s1,
                   int c1, const uint32_t* s2, int c2, const uint32_t* s3, int c3) {
