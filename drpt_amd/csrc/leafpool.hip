@@ -671,15 +671,17 @@ const void* pool_leaf_instance(int mode, int tk, int ks) {
 }
 
 // one launch of instance fn over the items of b with sizes [b.size_lo,
-// b.size_hi], mmax staged rows, `lds` bytes of dynamic LDS, `per_cu` blocks per
-// CU; q: kPlQueues zeroed work-queue counters of its own
+// b.size_hi], mmax staged rows, `lds` bytes of dynamic LDS, `per_cu` blocks per CU
 int launch_pool_leaf_one(drpt_ctx* ctx, const void* fn, KnnArgs b, int mmax, size_t lds, int per_cu,
-                         int threads, unsigned* q) {
+                         int threads) {
   auto it = ctx->fn_lds.find(fn);
   if (it == ctx->fn_lds.end() || it->second < (int)lds) {
     DRPT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     ctx->fn_lds[fn] = (int)lds;
   }
+  unsigned* q;
+  DRPT_CHECK(ws(ctx, "knn.queue_pl", kPlQueues, &q));
+  DRPT_CHECK(fill_u32(ctx, q, 0u, kPlQueues));
   b.queue = q;
   void* args[] = {(void*)&b, (void*)&mmax};
   DRPT_HIP(hipLaunchKernel(fn, dim3((unsigned)(ctx->num_cus * per_cu)), dim3(threads), args, lds,
@@ -726,12 +728,8 @@ int launch_pool_leaf(drpt_ctx* ctx, int mode, int kp, const KnnArgs& a0) {
   KnnArgs a = a0;
   if (!i8) a.notu8 = nullptr;
   const size_t lds = (size_t)mmax * pool_leaf_row_bytes(mode) + kPlMisc * 4;
-  // the work-queue counters of up to three launches, cleared together
-  unsigned* q;
-  DRPT_CHECK(ws(ctx, "knn.queue_pl", 3 * (size_t)kPlQueues, &q));
-  DRPT_CHECK(fill_u32(ctx, q, 0u, 3 * (size_t)kPlQueues));
   DRPT_CHECK(launch_pool_leaf_one(ctx, pool_leaf_instance<false>(mode, tk, a0.d / 16), a, mmax, lds, 1,
-                                  kPlThreads, q));
+                                  kPlThreads));
   if (!i8) return DRPT_OK;
   const void* fn8 = pool_leaf_instance<true>(mode, tk, a0.d / 32);
   // two blocks per CU for the leaves whose rows fit half the LDS, one for the rest
@@ -740,7 +738,7 @@ int launch_pool_leaf(drpt_ctx* ctx, int mode, int kp, const KnnArgs& a0) {
     KnnArgs b = a;
     b.size_hi = std::min(a0.size_hi, m2);
     DRPT_CHECK(launch_pool_leaf_one(ctx, fn8, b, m2, (size_t)m2 * pool_leaf_row_bytes8() + kPlMisc * 4, 2,
-                                    kPlThreads8, q + kPlQueues));
+                                    kPlThreads8));
     if (a0.size_hi <= m2) return DRPT_OK;
     a.size_lo = m2 + 1;
   }
@@ -757,7 +755,7 @@ int launch_pool_leaf(drpt_ctx* ctx, int mode, int kp, const KnnArgs& a0) {
     a.range = big_range;
   }
   return launch_pool_leaf_one(ctx, fn8, a, mmax, (size_t)mmax * pool_leaf_row_bytes8() + kPlMisc * 4, 1,
-                              kPlThreads8, q + 2 * kPlQueues);
+                              kPlThreads8);
 }
 
 }  // namespace drpt
